@@ -1,0 +1,254 @@
+"""Forward-rasterizer benchmark on one MI355X.
+
+Workload: the mixed-regime scene -- 6M synthetic Gaussians (log-scales from a
+90/9/1 mixture of N(-5,0.4), N(-3,0.4), N(-1,0.3)) in a 50^3 box, rendered at
+3840x2160 with 16x16 tiles.  One step is one frame: preprocess -> radix sort
+-> tile ranges -> render, with the scene already resident in HBM.
+
+Reported value: ms/frame of the FP16-feature persistent renderer (FP16 SH,
+24-byte FP16 feature records, FP32 projection and blending).  Reported
+beside it: the plain FP32 path (FP32 SH and features, one workgroup per
+tile), per-stage times with HBM roofline fractions, PSNR of both images
+against the FP64 evaluation, and the CPU oracle timed on the host cores.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "ms/frame + HBM GB/s @4K/6M Gaussians FP16 vs FP32; p99 latency @60Hz stream"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c3")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the rasterizer runs each frame on a single device)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import polysplat_amd as polysplat
+    from polysplat_amd import scenes
+    from polysplat_amd.metrics import diff_metrics
+
+    ops = polysplat.ops
+    dev = torch.device("cuda:0")
+    cfg = scenes.CONFIGS[args.config]
+    spec = scenes.config_camera(args.config)
+    W, H = spec.width, spec.height
+    T = ((W + 15) // 16) * ((H + 15) // 16)
+    t0 = time.time()
+    arrays = scenes.make_config_scene(args.config, seed=1, device=dev)
+    torch.cuda.synchronize()
+    log(f"scene {args.config}: P={arrays.num_vertex} {W}x{H} generated in {time.time() - t0:.1f}s")
+    scene = polysplat.Scene.from_arrays(arrays, dev)
+    P = scene.num_vertex
+    cap = 6 * P
+    bg = torch.zeros(3)
+
+    def make_rast(feat):
+        return polysplat.Rasterizer(scene, MAX_NUM_RENDERED=cap, MAX_NUM_TILES=T,
+                                    feature_dtype=feat)
+
+    paths = {
+        "fp16_persistent": (make_rast("fp16"), "preranges_smem_persistent_lite_e2e_preranges"),
+        "fp32_plain": (make_rast("fp32"), "default"),
+    }
+    VAR = polysplat.rasterizer.VARIANT_IDS
+    PREC = {"fp16": 1, "fp32": 0, "bf16": 2}
+
+    def frame_fn(rast, variant):
+        fp32 = variant == "default"
+        prec = 0 if fp32 else PREC[rast.feature_dtype]
+        shs = scene.shs if fp32 else scene.shs_half
+        v = VAR[variant]
+        rng = rast.tile_ranges_buf if v == 1 else rast._empty
+        rec = rast.records if prec else rast._empty
+
+        def run(sync=False):
+            return ops.forward_frame(scene.position, shs, scene.opacity, scene.cov3d, W, H,
+                                     spec.position, spec.rotation, spec.focal_x, spec.focal_y,
+                                     spec.zFar, spec.zNear, v, prec, rast.curr_offset,
+                                     rast.list_sorting_space, rast.gaussian_keys_unsorted,
+                                     rast.gaussian_values_unsorted, rast.gaussian_keys_sorted,
+                                     rast.gaussian_values_sorted, rast.points_xy, rast.rgb_depth,
+                                     rast.conic_opacity, rec, rng, bg, rast._get_out_color(H, W, dev),
+                                     rast.workspace, sync)
+        return run
+
+    results = {}
+    images = {}
+    for name, (rast, variant) in paths.items():
+        run = frame_fn(rast, variant)
+        M = run(sync=True)
+        for _ in range(args.warmup):
+            run()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            run()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3 / args.steps
+        images[name] = rast._get_out_color(H, W, dev).cpu().numpy().copy()
+        results[name] = {"ms_per_frame": ms, "num_rendered": int(M)}
+        log(f"{name}: {ms:.3f} ms/frame, M={M}")
+
+    # ---------------------------------------------------------- per-stage timing
+    rast, _ = paths["fp16_persistent"]
+    M = results["fp16_persistent"]["num_rendered"]
+    shs = scene.shs_half
+    cam_args = (W, H, 16, 16, spec.position, spec.rotation, spec.focal_x, spec.focal_y,
+                spec.zFar, spec.zNear)
+
+    def staged(ev):
+        rast.curr_offset.zero_()
+        ev[0].record()
+        ops.preprocess_records(scene.position, shs, scene.opacity, scene.cov3d, *cam_args,
+                               rast.records, 1, rast.gaussian_keys_unsorted,
+                               rast.gaussian_values_unsorted, rast.curr_offset)
+        ev[1].record()
+        ops.sort_gaussian(M, W, H, 16, 16, rast.list_sorting_space, rast.gaussian_keys_unsorted,
+                          rast.gaussian_values_unsorted, rast.gaussian_keys_sorted,
+                          rast.gaussian_values_sorted)
+        ev[2].record()
+        ops.precompute_tile_ranges_scan(M, W, H, 16, 16, rast.gaussian_keys_sorted,
+                                        rast.tile_ranges_buf)
+        ev[3].record()
+        ops.render_records(M, W, H, rast.records, 1, False, rast.gaussian_values_sorted,
+                           rast.tile_ranges_buf, bg, rast._get_out_color(H, W, dev))
+        ev[4].record()
+
+    stage_ms = np.zeros(4)
+    reps = max(3, args.steps // 2)
+    for r in range(reps + 2):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        staged(ev)
+        torch.cuda.synchronize()
+        if r >= 2:
+            stage_ms += [ev[i].elapsed_time(ev[i + 1]) for i in range(4)]
+    stage_ms /= reps
+
+    # ---------------------------------------------------------- oracle leg
+    cpu = None
+    need = None
+    quality = {}
+    V = None
+    if not args.no_cpu:
+        import oracle as O
+
+        os.environ.setdefault("OMP_NUM_THREADS", str(args.cpu_threads))
+        pos = arrays.position.cpu().numpy()
+        op = arrays.opacity.cpu().numpy()
+        cov = arrays.cov3d.cpu().numpy()
+        sh16 = arrays.shs.half().cpu().numpy().view(np.uint16)
+        ocam = O.camera_from_spec(spec)
+        t = time.perf_counter()
+        ref16 = O.frame(pos, sh16, op, cov, ocam, (0, 0, 0), sh_type="fp16", feat_prec="fp16",
+                        check_every=8, fp64=False)
+        cpu_s = time.perf_counter() - t
+        need = int(ref16["need"].sum())
+        V = int((ref16["pre"]["counts"] > 0).sum())
+        threads = O.lib().orc_num_threads()
+        cpu = {"value": cpu_s * 1e3, "unit": "ms/frame", "cores": threads, "kind": "port",
+               "sample": f"one full {W}x{H} frame of the same {P}-Gaussian scene (FP16 SH and "
+                         f"features), oracle preprocess + stable sort + ranges + blend"}
+        log(f"oracle frame: {cpu_s:.1f}s on {threads} threads, M={ref16['pre']['M']}, "
+            f"M_need={need}")
+        t = time.perf_counter()
+        sh32 = arrays.shs.cpu().numpy()
+        ref32 = O.frame(pos, sh32, op, cov, ocam, (0, 0, 0), sh_type="fp32", check_every=2,
+                        fp64=True)
+        log(f"oracle fp32+fp64 frame: {time.perf_counter() - t:.1f}s")
+        quality["fp16_persistent"] = diff_metrics(images["fp16_persistent"], ref32["image64"])
+        quality["fp32_plain"] = diff_metrics(images["fp32_plain"], ref32["image64"])
+        quality["fp16_vs_oracle_fp16"] = diff_metrics(images["fp16_persistent"], ref16["image"])
+        quality["fp32_vs_oracle_fp32"] = diff_metrics(images["fp32_plain"], ref32["image"])
+        assert ref16["pre"]["M"] == M, (ref16["pre"]["M"], M)
+
+    # ---------------------------------------------------------- roofline
+    S_sh, F_rec = 96, 24  # FP16 SH bytes, FP16 record bytes per Gaussian
+    passes = (32 + int(T).bit_length() + 7) // 8
+    Vv = V if V is not None else M  # visible Gaussians (upper bound M if unknown)
+    Mn = need if need is not None else M
+    alg = {
+        "preprocess": P * 40 + Vv * S_sh + Vv * F_rec + M * 12,
+        "sort": passes * M * 24,
+        "ranges": M * 8 + T * 8,
+        "render": Mn * (4 + F_rec) + T * 8 + 3 * W * H,
+    }
+    names = ["preprocess", "sort", "ranges", "render"]
+    stages = {}
+    for i, n in enumerate(names):
+        gbs = alg[n] / (stage_ms[i] * 1e-3) / 1e9
+        stages[n] = {"ms": round(float(stage_ms[i]), 4), "alg_bytes": int(alg[n]),
+                     "GB_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    dom = max(names, key=lambda n: stages[n]["ms"])
+    roof = {"bound": "hbm", "kernel": dom, "achieved": stages[dom]["GB_s"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": stages[dom]["hbm_frac"],
+            "traffic": None}
+    if Mn:
+        # render is VALU-bound: ~11 VALU lane-ops + one exp2 per pixel-Gaussian
+        px_evals = Mn * 256
+        roof["render_pixel_gaussians_per_s"] = px_evals / (stage_ms[3] * 1e-3)
+
+    fp16 = results["fp16_persistent"]["ms_per_frame"]
+    fp32 = results["fp32_plain"]["ms_per_frame"]
+    line = {
+        "metric": METRIC,
+        "value": round(fp16, 4),
+        "unit": "ms/frame",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fp16, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 blend, fp16 SH + features",
+        "data": "synthetic (seeded generator, mixed-regime scale mixture)",
+        "config": {"workload": f"{args.config}: {P} Gaussians, {W}x{H}, 16x16 tiles, "
+                               "FP16-feature persistent renderer",
+                   "gaussians": P, "width": W, "height": H, "num_rendered": M,
+                   "num_rendered_needed": need},
+        "fp32_plain_ms_per_frame": round(fp32, 4),
+        "speedup_fp16_persistent_vs_fp32_plain": round(fp32 / fp16, 3),
+        "frames_per_s": round(1e3 / fp16, 2),
+        "stages_fp16": stages,
+        "quality_vs_fp64": {k: {"psnr_db": (None if v["psnr_db"] == float("inf")
+                                            else round(v["psnr_db"], 2)),
+                                "max_abs_diff": v["max_abs_diff"]}
+                            for k, v in quality.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

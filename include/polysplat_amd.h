@@ -1,0 +1,210 @@
+/*
+ * polysplat_amd: C ABI of the MI355X (gfx950) forward 3D Gaussian Splatting
+ * rasterizer.
+ *
+ * Every entry point takes plain device pointers, sizes and a HIP stream, and
+ * returns a status code.  Buffers are owned by the caller.  Nothing here
+ * synchronises the stream except ps_fetch_num_rendered and
+ * ps_forward(sync_count != 0).  Internal state lives only in caller-provided
+ * workspaces, so concurrent frames on different streams are safe when each
+ * has its own workspaces.
+ *
+ * Reference interface each function replaces is cited as file:line in
+ * LuoLongZan/PolySplat.
+ */
+#ifndef POLYSPLAT_AMD_H
+#define POLYSPLAT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same type as hipStream_t; NULL is the default stream. */
+typedef struct ihipStream_t* ps_stream_t;
+
+/* Status codes. */
+enum {
+  PS_OK = 0,
+  PS_ERR_INVALID = 1,     /* bad argument (message in ps_last_error) */
+  PS_ERR_HIP = 2,         /* HIP runtime error */
+  PS_ERR_OVERFLOW = 3,    /* more (tile, Gaussian) pairs than the key buffers hold */
+  PS_ERR_UNSUPPORTED = 4  /* configuration not implemented (e.g. non-16x16 render) */
+};
+
+/* Storage precision of SH coefficients and of the render feature records. */
+enum { PS_PREC_F32 = 0, PS_PREC_F16 = 1, PS_PREC_BF16 = 2 };
+
+/* Render variants understood by ps_forward (example.py:293-466). */
+enum {
+  PS_VARIANT_DEFAULT = 0,           /* "default": FP32 SH, plain per-tile render */
+  PS_VARIANT_PERSISTENT_LITE = 1,   /* "preranges_smem_persistent_lite[_e2e_preranges]" */
+  PS_VARIANT_PERSISTENT_FUSED = 2,  /* "preranges_smem_persistent_lite_e2e": in-kernel range search */
+  PS_VARIANT_PERSISTENT_DT = 3      /* "preranges_smem_persistent_lite_dt": skip alpha < 1/255 */
+};
+
+/* Camera exactly as the reference bindings receive it (pybind.cpp:122-150):
+ * rotation holds the 9 values of cameras.json "rotation" row by row. */
+typedef struct ps_camera {
+  int width, height;
+  int block_x, block_y;
+  float position[3];
+  float rotation[9];
+  float focal_x, focal_y;
+  float z_far, z_near;
+} ps_camera;
+
+const char* ps_status_string(int status);
+/* Detail of the last error on the calling thread ("" if none). */
+const char* ps_last_error(void);
+/* Library version string. */
+const char* ps_version(void);
+
+/* View/projection matrices (column-major) and tan(fov/2); host only.
+ * Replaces getViewMatrix/getProjectionMatrix, preprocess.cu:643-671 and the
+ * tan_fov lines at preprocess.cu:1263-1266. */
+int ps_camera_matrices(const ps_camera* cam, float view[16], float proj[16], float tan_fov[2]);
+
+/* ---------------------------------------------------------------- preprocess
+ * Replaces preprocess (preprocess.cu:1252, ops.h:42) and preprocess_half_sh
+ * (preprocess.cu:1292, ops.h:110); bound in the reference at
+ * pybind.cpp:122-181.
+ *
+ * positions f32[P][3], shs [P][48] in sh_prec, opacities f32[P],
+ * cov3d f32[P][6].  Outputs (each may be NULL): points_xy f32[P][2],
+ * rgb_depth f32[P][4], conic_opacity f32[P][4], and feature records for the
+ * reduced-precision renderer (24 bytes per Gaussian, feat_prec F16/BF16).
+ * Features are written for every Gaussian that emits at least one key.
+ *
+ * Keys u64[capacity] (tile << 32 | depth bits) and values u32[capacity] are
+ * appended at *curr_offset in a canonical order (Gaussian id, then tile
+ * row-major); *curr_offset is advanced by the number of pairs.  If the pairs
+ * do not fit, nothing is emitted, *curr_offset still receives the required
+ * total and bit 0 of the status word in the workspace is set (read it with
+ * ps_preprocess_status).  workspace: ps_preprocess_workspace_size(P) bytes. */
+size_t ps_preprocess_workspace_size(int P);
+int ps_preprocess(int P, const float* positions, const void* shs, int sh_prec,
+                  const float* opacities, const float* cov3d, const ps_camera* cam,
+                  float* points_xy, float* rgb_depth, float* conic_opacity,
+                  void* feat_records, int feat_prec,
+                  uint64_t* keys_unsorted, uint32_t* values_unsorted, int capacity,
+                  int* curr_offset, void* workspace, size_t workspace_bytes,
+                  ps_stream_t stream);
+/* Device pointer to the pair count the preprocess left for the later stages
+ * (0 after an overflow), and to the overflow status word. */
+const int* ps_preprocess_count_ptr(const void* workspace);
+const int* ps_preprocess_status_ptr(const void* workspace);
+
+/* ---------------------------------------------------------------- sort
+ * Stable LSD radix sort of (u64 key, u32 value) pairs over bits
+ * [0, 32 + bit_length(tiles)).  Replaces sort_gaussian (sort.cu:107-125,
+ * ops.h:50) and get_sort_buffer_size (sort.cu:127-149, ops.h:56); bound at
+ * pybind.cpp:333-349.  The pair count comes from num_rendered when
+ * d_num_rendered is NULL, else from device memory (no host round trip).
+ * Inputs are preserved; outputs must not alias inputs. */
+size_t ps_get_sort_buffer_size(int max_pairs);
+int ps_sort_gaussian(int num_rendered, const int* d_num_rendered, int max_pairs,
+                     int width, int height, int block_x, int block_y,
+                     void* sort_space, size_t sort_space_bytes,
+                     const uint64_t* keys_unsorted, const uint32_t* values_unsorted,
+                     uint64_t* keys_sorted, uint32_t* values_sorted, ps_stream_t stream);
+
+/* Copies *d_curr_offset to the host through pinned memory and waits for it.
+ * Replaces fetch_num_rendered_async (sort.cu:289-300, ops.h:103). */
+int ps_fetch_num_rendered(const int* d_curr_offset, int* out_num_rendered, ps_stream_t stream);
+
+/* ---------------------------------------------------------------- ranges
+ * tile_ranges int32[tiles][2] = [start, end) of each tile in the sorted keys.
+ * precompute_tile_ranges (render.cu:6812, binary search: empty tile ->
+ * (lo, lo)) and precompute_tile_ranges_scan (render.cu:6826, boundary scan:
+ * empty tile -> (0, 0)); bound at pybind.cpp:441-459. */
+int ps_precompute_tile_ranges(int num_rendered, const int* d_num_rendered,
+                              int width, int height, int block_x, int block_y,
+                              const uint64_t* keys_sorted, int* tile_ranges, ps_stream_t stream);
+int ps_precompute_tile_ranges_scan(int num_rendered, const int* d_num_rendered, int max_pairs,
+                                   int width, int height, int block_x, int block_y,
+                                   const uint64_t* keys_sorted, int* tile_ranges,
+                                   ps_stream_t stream);
+
+/* ---------------------------------------------------------------- gather
+ * dst[i] = src[values_sorted[i]] for xy f32x2, rgb_depth f32x4,
+ * conic_opacity f32x4.  Replaces gather_features (gather.cu:46-58, ops.h:105);
+ * bound at pybind.cpp:351-364. */
+int ps_gather_features(int num_rendered, const int* d_num_rendered, int max_pairs,
+                       const uint32_t* values_sorted,
+                       const float* src_xy, const float* src_rgb_depth, const float* src_conic,
+                       float* dst_xy, float* dst_rgb_depth, float* dst_conic, ps_stream_t stream);
+/* Same for the 24-byte reduced-precision records. */
+int ps_gather_records(int num_rendered, const int* d_num_rendered, int max_pairs,
+                      const uint32_t* values_sorted, const void* src_records, void* dst_records,
+                      ps_stream_t stream);
+
+/* ---------------------------------------------------------------- render
+ * Output u8[height][width][3].  bg: 3 floats on the host.
+ *
+ * ps_render_16x16: plain per-tile renderer, one wave per 16x16 tile, range
+ * found by an in-kernel search over keys_sorted.  Replaces render_16x16
+ * (render.cu:6445 -> renderCUDA render.cu:2176-2621); bound at
+ * pybind.cpp:616-632.
+ *
+ * ps_render_16x16_persistent: persistent tile-queue renderer with LDS
+ * double buffering.  Ranges come from tile_ranges (keys_sorted may be NULL),
+ * or, when tile_ranges is NULL, from an in-kernel search of keys_sorted.
+ * Replaces render_16x16_preranges_smem_persistent_lite (render.cu:6537),
+ * ..._dt (render.cu:6576) and ..._fused (render.cu:6616); bound at
+ * pybind.cpp:691-745.  feat_prec selects f32 arrays (points_xy, rgb_depth,
+ * conic_opacity) or 24-byte records (records).  `gathered` != 0 means the
+ * features are already in sorted-pair order (index = pair position).
+ * queue: 2 u32 of device memory, zero before the first use; every launch
+ * leaves them zero again.  dt != 0 skips pixels with alpha < 1/255. */
+int ps_render_16x16(int num_rendered, const int* d_num_rendered, int width, int height,
+                    const float* points_xy, const float* rgb_depth, const float* conic_opacity,
+                    const uint64_t* keys_sorted, const uint32_t* values_sorted,
+                    const float bg[3], uint8_t* out_color, ps_stream_t stream);
+int ps_render_16x16_persistent(int num_rendered, const int* d_num_rendered, int width, int height,
+                               int feat_prec, const float* points_xy, const float* rgb_depth,
+                               const float* conic_opacity, const void* records, int gathered,
+                               const uint64_t* keys_sorted, const uint32_t* values_sorted,
+                               const int* tile_ranges, int dt, const float bg[3],
+                               uint8_t* out_color, unsigned* queue, ps_stream_t stream);
+
+/* ---------------------------------------------------------------- forward
+ * One frame on one stream: preprocess -> sort -> ranges -> render, with the
+ * pair count passed between stages in device memory.  Replaces forward_fused
+ * (pybind.cpp:932-1012), forward_fused_e2e_preranges (pybind.cpp:1016-1080)
+ * and forward_fused_e2e (pybind.cpp:1085-1149).  feat_prec F32 writes the
+ * f32 feature arrays; F16/BF16 writes 24-byte records into `records`
+ * instead.  With sync_count != 0 the call waits for the count and returns
+ * PS_ERR_OVERFLOW when the pairs did not fit; *num_rendered_out receives the
+ * count in that case.  Workspace: ps_forward_workspace_size(). */
+typedef struct ps_frame_buffers {
+  int* curr_offset;               /* i32[1] */
+  void* sort_space;               /* ps_get_sort_buffer_size(capacity) bytes */
+  size_t sort_space_bytes;
+  uint64_t* keys_unsorted;        /* [capacity] */
+  uint32_t* values_unsorted;
+  uint64_t* keys_sorted;
+  uint32_t* values_sorted;
+  int capacity;
+  float* points_xy;               /* f32 features [P] (F32 mode) */
+  float* rgb_depth;
+  float* conic_opacity;
+  void* records;                  /* 24-byte records [P] (F16/BF16 mode) */
+  int* tile_ranges;               /* i32[tiles][2], may be NULL for the fused variant */
+  void* workspace;                /* ps_forward_workspace_size(P) bytes */
+  size_t workspace_bytes;
+} ps_frame_buffers;
+
+size_t ps_forward_workspace_size(int P);
+int ps_forward(int P, const float* positions, const void* shs, int sh_prec,
+               const float* opacities, const float* cov3d, const ps_camera* cam,
+               int variant, int feat_prec, const ps_frame_buffers* fb, const float bg[3],
+               uint8_t* out_color, int sync_count, int* num_rendered_out, ps_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POLYSPLAT_AMD_H */

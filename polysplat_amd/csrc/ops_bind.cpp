@@ -1,0 +1,507 @@
+// pybind11 module `polysplat_amd._C`, exposed to Python as `polysplat_amd.ops`.
+//
+// It keeps the names, positional signatures and in-place output convention
+// of the reference's `polysplat.ops` hot-path bindings (pybind.cpp:122-181,
+// :333-364, :441-459, :616-632, :691-745, :932-1149).  Each binding only
+// unpacks tensors into raw pointers and calls the C ABI in
+// include/polysplat_amd.h on torch's current HIP stream; no arithmetic
+// happens here.  Errors become RuntimeError (the reference raises
+// RuntimeError for CUB failures, sort.cu:121-124).
+
+#include <c10/hip/HIPStream.h>
+#include <torch/extension.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "polysplat_amd.h"
+
+namespace {
+
+using at::Tensor;
+
+ps_stream_t cur_stream() {
+  return reinterpret_cast<ps_stream_t>(c10::hip::getCurrentHIPStream().stream());
+}
+
+void check(int status, const char* what) {
+  if (status != PS_OK) {
+    throw std::runtime_error(std::string(what) + ": " + ps_status_string(status) + " (" +
+                             ps_last_error() + ")");
+  }
+}
+
+void need(bool cond, const std::string& msg) {
+  if (!cond) throw std::invalid_argument(msg);
+}
+
+void need_dev(const Tensor& t, const char* name, at::ScalarType dt) {
+  need(t.is_cuda(), std::string(name) + " must be a GPU tensor");
+  need(t.scalar_type() == dt, std::string(name) + " has dtype " +
+                                  std::string(c10::toString(t.scalar_type())) + ", expected " +
+                                  std::string(c10::toString(dt)));
+  need(t.is_contiguous(), std::string(name) + " must be contiguous");
+}
+
+template <typename T>
+T* ptr(const Tensor& t) {
+  return reinterpret_cast<T*>(t.data_ptr());
+}
+
+std::vector<float> host_floats(const Tensor& t, int64_t n, const char* name) {
+  Tensor c = t.detach().to(at::kCPU, at::kFloat).contiguous();
+  need(c.numel() >= n, std::string(name) + " needs " + std::to_string(n) + " values");
+  const float* p = c.data_ptr<float>();
+  return std::vector<float>(p, p + n);
+}
+
+ps_camera make_camera(int width, int height, int block_x, int block_y, const Tensor& position,
+                      const Tensor& rotation, float fx, float fy, float zfar, float znear) {
+  ps_camera c;
+  c.width = width;
+  c.height = height;
+  c.block_x = block_x;
+  c.block_y = block_y;
+  auto p = host_floats(position, 3, "position");
+  auto r = host_floats(rotation, 9, "rotation");
+  for (int i = 0; i < 3; ++i) c.position[i] = p[i];
+  for (int i = 0; i < 9; ++i) c.rotation[i] = r[i];
+  c.focal_x = fx;
+  c.focal_y = fy;
+  c.z_far = zfar;
+  c.z_near = znear;
+  return c;
+}
+
+int sh_prec_of(const Tensor& shs) {
+  switch (shs.scalar_type()) {
+    case at::kFloat: return PS_PREC_F32;
+    case at::kHalf: return PS_PREC_F16;
+    case at::kBFloat16: return PS_PREC_BF16;
+    default: throw std::invalid_argument("shs must be float32, float16 or bfloat16");
+  }
+}
+
+void check_scene(const Tensor& pts, const Tensor& shs, const Tensor& op, const Tensor& cov) {
+  const int64_t P = op.size(0);
+  need_dev(pts, "orig_points", at::kFloat);
+  need_dev(op, "opacities", at::kFloat);
+  need_dev(cov, "cov3Ds", at::kFloat);
+  need(shs.is_cuda() && shs.is_contiguous(), "shs must be a contiguous GPU tensor");
+  need(pts.numel() == P * 3, "orig_points must be [P,3]");
+  need(shs.numel() == P * 48, "shs must be [P,48] (or [P,16,3])");
+  need(cov.numel() == P * 6, "cov3Ds must be [P,6]");
+}
+
+// Scratch owned by the module for bindings whose reference signature has no
+// workspace argument.  Grown on demand from torch's caching allocator and
+// zero-filled when created (the persistent queue words must start at 0).
+Tensor cached_workspace(const c10::Device& dev, size_t bytes, const char* tag) {
+  static std::mutex mu;
+  static std::map<std::pair<std::string, int>, Tensor> cache;
+  std::lock_guard<std::mutex> g(mu);
+  auto key = std::make_pair(std::string(tag), (int)dev.index());
+  auto it = cache.find(key);
+  if (it == cache.end() || (size_t)it->second.numel() < bytes) {
+    Tensor t = at::zeros({(int64_t)bytes}, at::TensorOptions().dtype(at::kByte).device(dev));
+    cache[key] = t;
+    return t;
+  }
+  return it->second;
+}
+
+// ------------------------------------------------------------------ preprocess
+void preprocess_impl(Tensor& orig_points, Tensor& shs, Tensor& opacities, Tensor& cov3Ds,
+                     int width, int height, int block_x, int block_y, Tensor& position,
+                     Tensor& rotation, float focal_x, float focal_y, float zFar, float zNear,
+                     Tensor& points_xy, Tensor& rgb_depth, Tensor& conic_opacity,
+                     Tensor& keys_unsorted, Tensor& values_unsorted, Tensor& curr_offset) {
+  check_scene(orig_points, shs, opacities, cov3Ds);
+  need_dev(points_xy, "points_xy", at::kFloat);
+  need_dev(rgb_depth, "rgb_depth", at::kFloat);
+  need_dev(conic_opacity, "conic_opacity", at::kFloat);
+  need_dev(keys_unsorted, "gaussian_keys_unsorted", at::kLong);
+  need_dev(values_unsorted, "gaussian_values_unsorted", at::kInt);
+  need_dev(curr_offset, "curr_offset", at::kInt);
+  const int P = (int)opacities.size(0);
+  need(points_xy.numel() >= 2LL * P && rgb_depth.numel() >= 4LL * P &&
+           conic_opacity.numel() >= 4LL * P,
+       "feature buffers must hold P entries");
+  const int cap = (int)std::min<int64_t>(keys_unsorted.numel(), values_unsorted.numel());
+  ps_camera cam = make_camera(width, height, block_x, block_y, position, rotation, focal_x,
+                              focal_y, zFar, zNear);
+  const size_t wsb = ps_preprocess_workspace_size(P);
+  Tensor ws = at::empty({(int64_t)wsb}, opacities.options().dtype(at::kByte));
+  check(ps_preprocess(P, ptr<float>(orig_points), shs.data_ptr(), sh_prec_of(shs),
+                      ptr<float>(opacities), ptr<float>(cov3Ds), &cam, ptr<float>(points_xy),
+                      ptr<float>(rgb_depth), ptr<float>(conic_opacity), nullptr, PS_PREC_F16,
+                      ptr<uint64_t>(keys_unsorted), ptr<uint32_t>(values_unsorted), cap,
+                      ptr<int>(curr_offset), ws.data_ptr(), wsb, cur_stream()),
+        "preprocess");
+}
+
+// Same as preprocess, writing the 24-byte reduced-precision records too.
+void preprocess_records(Tensor& orig_points, Tensor& shs, Tensor& opacities, Tensor& cov3Ds,
+                        int width, int height, int block_x, int block_y, Tensor& position,
+                        Tensor& rotation, float focal_x, float focal_y, float zFar, float zNear,
+                        Tensor& records, int feat_prec, Tensor& keys_unsorted,
+                        Tensor& values_unsorted, Tensor& curr_offset) {
+  check_scene(orig_points, shs, opacities, cov3Ds);
+  const int P = (int)opacities.size(0);
+  need_dev(records, "records", at::kByte);
+  need(records.numel() >= 24LL * P, "records must hold 24 bytes per Gaussian");
+  need_dev(keys_unsorted, "gaussian_keys_unsorted", at::kLong);
+  need_dev(values_unsorted, "gaussian_values_unsorted", at::kInt);
+  need_dev(curr_offset, "curr_offset", at::kInt);
+  const int cap = (int)std::min<int64_t>(keys_unsorted.numel(), values_unsorted.numel());
+  ps_camera cam = make_camera(width, height, block_x, block_y, position, rotation, focal_x,
+                              focal_y, zFar, zNear);
+  const size_t wsb = ps_preprocess_workspace_size(P);
+  Tensor ws = at::empty({(int64_t)wsb}, opacities.options().dtype(at::kByte));
+  check(ps_preprocess(P, ptr<float>(orig_points), shs.data_ptr(), sh_prec_of(shs),
+                      ptr<float>(opacities), ptr<float>(cov3Ds), &cam, nullptr, nullptr, nullptr,
+                      records.data_ptr(), feat_prec, ptr<uint64_t>(keys_unsorted),
+                      ptr<uint32_t>(values_unsorted), cap, ptr<int>(curr_offset), ws.data_ptr(),
+                      wsb, cur_stream()),
+        "preprocess_records");
+}
+
+// ------------------------------------------------------------------ sort
+size_t get_sort_buffer_size(int64_t num_rendered) {
+  need(num_rendered >= 0 && num_rendered < (1LL << 31), "num_rendered out of range");
+  return ps_get_sort_buffer_size((int)num_rendered);
+}
+
+void sort_gaussian(int num_rendered, int width, int height, int block_x, int block_y,
+                   Tensor& list_sorting_space, Tensor& keys_unsorted, Tensor& values_unsorted,
+                   Tensor& keys_sorted, Tensor& values_sorted) {
+  need_dev(list_sorting_space, "list_sorting_space", at::kChar);
+  need_dev(keys_unsorted, "gaussian_keys_unsorted", at::kLong);
+  need_dev(values_unsorted, "gaussian_values_unsorted", at::kInt);
+  need_dev(keys_sorted, "gaussian_keys_sorted", at::kLong);
+  need_dev(values_sorted, "gaussian_values_sorted", at::kInt);
+  const int cap = (int)std::min({keys_unsorted.numel(), values_unsorted.numel(),
+                                 keys_sorted.numel(), values_sorted.numel()});
+  // The sort space was sized for some capacity; find the largest it serves.
+  int max_pairs = cap;
+  const size_t have = (size_t)list_sorting_space.numel();
+  while (max_pairs > 0 && ps_get_sort_buffer_size(max_pairs) > have) max_pairs /= 2;
+  need(num_rendered <= max_pairs, "list_sorting_space too small for num_rendered");
+  check(ps_sort_gaussian(num_rendered, nullptr, max_pairs, width, height, block_x, block_y,
+                         list_sorting_space.data_ptr(), have, ptr<uint64_t>(keys_unsorted),
+                         ptr<uint32_t>(values_unsorted), ptr<uint64_t>(keys_sorted),
+                         ptr<uint32_t>(values_sorted), cur_stream()),
+        "sort_gaussian");
+}
+
+// ------------------------------------------------------------------ gather / ranges
+void gather_features(int num_rendered, Tensor& values_sorted, Tensor& src_xy, Tensor& src_rgbd,
+                     Tensor& src_conic, Tensor& dst_xy, Tensor& dst_rgbd, Tensor& dst_conic) {
+  need_dev(values_sorted, "sorted_values", at::kInt);
+  for (auto* t : {&src_xy, &src_rgbd, &src_conic, &dst_xy, &dst_rgbd, &dst_conic})
+    need_dev(*t, "feature", at::kFloat);
+  need(num_rendered >= 0 && num_rendered <= values_sorted.numel(), "num_rendered out of range");
+  need(dst_xy.numel() >= 2LL * num_rendered && dst_rgbd.numel() >= 4LL * num_rendered &&
+           dst_conic.numel() >= 4LL * num_rendered,
+       "destination buffers too small");
+  check(ps_gather_features(num_rendered, nullptr, num_rendered, ptr<uint32_t>(values_sorted),
+                           ptr<float>(src_xy), ptr<float>(src_rgbd), ptr<float>(src_conic),
+                           ptr<float>(dst_xy), ptr<float>(dst_rgbd), ptr<float>(dst_conic),
+                           cur_stream()),
+        "gather_features");
+}
+
+int64_t tiles_of(int width, int height, int bx, int by) {
+  return (int64_t)((width + bx - 1) / bx) * ((height + by - 1) / by);
+}
+
+void precompute_tile_ranges(int num_rendered, int width, int height, int block_x, int block_y,
+                            Tensor& keys_sorted, Tensor& tile_ranges) {
+  need_dev(keys_sorted, "gaussian_keys_sorted", at::kLong);
+  need_dev(tile_ranges, "tile_ranges_buf", at::kInt);
+  need(tile_ranges.numel() >= 2 * tiles_of(width, height, block_x, block_y),
+       "tile_ranges_buf too small");
+  check(ps_precompute_tile_ranges(num_rendered, nullptr, width, height, block_x, block_y,
+                                  ptr<uint64_t>(keys_sorted), ptr<int>(tile_ranges), cur_stream()),
+        "precompute_tile_ranges");
+}
+
+void precompute_tile_ranges_scan(int num_rendered, int width, int height, int block_x,
+                                 int block_y, Tensor& keys_sorted, Tensor& tile_ranges) {
+  need_dev(keys_sorted, "gaussian_keys_sorted", at::kLong);
+  need_dev(tile_ranges, "tile_ranges_buf", at::kInt);
+  need(tile_ranges.numel() >= 2 * tiles_of(width, height, block_x, block_y),
+       "tile_ranges_buf too small");
+  check(ps_precompute_tile_ranges_scan(num_rendered, nullptr, num_rendered, width, height,
+                                       block_x, block_y, ptr<uint64_t>(keys_sorted),
+                                       ptr<int>(tile_ranges), cur_stream()),
+        "precompute_tile_ranges_scan");
+}
+
+// ------------------------------------------------------------------ render
+void check_out(const Tensor& out, int width, int height) {
+  need_dev(out, "out_color", at::kByte);
+  need(out.numel() == 3LL * width * height, "out_color must be [H,W,3]");
+}
+
+void check_feats(const Tensor& xy, const Tensor& rgbd, const Tensor& con) {
+  need_dev(xy, "points_xy", at::kFloat);
+  need_dev(rgbd, "rgb_depth", at::kFloat);
+  need_dev(con, "conic_opacity", at::kFloat);
+}
+
+void render_16x16(int num_rendered, int width, int height, Tensor& points_xy, Tensor& rgb_depth,
+                  Tensor& conic_opacity, Tensor& keys_sorted, Tensor& values_sorted,
+                  Tensor& bg_color, Tensor& out_color) {
+  check_feats(points_xy, rgb_depth, conic_opacity);
+  need_dev(keys_sorted, "gaussian_keys_sorted", at::kLong);
+  need_dev(values_sorted, "gaussian_values_sorted", at::kInt);
+  check_out(out_color, width, height);
+  auto bg = host_floats(bg_color, 3, "bg_color");
+  check(ps_render_16x16(num_rendered, nullptr, width, height, ptr<float>(points_xy),
+                        ptr<float>(rgb_depth), ptr<float>(conic_opacity),
+                        ptr<uint64_t>(keys_sorted), ptr<uint32_t>(values_sorted), bg.data(),
+                        ptr<uint8_t>(out_color), cur_stream()),
+        "render_16x16");
+}
+
+void persistent_common(int num_rendered, int width, int height, Tensor* xy, Tensor* rgbd,
+                       Tensor* con, Tensor* records, int prec, bool gathered, Tensor* keys,
+                       Tensor& values, Tensor* ranges, bool dt, Tensor& bg_color, Tensor& out,
+                       const char* what) {
+  if (prec == PS_PREC_F32) check_feats(*xy, *rgbd, *con);
+  else need_dev(*records, "records", at::kByte);
+  need_dev(values, "gaussian_values_sorted", at::kInt);
+  if (keys) need_dev(*keys, "gaussian_keys_sorted", at::kLong);
+  if (ranges) {
+    need_dev(*ranges, "tile_ranges_buf", at::kInt);
+    need(ranges->numel() >= 2 * tiles_of(width, height, 16, 16), "tile_ranges_buf too small");
+  }
+  check_out(out, width, height);
+  auto bg = host_floats(bg_color, 3, "bg_color");
+  Tensor q = cached_workspace(out.device(), 256, "queue");
+  check(ps_render_16x16_persistent(
+            num_rendered, nullptr, width, height, prec, xy ? ptr<float>(*xy) : nullptr,
+            rgbd ? ptr<float>(*rgbd) : nullptr, con ? ptr<float>(*con) : nullptr,
+            records ? records->data_ptr() : nullptr, gathered ? 1 : 0,
+            keys ? ptr<uint64_t>(*keys) : nullptr, ptr<uint32_t>(values),
+            ranges ? ptr<int>(*ranges) : nullptr, dt ? 1 : 0, bg.data(), ptr<uint8_t>(out),
+            ptr<unsigned>(q), cur_stream()),
+        what);
+}
+
+void render_lite(int num_rendered, int width, int height, Tensor& xy, Tensor& rgbd, Tensor& con,
+                 Tensor& values_sorted, Tensor& tile_ranges, Tensor& bg, Tensor& out) {
+  persistent_common(num_rendered, width, height, &xy, &rgbd, &con, nullptr, PS_PREC_F32, false,
+                    nullptr, values_sorted, &tile_ranges, false, bg, out,
+                    "render_16x16_preranges_smem_persistent_lite");
+}
+
+void render_lite_dt(int num_rendered, int width, int height, Tensor& xy, Tensor& rgbd,
+                    Tensor& con, Tensor& values_sorted, Tensor& tile_ranges, Tensor& bg,
+                    Tensor& out) {
+  persistent_common(num_rendered, width, height, &xy, &rgbd, &con, nullptr, PS_PREC_F32, false,
+                    nullptr, values_sorted, &tile_ranges, true, bg, out,
+                    "render_16x16_preranges_smem_persistent_lite_dt");
+}
+
+void render_lite_fused(int num_rendered, int width, int height, Tensor& xy, Tensor& rgbd,
+                       Tensor& con, Tensor& keys_sorted, Tensor& values_sorted, Tensor& bg,
+                       Tensor& out) {
+  persistent_common(num_rendered, width, height, &xy, &rgbd, &con, nullptr, PS_PREC_F32, false,
+                    &keys_sorted, values_sorted, nullptr, false, bg, out,
+                    "render_16x16_preranges_smem_persistent_lite_fused");
+}
+
+// Persistent render of 24-byte records (FP16/BF16 features).
+void render_records(int num_rendered, int width, int height, Tensor& records, int feat_prec,
+                    bool gathered, Tensor& values_sorted, Tensor& tile_ranges, Tensor& bg,
+                    Tensor& out) {
+  persistent_common(num_rendered, width, height, nullptr, nullptr, nullptr, &records, feat_prec,
+                    gathered, nullptr, values_sorted, &tile_ranges, false, bg, out,
+                    "render_records");
+}
+
+void gather_records(int num_rendered, Tensor& values_sorted, Tensor& src, Tensor& dst) {
+  need_dev(values_sorted, "sorted_values", at::kInt);
+  need_dev(src, "src", at::kByte);
+  need_dev(dst, "dst", at::kByte);
+  need(dst.numel() >= 24LL * num_rendered, "dst too small");
+  check(ps_gather_records(num_rendered, nullptr, num_rendered, ptr<uint32_t>(values_sorted),
+                          src.data_ptr(), dst.data_ptr(), cur_stream()),
+        "gather_records");
+}
+
+// ------------------------------------------------------------------ forward
+struct FrameArgs {
+  Tensor* curr_offset;
+  Tensor* sort_space;
+  Tensor* ku;
+  Tensor* vu;
+  Tensor* ks;
+  Tensor* vs;
+  Tensor* xy;
+  Tensor* rgbd;
+  Tensor* con;
+  Tensor* records;
+  Tensor* ranges;
+};
+
+int forward_common(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width, int height,
+                   int block_x, int block_y, Tensor& position, Tensor& rotation, float fx,
+                   float fy, float zfar, float znear, int variant, int feat_prec,
+                   const FrameArgs& a, Tensor& bg_color, Tensor& out, Tensor* workspace,
+                   bool sync, const char* what) {
+  check_scene(pts, shs, op, cov);
+  need(block_x == 16 && block_y == 16, "only 16x16 tiles are implemented");
+  need_dev(*a.curr_offset, "curr_offset", at::kInt);
+  need_dev(*a.sort_space, "list_sorting_space", at::kChar);
+  need_dev(*a.ku, "gaussian_keys_unsorted", at::kLong);
+  need_dev(*a.vu, "gaussian_values_unsorted", at::kInt);
+  need_dev(*a.ks, "gaussian_keys_sorted", at::kLong);
+  need_dev(*a.vs, "gaussian_values_sorted", at::kInt);
+  check_out(out, width, height);
+  const int P = (int)op.size(0);
+  ps_frame_buffers fb;
+  memset(&fb, 0, sizeof(fb));
+  fb.curr_offset = ptr<int>(*a.curr_offset);
+  fb.sort_space = a.sort_space->data_ptr();
+  fb.sort_space_bytes = (size_t)a.sort_space->numel();
+  fb.keys_unsorted = ptr<uint64_t>(*a.ku);
+  fb.values_unsorted = ptr<uint32_t>(*a.vu);
+  fb.keys_sorted = ptr<uint64_t>(*a.ks);
+  fb.values_sorted = ptr<uint32_t>(*a.vs);
+  int cap = (int)std::min({a.ku->numel(), a.vu->numel(), a.ks->numel(), a.vs->numel()});
+  while (cap > 0 && ps_get_sort_buffer_size(cap) > fb.sort_space_bytes) cap /= 2;
+  fb.capacity = cap;
+  if (feat_prec == PS_PREC_F32) {
+    check_feats(*a.xy, *a.rgbd, *a.con);
+    fb.points_xy = ptr<float>(*a.xy);
+    fb.rgb_depth = ptr<float>(*a.rgbd);
+    fb.conic_opacity = ptr<float>(*a.con);
+  } else {
+    need_dev(*a.records, "records", at::kByte);
+    fb.records = a.records->data_ptr();
+  }
+  if (a.ranges) {
+    need_dev(*a.ranges, "tile_ranges_buf", at::kInt);
+    need(a.ranges->numel() >= 2 * tiles_of(width, height, 16, 16), "tile_ranges_buf too small");
+    fb.tile_ranges = ptr<int>(*a.ranges);
+  }
+  const size_t wsb = ps_forward_workspace_size(P);
+  Tensor ws;
+  if (workspace && workspace->defined() && workspace->numel() > 0) {
+    need_dev(*workspace, "workspace", at::kByte);
+    need((size_t)workspace->numel() >= wsb, "workspace too small");
+    ws = *workspace;
+  } else {
+    ws = cached_workspace(op.device(), wsb, "forward");
+  }
+  fb.workspace = ws.data_ptr();
+  fb.workspace_bytes = (size_t)ws.numel();
+  ps_camera cam = make_camera(width, height, block_x, block_y, position, rotation, fx, fy, zfar,
+                              znear);
+  auto bg = host_floats(bg_color, 3, "bg_color");
+  int n = 0;
+  const int st = ps_forward(P, ptr<float>(pts), shs.data_ptr(), sh_prec_of(shs), ptr<float>(op),
+                            ptr<float>(cov), &cam, variant, feat_prec, &fb, bg.data(),
+                            ptr<uint8_t>(out), sync ? 1 : 0, &n, cur_stream());
+  if (st == PS_ERR_OVERFLOW)
+    throw std::runtime_error(std::string(what) + ": Too many k-v pairs! (" + ps_last_error() + ")");
+  check(st, what);
+  return n;
+}
+
+int forward_fused(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width, int height,
+                  int block_x, int block_y, Tensor& position, Tensor& rotation, float fx, float fy,
+                  float zfar, float znear, const std::string& render_variant, Tensor& curr_offset,
+                  Tensor& sort_space, Tensor& ku, Tensor& vu, Tensor& ks, Tensor& vs,
+                  Tensor& xy, Tensor& rgbd, Tensor& con, Tensor& bg, Tensor& out) {
+  need(render_variant == "default",
+       "forward_fused implements render_variant 'default' only (got '" + render_variant + "')");
+  FrameArgs a{&curr_offset, &sort_space, &ku, &vu, &ks, &vs, &xy, &rgbd, &con, nullptr, nullptr};
+  return forward_common(pts, shs, op, cov, width, height, block_x, block_y, position, rotation,
+                        fx, fy, zfar, znear, PS_VARIANT_DEFAULT, PS_PREC_F32, a, bg, out, nullptr,
+                        true, "forward_fused");
+}
+
+int forward_fused_e2e(Tensor& pts, Tensor& shs_half, Tensor& op, Tensor& cov, int width,
+                      int height, int block_x, int block_y, Tensor& position, Tensor& rotation,
+                      float fx, float fy, float zfar, float znear, Tensor& curr_offset,
+                      Tensor& sort_space, Tensor& ku, Tensor& vu, Tensor& ks, Tensor& vs,
+                      Tensor& xy, Tensor& rgbd, Tensor& con, Tensor& bg, Tensor& out) {
+  FrameArgs a{&curr_offset, &sort_space, &ku, &vu, &ks, &vs, &xy, &rgbd, &con, nullptr, nullptr};
+  return forward_common(pts, shs_half, op, cov, width, height, block_x, block_y, position,
+                        rotation, fx, fy, zfar, znear, PS_VARIANT_PERSISTENT_FUSED, PS_PREC_F32, a,
+                        bg, out, nullptr, true, "forward_fused_e2e");
+}
+
+int forward_fused_e2e_preranges(Tensor& pts, Tensor& shs_half, Tensor& op, Tensor& cov, int width,
+                                int height, int block_x, int block_y, Tensor& position,
+                                Tensor& rotation, float fx, float fy, float zfar, float znear,
+                                Tensor& curr_offset, Tensor& sort_space, Tensor& ku, Tensor& vu,
+                                Tensor& ks, Tensor& vs, Tensor& xy, Tensor& rgbd, Tensor& con,
+                                Tensor& tile_ranges, Tensor& bg, Tensor& out) {
+  FrameArgs a{&curr_offset, &sort_space, &ku, &vu, &ks, &vs, &xy, &rgbd, &con, nullptr,
+              &tile_ranges};
+  return forward_common(pts, shs_half, op, cov, width, height, block_x, block_y, position,
+                        rotation, fx, fy, zfar, znear, PS_VARIANT_PERSISTENT_LITE, PS_PREC_F32, a,
+                        bg, out, nullptr, true, "forward_fused_e2e_preranges");
+}
+
+// Generic frame: any variant, any feature precision, optional caller
+// workspace, optional end-of-frame count readback.
+int forward_frame(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width, int height,
+                  Tensor& position, Tensor& rotation, float fx, float fy, float zfar, float znear,
+                  int variant, int feat_prec, Tensor& curr_offset, Tensor& sort_space, Tensor& ku,
+                  Tensor& vu, Tensor& ks, Tensor& vs, Tensor& xy, Tensor& rgbd, Tensor& con,
+                  Tensor& records, Tensor& tile_ranges, Tensor& bg, Tensor& out,
+                  Tensor& workspace, bool sync) {
+  FrameArgs a{&curr_offset, &sort_space, &ku, &vu, &ks, &vs, &xy, &rgbd, &con, &records,
+              tile_ranges.numel() > 0 ? &tile_ranges : nullptr};
+  return forward_common(pts, shs, op, cov, width, height, 16, 16, position, rotation, fx, fy,
+                        zfar, znear, variant, feat_prec, a, bg, out, &workspace, sync,
+                        "forward_frame");
+}
+
+py::tuple camera_matrices(int width, int height, Tensor& position, Tensor& rotation, float fx,
+                          float fy, float zfar, float znear) {
+  ps_camera cam = make_camera(width, height, 16, 16, position, rotation, fx, fy, zfar, znear);
+  Tensor v = at::empty({16}, at::kFloat), p = at::empty({16}, at::kFloat),
+         t = at::empty({2}, at::kFloat);
+  check(ps_camera_matrices(&cam, v.data_ptr<float>(), p.data_ptr<float>(), t.data_ptr<float>()),
+        "camera_matrices");
+  return py::make_tuple(v, p, t);
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_C, m) {
+  m.doc() = "MI355X polysplat forward rasterizer (HIP, gfx950)";
+  m.def("version", []() { return std::string(ps_version()); });
+  auto ops = m.def_submodule("ops", "polysplat.ops-compatible operators");
+  ops.def("preprocess", &preprocess_impl);
+  ops.def("preprocess_half_sh", &preprocess_impl);
+  ops.def("preprocess_records", &preprocess_records);
+  ops.def("get_sort_buffer_size", &get_sort_buffer_size);
+  ops.def("sort_gaussian", &sort_gaussian);
+  ops.def("gather_features", &gather_features);
+  ops.def("gather_records", &gather_records);
+  ops.def("precompute_tile_ranges", &precompute_tile_ranges);
+  ops.def("precompute_tile_ranges_scan", &precompute_tile_ranges_scan);
+  ops.def("render_16x16", &render_16x16);
+  ops.def("render_16x16_preranges_smem_persistent_lite", &render_lite);
+  ops.def("render_16x16_preranges_smem_persistent_lite_dt", &render_lite_dt);
+  ops.def("render_16x16_preranges_smem_persistent_lite_fused", &render_lite_fused);
+  ops.def("render_records", &render_records);
+  ops.def("forward_fused", &forward_fused);
+  ops.def("forward_fused_e2e", &forward_fused_e2e);
+  ops.def("forward_fused_e2e_preranges", &forward_fused_e2e_preranges);
+  ops.def("forward_frame", &forward_frame);
+  ops.def("camera_matrices", &camera_matrices);
+  ops.def("forward_workspace_size",
+          [](int64_t P) { return ps_forward_workspace_size((int)P); });
+  ops.def("preprocess_workspace_size",
+          [](int64_t P) { return ps_preprocess_workspace_size((int)P); });
+}

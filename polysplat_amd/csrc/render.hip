@@ -1,0 +1,433 @@
+// Tile renderers for gfx950: one 64-lane wave per 16x16 tile, 2x2 pixels per
+// lane, front-to-back alpha blending in FP32.
+//
+// Per pixel (render.cu:228-271): d = centre - pixel (integer pixel coords),
+//   power = log2(o) + c.a*dx^2 + c.b*dx*dy + c.c*dy^2  (conic pre-scaled by
+//   -0.5*log2(e), -log2(e), -0.5*log2(e)),  alpha = 2^power,
+//   C += rgb * alpha * T,  T -= alpha * T.
+// The tile stops once every pixel has T < 1e-4; that test runs every
+// kCheck Gaussians.  Output (render.cu:375-394): T < 1e-4 -> 0,
+//   u8 = trunc(clamp((C + T*bg) * 255, 0, 255)); an empty tile writes bg.
+//
+// Features reach the shading loop through LDS: each lane stages one
+// Gaussian of the next 64-Gaussian batch (loads issued a batch ahead, so they
+// overlap the current batch's shading), converting reduced-precision records
+// to FP32 once per Gaussian instead of once per lane; the shading loop then
+// reads wave-uniform LDS addresses (broadcast, no bank conflicts).
+//
+// Scheduling: the plain renderer launches one workgroup per tile
+// (render.cu:4365-4385).  The persistent renderer launches as many one-wave
+// workgroups as fit on the device and pulls tiles from an atomic counter in a
+// caller-owned queue word (render.cu:1717-1864 uses a __device__ global); the
+// last workgroup to finish zeroes the queue, so the next launch needs no
+// reset.
+
+#include "ps_common.h"
+#include "ps_host.h"
+#include "ps_wave.h"
+
+namespace ps {
+namespace {
+
+constexpr int kTile = 16;
+constexpr int kBatch = 64;
+constexpr float kTMin = 0.0001f;
+
+// Range of tile `t` in the sorted keys, found by a cooperative 64-ary search
+// (ceil(log64(n)) dependent loads per bound).
+__device__ __forceinline__ int wave_bound(const uint64_t* keys, int n, uint32_t t, bool upper,
+                                          int lane) {
+  int lo = 0, hi = n;
+  while (hi - lo > kWave) {
+    const int stride = (hi - lo + kWave - 1) / kWave;
+    const int pos = lo + lane * stride;
+    bool below = false;
+    if (pos < hi) {
+      const uint32_t kt = (uint32_t)(keys[pos] >> 32);
+      below = upper ? (kt <= t) : (kt < t);
+    }
+    const int k = popc64(ballot(below));
+    if (k == 0) return lo;
+    const int nlo = lo + (k - 1) * stride + 1;
+    int nhi = lo + k * stride;
+    if (nhi > hi) nhi = hi;
+    lo = nlo;
+    hi = nhi;
+  }
+  const int pos = lo + lane;
+  bool below = false;
+  if (pos < hi) {
+    const uint32_t kt = (uint32_t)(keys[pos] >> 32);
+    below = upper ? (kt <= t) : (kt < t);
+  }
+  return lo + popc64(ballot(below));
+}
+
+// Nine FP32 values per Gaussian as the shading loop wants them.
+struct Feat {
+  float x, y, w, a, b, c, r, g, bl;
+};
+
+template <int PREC>
+__device__ __forceinline__ Feat load_feat(int idx, const float2* __restrict__ xy,
+                                          const float4* __restrict__ rgbd,
+                                          const float4* __restrict__ con,
+                                          const FeatRec16* __restrict__ rec) {
+  Feat f;
+  if (PREC == kFeatF32) {
+    const float2 p = xy[idx];
+    const float4 q = con[idx];
+    const float4 c = rgbd[idx];
+    f.x = p.x; f.y = p.y;
+    f.a = q.x; f.b = q.y; f.c = q.z; f.w = q.w;
+    f.r = c.x; f.g = c.y; f.bl = c.z;
+  } else {
+    const uint2* p = reinterpret_cast<const uint2*>(rec + idx);
+    const uint2 u0 = p[0], u1 = p[1], u2 = p[2];
+    f.x = __uint_as_float(u0.x);
+    f.y = __uint_as_float(u0.y);
+    f.w = __uint_as_float(u1.x);
+    const uint16_t h[6] = {(uint16_t)(u1.y & 0xffffu), (uint16_t)(u1.y >> 16),
+                           (uint16_t)(u2.x & 0xffffu), (uint16_t)(u2.x >> 16),
+                           (uint16_t)(u2.y & 0xffffu), (uint16_t)(u2.y >> 16)};
+    float v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      v[k] = (PREC == kFeatBF16) ? bf16_to_f32(h[k]) : __half2float(__ushort_as_half(h[k]));
+    const float inv = 1.0f / kConicScale;
+    f.a = v[0] * inv; f.b = v[1] * inv; f.c = v[2] * inv;
+    f.r = v[3]; f.g = v[4]; f.bl = v[5];
+  }
+  return f;
+}
+
+__device__ __forceinline__ uint8_t encode(float x) {
+  return (uint8_t)fminf(fmaxf(0.0f, x * 255.0f), 255.0f);
+}
+
+// Shared per-tile body.  Returns nothing; writes the tile's pixels.
+// PERSIST: tile ids come from the queue; RANGES: ranges from tile_ranges,
+// else from a wave search over keys; GATHERED: features indexed by pair
+// position (no id indirection); CHECK: early-exit test cadence; DT: skip
+// pixels whose alpha is below 1/255.
+template <int PREC, bool GATHERED, int CHECK, bool DT>
+__device__ __forceinline__ void render_tile(
+    int tile, int start, int end, int W, int H, int gx, const uint32_t* __restrict__ order,
+    const float2* __restrict__ xy, const float4* __restrict__ rgbd, const float4* __restrict__ con,
+    const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
+    float4 (*sA)[kBatch], float4 (*sB)[kBatch], float (*sC)[kBatch], int lane) {
+  const int tx = tile % gx, ty = tile / gx;
+  const int px0 = tx * kTile + 2 * (lane & 7);
+  const int py0 = ty * kTile + 2 * (lane >> 3);
+  const float pxf0 = (float)px0, pxf1 = (float)(px0 + 1);
+  const float pyf0 = (float)py0, pyf1 = (float)(py0 + 1);
+  float T[4] = {1.f, 1.f, 1.f, 1.f};
+  float Cr[4] = {0.f, 0.f, 0.f, 0.f}, Cg[4] = {0.f, 0.f, 0.f, 0.f}, Cb[4] = {0.f, 0.f, 0.f, 0.f};
+  const int n = end - start;
+  if (n > 0) {
+    // Stage batch 0, prefetch batch 1.
+    Feat pre;
+    if (lane < n) {
+      const int idx = GATHERED ? start + lane : (int)order[start + lane];
+      pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
+    }
+    int cur = 0;
+    sA[0][lane] = make_float4(pre.x, pre.y, pre.w, pre.a);
+    sB[0][lane] = make_float4(pre.b, pre.c, pre.r, pre.g);
+    sC[0][lane] = pre.bl;
+    if (kBatch + lane < n) {
+      const int idx = GATHERED ? start + kBatch + lane : (int)order[start + kBatch + lane];
+      pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
+    }
+    __syncthreads();
+    for (int b0 = 0;; b0 += kBatch) {
+      const int cnt = (n - b0 < kBatch) ? (n - b0) : kBatch;
+      bool done = false;
+      for (int k = 0; k < cnt; ++k) {
+        const float4 A = sA[cur][k];
+        const float4 B = sB[cur][k];
+        const float bl = sC[cur][k];
+        // A = (x, y, w, a), B = (b, c, r, g)
+        const float dx0 = A.x - pxf0, dx1 = A.x - pxf1;
+        const float dy0 = A.y - pyf0, dy1 = A.y - pyf1;
+        const float xs0 = A.w * dx0 * dx0, xs1 = A.w * dx1 * dx1;
+        const float xc0 = B.x * dx0, xc1 = B.x * dx1;
+        const float ys0 = A.z + B.y * dy0 * dy0, ys1 = A.z + B.y * dy1 * dy1;
+        float pw[4];
+        pw[0] = fmaf(xc0, dy0, xs0 + ys0);
+        pw[1] = fmaf(xc1, dy0, xs1 + ys0);
+        pw[2] = fmaf(xc0, dy1, xs0 + ys1);
+        pw[3] = fmaf(xc1, dy1, xs1 + ys1);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          if (DT && pw[p] < -7.994353375f) continue;
+          const float alpha = __builtin_amdgcn_exp2f(pw[p]);
+          const float wgt = alpha * T[p];
+          Cr[p] = fmaf(B.z, wgt, Cr[p]);
+          Cg[p] = fmaf(B.w, wgt, Cg[p]);
+          Cb[p] = fmaf(bl, wgt, Cb[p]);
+          T[p] = T[p] - wgt;
+        }
+        if ((k % CHECK) == CHECK - 1) {
+          const bool sat = T[0] < kTMin && T[1] < kTMin && T[2] < kTMin && T[3] < kTMin;
+          if (ballot(!sat) == 0) {
+            done = true;
+            break;
+          }
+        }
+      }
+      if (done || b0 + kBatch >= n) break;
+      // Publish the prefetched batch, start loading the one after it.
+      __syncthreads();
+      cur ^= 1;
+      sA[cur][lane] = make_float4(pre.x, pre.y, pre.w, pre.a);
+      sB[cur][lane] = make_float4(pre.b, pre.c, pre.r, pre.g);
+      sC[cur][lane] = pre.bl;
+      const int nb = b0 + 2 * kBatch + lane;
+      if (nb < n) {
+        const int idx = GATHERED ? start + nb : (int)order[start + nb];
+        pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int px = px0 + (p & 1), py = py0 + (p >> 1);
+    if (px < W && py < H) {
+      uint8_t* o = out + ((size_t)py * W + px) * 3;
+      if (n > 0) {
+        const float t = T[p] < kTMin ? 0.0f : T[p];
+        o[0] = encode(Cr[p] + t * bg.x);
+        o[1] = encode(Cg[p] + t * bg.y);
+        o[2] = encode(Cb[p] + t * bg.z);
+      } else {
+        o[0] = encode(bg.x);
+        o[1] = encode(bg.y);
+        o[2] = encode(bg.z);
+      }
+    }
+  }
+}
+
+// One workgroup per tile.
+template <int PREC, int CHECK>
+__global__ __launch_bounds__(kWave) void ps_render_plain(
+    const uint64_t* __restrict__ keys, const int* __restrict__ d_n, int n_host, int W, int H,
+    int gx, const uint32_t* __restrict__ order, const float2* __restrict__ xy,
+    const float4* __restrict__ rgbd, const float4* __restrict__ con,
+    const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out) {
+  __shared__ float4 sA[2][kBatch];
+  __shared__ float4 sB[2][kBatch];
+  __shared__ float sC[2][kBatch];
+  const int lane = threadIdx.x;
+  const int tile = blockIdx.y * gx + blockIdx.x;
+  const int n = d_n ? *d_n : n_host;
+  const int start = wave_bound(keys, n, (uint32_t)tile, false, lane);
+  const int end = wave_bound(keys, n, (uint32_t)tile, true, lane);
+  render_tile<PREC, false, CHECK, false>(tile, start, end, W, H, gx, order, xy, rgbd, con, rec, bg,
+                                         out, sA, sB, sC, lane);
+}
+
+// Persistent: tiles from the queue.  RANGES: read tile_ranges, else search.
+template <int PREC, bool RANGES, bool GATHERED, int CHECK, bool DT>
+__global__ __launch_bounds__(kWave) void ps_render_persistent(
+    const uint64_t* __restrict__ keys, const int* __restrict__ d_n, int n_host,
+    const int2* __restrict__ ranges, int W, int H, int gx, int tiles,
+    const uint32_t* __restrict__ order, const float2* __restrict__ xy,
+    const float4* __restrict__ rgbd, const float4* __restrict__ con,
+    const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
+    unsigned* __restrict__ queue) {
+  __shared__ float4 sA[2][kBatch];
+  __shared__ float4 sB[2][kBatch];
+  __shared__ float sC[2][kBatch];
+  const int lane = threadIdx.x;
+  const int n = RANGES ? 0 : (d_n ? *d_n : n_host);
+  int tile = blockIdx.x;
+  while (tile < tiles) {
+    int start, end;
+    if (RANGES) {
+      const int2 r = ranges[tile];
+      start = r.x;
+      end = r.y;
+    } else {
+      start = wave_bound(keys, n, (uint32_t)tile, false, lane);
+      end = wave_bound(keys, n, (uint32_t)tile, true, lane);
+    }
+    render_tile<PREC, GATHERED, CHECK, DT>(tile, start, end, W, H, gx, order, xy, rgbd, con, rec,
+                                           bg, out, sA, sB, sC, lane);
+    int next = 0;
+    if (lane == 0) next = (int)atomicAdd(&queue[0], 1u) + (int)gridDim.x;
+    tile = bcast(next, 0);
+  }
+  // The last workgroup out resets the queue for the next launch.
+  if (lane == 0) {
+    const unsigned done = atomicAdd(&queue[1], 1u);
+    if (done == gridDim.x - 1) {
+      atomicExch(&queue[0], 0u);
+      atomicExch(&queue[1], 0u);
+    }
+  }
+}
+
+struct GridCache {
+  int device = -1;
+  int cus = 0;
+};
+
+int device_cus(hipStream_t) {
+  static thread_local GridCache c;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (c.device != dev) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 0;
+    c.device = dev;
+    c.cus = cus;
+  }
+  return c.cus;
+}
+
+template <typename K>
+int persistent_grid(K kernel, int tiles) {
+  int cus = device_cus(nullptr);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWave, 0) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 8;
+  long long g = (long long)(cus > 0 ? cus : 256) * per_cu;
+  if (g > tiles) g = tiles;
+  return (int)(g < 1 ? 1 : g);
+}
+
+template <int PREC, bool RANGES, bool GATHERED, int CHECK, bool DT>
+int launch_persistent(const uint64_t* keys, const int* d_n, int n_host, const int* ranges, int W,
+                      int H, const uint32_t* order, const float* xy, const float* rgbd,
+                      const float* con, const void* rec, float3 bg, uint8_t* out, unsigned* queue,
+                      hipStream_t stream) {
+  const int gx = (W + kTile - 1) / kTile, gy = (H + kTile - 1) / kTile;
+  const int tiles = gx * gy;
+  auto k = ps_render_persistent<PREC, RANGES, GATHERED, CHECK, DT>;
+  const int grid = persistent_grid(k, tiles);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kWave), 0, stream, keys, d_n, n_host,
+                     reinterpret_cast<const int2*>(ranges), W, H, gx, tiles, order,
+                     (const float2*)xy, (const float4*)rgbd, (const float4*)con,
+                     (const FeatRec16*)rec, bg, out, queue);
+  PS_LAUNCH_CHECK();
+  return PS_OK;
+}
+
+template <int PREC, bool RANGES, bool GATHERED>
+int dispatch_dt(bool dt, const uint64_t* keys, const int* d_n, int n_host, const int* ranges,
+                int W, int H, const uint32_t* order, const float* xy, const float* rgbd,
+                const float* con, const void* rec, float3 bg, uint8_t* out, unsigned* queue,
+                hipStream_t s) {
+  // Cadence of the saturation test: every 8 Gaussians with precomputed
+  // ranges (render.cu:1814), every 16 with the in-kernel search (render.cu:2124).
+  constexpr int kCheck = RANGES ? 8 : 16;
+  if (dt)
+    return launch_persistent<PREC, RANGES, GATHERED, kCheck, true>(keys, d_n, n_host, ranges, W, H,
+                                                                   order, xy, rgbd, con, rec, bg,
+                                                                   out, queue, s);
+  return launch_persistent<PREC, RANGES, GATHERED, kCheck, false>(keys, d_n, n_host, ranges, W, H,
+                                                                  order, xy, rgbd, con, rec, bg,
+                                                                  out, queue, s);
+}
+
+template <int PREC>
+int dispatch_persistent(bool ranges_mode, bool gathered, bool dt, const uint64_t* keys,
+                        const int* d_n, int n_host, const int* ranges, int W, int H,
+                        const uint32_t* order, const float* xy, const float* rgbd,
+                        const float* con, const void* rec, float3 bg, uint8_t* out,
+                        unsigned* queue, hipStream_t s) {
+  if (ranges_mode) {
+    if (gathered)
+      return dispatch_dt<PREC, true, true>(dt, keys, d_n, n_host, ranges, W, H, order, xy, rgbd,
+                                           con, rec, bg, out, queue, s);
+    return dispatch_dt<PREC, true, false>(dt, keys, d_n, n_host, ranges, W, H, order, xy, rgbd,
+                                          con, rec, bg, out, queue, s);
+  }
+  if (gathered)
+    return dispatch_dt<PREC, false, true>(dt, keys, d_n, n_host, ranges, W, H, order, xy, rgbd,
+                                          con, rec, bg, out, queue, s);
+  return dispatch_dt<PREC, false, false>(dt, keys, d_n, n_host, ranges, W, H, order, xy, rgbd, con,
+                                         rec, bg, out, queue, s);
+}
+
+}  // namespace
+
+int render_plain_launch(int n_host, const int* d_n, int W, int H, const float* xy,
+                        const float* rgbd, const float* con, const uint64_t* keys,
+                        const uint32_t* order, const float bg[3], uint8_t* out,
+                        hipStream_t stream) {
+  PS_REQUIRE(W > 0 && H > 0, "bad image size");
+  PS_REQUIRE(out != nullptr && bg != nullptr, "output or background is NULL");
+  PS_REQUIRE(d_n != nullptr || n_host >= 0, "negative num_rendered");
+  const int gx = (W + kTile - 1) / kTile, gy = (H + kTile - 1) / kTile;
+  const float3 b = make_float3(bg[0], bg[1], bg[2]);
+  // Saturation test every 2 Gaussians, as the reference's plain path does
+  // (render.cu:2436-2447).
+  hipLaunchKernelGGL((ps_render_plain<kFeatF32, 2>), dim3(gx, gy), dim3(kWave), 0, stream, keys, d_n,
+                     n_host, W, H, gx, order, (const float2*)xy, (const float4*)rgbd,
+                     (const float4*)con, (const FeatRec16*)nullptr, b, out);
+  PS_LAUNCH_CHECK();
+  return PS_OK;
+}
+
+int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_prec,
+                             const float* xy, const float* rgbd, const float* con, const void* rec,
+                             int gathered, const uint64_t* keys, const uint32_t* order,
+                             const int* ranges, int dt, const float bg[3], uint8_t* out,
+                             unsigned* queue, hipStream_t stream) {
+  PS_REQUIRE(W > 0 && H > 0, "bad image size");
+  PS_REQUIRE(out != nullptr && bg != nullptr && queue != nullptr, "NULL output/bg/queue");
+  PS_REQUIRE(ranges != nullptr || keys != nullptr, "need tile_ranges or keys_sorted");
+  PS_REQUIRE(gathered || order != nullptr, "values_sorted is NULL");
+  PS_REQUIRE(d_n != nullptr || n_host >= 0, "negative num_rendered");
+  const float3 b = make_float3(bg[0], bg[1], bg[2]);
+  const bool rm = ranges != nullptr, g = gathered != 0, d = dt != 0;
+  switch (feat_prec) {
+    case PS_PREC_F32:
+      PS_REQUIRE(xy && rgbd && con, "f32 feature arrays are NULL");
+      return dispatch_persistent<kFeatF32>(rm, g, d, keys, d_n, n_host, ranges, W, H, order, xy,
+                                           rgbd, con, rec, b, out, queue, stream);
+    case PS_PREC_F16:
+      PS_REQUIRE(rec != nullptr, "records are NULL");
+      return dispatch_persistent<kFeatF16>(rm, g, d, keys, d_n, n_host, ranges, W, H, order, xy,
+                                           rgbd, con, rec, b, out, queue, stream);
+    case PS_PREC_BF16:
+      PS_REQUIRE(rec != nullptr, "records are NULL");
+      return dispatch_persistent<kFeatBF16>(rm, g, d, keys, d_n, n_host, ranges, W, H, order, xy,
+                                            rgbd, con, rec, b, out, queue, stream);
+    default:
+      set_last_error("unknown feature precision %d", feat_prec);
+      return PS_ERR_INVALID;
+  }
+}
+
+}  // namespace ps
+
+extern "C" int ps_render_16x16(int num_rendered, const int* d_num_rendered, int width, int height,
+                               const float* points_xy, const float* rgb_depth,
+                               const float* conic_opacity, const uint64_t* keys_sorted,
+                               const uint32_t* values_sorted, const float bg[3],
+                               uint8_t* out_color, ps_stream_t stream) {
+  return ps::render_plain_launch(num_rendered, d_num_rendered, width, height, points_xy, rgb_depth,
+                                 conic_opacity, keys_sorted, values_sorted, bg, out_color,
+                                 reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int ps_render_16x16_persistent(int num_rendered, const int* d_num_rendered, int width,
+                                          int height, int feat_prec, const float* points_xy,
+                                          const float* rgb_depth, const float* conic_opacity,
+                                          const void* records, int gathered,
+                                          const uint64_t* keys_sorted,
+                                          const uint32_t* values_sorted, const int* tile_ranges,
+                                          int dt, const float bg[3], uint8_t* out_color,
+                                          unsigned* queue, ps_stream_t stream) {
+  return ps::render_persistent_launch(num_rendered, d_num_rendered, width, height, feat_prec,
+                                      points_xy, rgb_depth, conic_opacity, records, gathered,
+                                      keys_sorted, values_sorted, tile_ranges, dt, bg, out_color,
+                                      queue, reinterpret_cast<hipStream_t>(stream));
+}

@@ -1,0 +1,285 @@
+"""Scene / Camera / Rasterizer facade with the reference's Python surface.
+
+Mirrors example.py:12-824 for the forward rasterizer variants this build
+implements:
+
+=============================================  ==========================================
+render_variant                                 pipeline
+=============================================  ==========================================
+"default"                                      FP32 SH -> sort -> plain per-tile render
+                                               (forward_fused, pybind.cpp:932)
+"preranges_smem_persistent_lite_e2e"           FP16 SH -> sort -> persistent render with
+                                               in-kernel range search (pybind.cpp:1085)
+"preranges_smem_persistent_lite_e2e_preranges" FP16 SH -> sort -> ranges -> persistent
+                                               render (pybind.cpp:1016)
+"preranges_smem_persistent_lite[_dt]"          stage-by-stage from Python: FP16 SH
+                                               preprocess, host count, sort, ranges,
+                                               persistent render (example.py:480-781)
+"auto"                                         resolves to the fastest of the above
+=============================================  ==========================================
+
+``feature_dtype`` ("fp32", "fp16", "bf16") selects how the persistent
+variants store per-Gaussian render features: FP32 in the reference's three
+arrays, or 24-byte records with FP16/BF16 conic and colour (projection,
+opacity and blending stay FP32).  The "default" variant always uses FP32.
+
+Other variants of the reference (top-k caches, tile reordering, split warps,
+the ES two-pass emission) are not part of this build and raise ValueError.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _C
+
+ops = _C.ops
+
+_PREC = {"fp32": 0, "fp16": 1, "bf16": 2}
+VARIANT_IDS = {
+    "default": 0,
+    "preranges_smem_persistent_lite_e2e_preranges": 1,
+    "preranges_smem_persistent_lite_e2e": 2,
+}
+STAGED = ("preranges_smem_persistent_lite", "preranges_smem_persistent_lite_dt")
+AUTO_CHOICE = "preranges_smem_persistent_lite_e2e_preranges"
+
+
+class Scene:
+    """Gaussians resident on one device (example.py:12-30)."""
+
+    def __init__(self, device):
+        self.device = torch.device(device)
+        self.num_vertex = 0
+        self.position = None
+        self.shs = None
+        self.shs_half = None
+        self.opacity = None
+        self.cov3d = None
+        self._shs_bf16 = None
+
+    def loadPly(self, scene_path):
+        """Loads an INRIA-layout PLY (pybind.cpp:58-120)."""
+        from .ply import load_ply
+
+        n, pos, shs, op, cov = load_ply(scene_path)
+        self.set_arrays(pos, shs, op, cov)
+        print("num_vertex = %d" % self.num_vertex)
+
+    def set_arrays(self, position, shs, opacity, cov3d):
+        self.num_vertex = int(opacity.shape[0])
+        self.position = position.to(self.device, torch.float32).contiguous()
+        self.shs = shs.reshape(self.num_vertex, 48).to(self.device, torch.float32).contiguous()
+        self.shs_half = self.shs.half()
+        self.opacity = opacity.to(self.device, torch.float32).contiguous()
+        self.cov3d = cov3d.to(self.device, torch.float32).contiguous()
+        self._shs_bf16 = None
+        return self
+
+    @property
+    def shs_bf16(self):
+        if self._shs_bf16 is None:
+            self._shs_bf16 = self.shs.to(torch.bfloat16)
+        return self._shs_bf16
+
+    @classmethod
+    def from_arrays(cls, arrays, device):
+        s = cls(device)
+        return s.set_arrays(arrays.position, arrays.shs, arrays.opacity, arrays.cov3d)
+
+
+class Camera:
+    """Camera from a cameras.json entry (example.py:33-44)."""
+
+    def __init__(self, camera_json):
+        self.id = camera_json["id"]
+        self.img_name = camera_json["img_name"]
+        self.width = camera_json["width"]
+        self.height = camera_json["height"]
+        self.position = torch.tensor(camera_json["position"], dtype=torch.float32)
+        self.rotation = torch.tensor(camera_json["rotation"], dtype=torch.float32)
+        self.focal_x = camera_json["fx"]
+        self.focal_y = camera_json["fy"]
+        self.zFar = 100.0
+        self.zNear = 0.01
+
+
+class Rasterizer:
+    """Owns every per-frame device buffer (example.py:56-139); forward()
+    allocates nothing except the output image on a resolution change."""
+
+    def __init__(self, scene, MAX_NUM_RENDERED, MAX_NUM_TILES, enable_es=True,
+                 feature_dtype="fp32"):
+        if feature_dtype not in _PREC:
+            raise ValueError(f"feature_dtype must be one of {sorted(_PREC)}")
+        dev = scene.device
+        self.device = dev
+        self.MAX_NUM_RENDERED = int(MAX_NUM_RENDERED)
+        self.MAX_NUM_TILES = int(MAX_NUM_TILES)
+        self.feature_dtype = feature_dtype
+        self.enable_es = enable_es
+        M = self.MAX_NUM_RENDERED
+        P = scene.num_vertex
+        self.gaussian_keys_unsorted = torch.zeros(M, device=dev, dtype=torch.int64)
+        self.gaussian_values_unsorted = torch.zeros(M, device=dev, dtype=torch.int32)
+        self.gaussian_keys_sorted = torch.zeros(M, device=dev, dtype=torch.int64)
+        self.gaussian_values_sorted = torch.zeros(M, device=dev, dtype=torch.int32)
+        self.SORT_BUFFER_SIZE = ops.get_sort_buffer_size(M)
+        self.list_sorting_space = torch.zeros(self.SORT_BUFFER_SIZE, device=dev, dtype=torch.int8)
+        self.curr_offset = torch.zeros(1, device=dev, dtype=torch.int32)
+        self.points_xy = torch.zeros((P, 2), device=dev, dtype=torch.float32)
+        self.rgb_depth = torch.zeros((P, 4), device=dev, dtype=torch.float32)
+        self.conic_opacity = torch.zeros((P, 4), device=dev, dtype=torch.float32)
+        self.records = torch.zeros(24 * P if feature_dtype != "fp32" else 0, device=dev,
+                                   dtype=torch.uint8)
+        self.tile_ranges_buf = torch.zeros((self.MAX_NUM_TILES, 2), device=dev, dtype=torch.int32)
+        # Zero-filled once: the persistent queue words inside must start at 0
+        # and every launch leaves them at 0 again.
+        self.workspace = torch.zeros(ops.forward_workspace_size(P), device=dev, dtype=torch.uint8)
+        self._empty = torch.zeros(0, device=dev, dtype=torch.uint8)
+        self._out_color = None
+
+    def _get_out_color(self, height, width, device):
+        if (self._out_color is None or self._out_color.shape[0] != height
+                or self._out_color.shape[1] != width):
+            self._out_color = torch.zeros((height, width, 3), device=device, dtype=torch.uint8)
+        return self._out_color
+
+    def _check_tiles(self, camera, block_x, block_y):
+        tiles = ((camera.width + block_x - 1) // block_x) * ((camera.height + block_y - 1) // block_y)
+        if tiles > self.MAX_NUM_TILES:
+            raise ValueError(f"{tiles} tiles exceed MAX_NUM_TILES={self.MAX_NUM_TILES}")
+
+    def _shs_for(self, scene, half):
+        if not half:
+            return scene.shs
+        return scene.shs_bf16 if self.feature_dtype == "bf16" else scene.shs_half
+
+    def forward(self, scene, camera, bg_color, block_x=16, block_y=16, render_variant="default",
+                measure_preprocess=False, measure_sort=False, measure_render=False,
+                return_stats=False, use_es="auto"):
+        requested = render_variant
+        if render_variant == "auto":
+            render_variant = AUTO_CHOICE
+        if use_es is True:
+            raise ValueError("the ES two-pass emission is not part of this build")
+        if block_x != 16 or block_y != 16:
+            raise ValueError("only 16x16 tiles are implemented")
+        measuring = measure_preprocess or measure_sort or measure_render
+        if render_variant.startswith("preranges_smem_persistent_lite_e2e") and measuring:
+            # E2E calls cannot expose per-stage events (example.py:362-368).
+            render_variant = "preranges_smem_persistent_lite"
+        if render_variant not in VARIANT_IDS and render_variant not in STAGED:
+            raise ValueError(f"render_variant {render_variant!r} is not implemented")
+        self._check_tiles(camera, block_x, block_y)
+        out_color = self._get_out_color(camera.height, camera.width, scene.device)
+        times = {"preprocess_ms": None, "sort_ms": None, "render_ms": None}
+        if render_variant in VARIANT_IDS and not (render_variant == "default" and measuring):
+            num_rendered = self._forward_fused(scene, camera, bg_color, render_variant, out_color)
+        else:
+            num_rendered = self._forward_staged(scene, camera, bg_color, render_variant,
+                                                out_color, measure_preprocess, measure_sort,
+                                                measure_render, times)
+        if return_stats:
+            return out_color, {
+                "requested_render_variant": requested,
+                "render_variant": render_variant,
+                "resolved_block_x": block_x,
+                "resolved_block_y": block_y,
+                "num_rendered": num_rendered,
+                "preprocess_ms": times["preprocess_ms"],
+                "sort_ms": times["sort_ms"],
+                "render_ms": times["render_ms"],
+                "used_es": False,
+            }
+        sel = [times[k] for k, f in (("preprocess_ms", measure_preprocess),
+                                     ("sort_ms", measure_sort),
+                                     ("render_ms", measure_render)) if f]
+        if sel:
+            return (out_color, *sel)
+        return out_color
+
+    def _forward_fused(self, scene, camera, bg_color, render_variant, out_color):
+        variant = VARIANT_IDS[render_variant]
+        fp32_path = render_variant == "default"
+        prec = 0 if fp32_path else _PREC[self.feature_dtype]
+        shs = self._shs_for(scene, half=not fp32_path)
+        ranges = self.tile_ranges_buf if variant == 1 else self._empty
+        records = self.records if prec else self._empty
+        return ops.forward_frame(
+            scene.position, shs, scene.opacity, scene.cov3d,
+            camera.width, camera.height, camera.position, camera.rotation,
+            camera.focal_x, camera.focal_y, camera.zFar, camera.zNear,
+            variant, prec, self.curr_offset, self.list_sorting_space,
+            self.gaussian_keys_unsorted, self.gaussian_values_unsorted,
+            self.gaussian_keys_sorted, self.gaussian_values_sorted,
+            self.points_xy, self.rgb_depth, self.conic_opacity, records, ranges,
+            bg_color, out_color, self.workspace, True)
+
+    def _forward_staged(self, scene, camera, bg_color, render_variant, out_color,
+                        m_pre, m_sort, m_render, times):
+        ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+        fp32_path = render_variant == "default"
+        prec = 0 if fp32_path else _PREC[self.feature_dtype]
+        shs = self._shs_for(scene, half=not fp32_path)
+        args_cam = (camera.width, camera.height, 16, 16, camera.position, camera.rotation,
+                    camera.focal_x, camera.focal_y, camera.zFar, camera.zNear)
+        self.curr_offset.fill_(0)
+        e = [ev(), ev(), ev(), ev(), ev(), ev()]
+        if m_pre:
+            e[0].record()
+        if prec == 0:
+            ops.preprocess(scene.position, shs, scene.opacity, scene.cov3d, *args_cam,
+                           self.points_xy, self.rgb_depth, self.conic_opacity,
+                           self.gaussian_keys_unsorted, self.gaussian_values_unsorted,
+                           self.curr_offset)
+        else:
+            ops.preprocess_records(scene.position, shs, scene.opacity, scene.cov3d, *args_cam,
+                                   self.records, prec, self.gaussian_keys_unsorted,
+                                   self.gaussian_values_unsorted, self.curr_offset)
+        if m_pre:
+            e[1].record()
+        num_rendered = int(self.curr_offset.cpu()[0])
+        if num_rendered > self.MAX_NUM_RENDERED:
+            raise RuntimeError("Too many k-v pairs!")
+        if m_sort:
+            e[2].record()
+        ops.sort_gaussian(num_rendered, camera.width, camera.height, 16, 16,
+                          self.list_sorting_space, self.gaussian_keys_unsorted,
+                          self.gaussian_values_unsorted, self.gaussian_keys_sorted,
+                          self.gaussian_values_sorted)
+        if m_sort:
+            e[3].record()
+        if not fp32_path:
+            ops.precompute_tile_ranges(num_rendered, camera.width, camera.height, 16, 16,
+                                       self.gaussian_keys_sorted, self.tile_ranges_buf)
+        if m_render:
+            e[4].record()
+        if fp32_path:
+            ops.render_16x16(num_rendered, camera.width, camera.height, self.points_xy,
+                             self.rgb_depth, self.conic_opacity, self.gaussian_keys_sorted,
+                             self.gaussian_values_sorted, bg_color, out_color)
+        elif prec:
+            ops.render_records(num_rendered, camera.width, camera.height, self.records, prec,
+                               False, self.gaussian_values_sorted, self.tile_ranges_buf,
+                               bg_color, out_color)
+        elif render_variant.endswith("_dt"):
+            ops.render_16x16_preranges_smem_persistent_lite_dt(
+                num_rendered, camera.width, camera.height, self.points_xy, self.rgb_depth,
+                self.conic_opacity, self.gaussian_values_sorted, self.tile_ranges_buf,
+                bg_color, out_color)
+        else:
+            ops.render_16x16_preranges_smem_persistent_lite(
+                num_rendered, camera.width, camera.height, self.points_xy, self.rgb_depth,
+                self.conic_opacity, self.gaussian_values_sorted, self.tile_ranges_buf,
+                bg_color, out_color)
+        if m_render:
+            e[5].record()
+        if m_pre or m_sort or m_render:
+            torch.cuda.synchronize()
+        if m_pre:
+            times["preprocess_ms"] = e[0].elapsed_time(e[1])
+        if m_sort:
+            times["sort_ms"] = e[2].elapsed_time(e[3])
+        if m_render:
+            times["render_ms"] = e[4].elapsed_time(e[5])
+        return num_rendered
