@@ -102,10 +102,17 @@ def build_pybind() -> Path:
     out = PKG / f"_C{suffix}"
     tmp = out.with_name(out.name + ".tmp")
     BUILD.mkdir(exist_ok=True)
-    obj = BUILD / "ops_bind.o"
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-variable", *cflags,
-          f"-I{ROOT / 'include'}", "-c", str(CSRC / "ops_bind.cpp"), "-o", str(obj)])
-    _run(["g++", "-shared", "-fPIC", str(obj), "-o", str(tmp), *libs])
+
+    def cc(src: str) -> Path:
+        obj = BUILD / (Path(src).stem + ".o")
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-variable",
+              "-ffp-contract=off", *cflags, f"-I{ROOT / 'include'}", "-c", str(CSRC / src), "-o",
+              str(obj)])
+        return obj
+
+    with ThreadPoolExecutor(max_workers=2) as ex:
+        objs = list(ex.map(cc, ["ops_bind.cpp", "ply.cpp"]))
+    _run(["g++", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp), *libs])
     os.replace(tmp, out)
     return out
 

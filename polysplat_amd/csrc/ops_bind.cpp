@@ -17,6 +17,11 @@
 
 #include "polysplat_amd.h"
 
+namespace ps_ply {
+std::tuple<int64_t, at::Tensor, at::Tensor, at::Tensor, at::Tensor> load_ply(
+    const std::string& name);
+}
+
 namespace {
 
 using at::Tensor;
@@ -481,6 +486,7 @@ PYBIND11_MODULE(_C, m) {
   m.doc() = "MI355X polysplat forward rasterizer (HIP, gfx950)";
   m.def("version", []() { return std::string(ps_version()); });
   auto ops = m.def_submodule("ops", "polysplat.ops-compatible operators");
+  ops.def("loadPly", &ps_ply::load_ply);
   ops.def("preprocess", &preprocess_impl);
   ops.def("preprocess_half_sh", &preprocess_impl);
   ops.def("preprocess_records", &preprocess_records);

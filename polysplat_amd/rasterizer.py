@@ -59,9 +59,7 @@ class Scene:
 
     def loadPly(self, scene_path):
         """Loads an INRIA-layout PLY (pybind.cpp:58-120)."""
-        from .ply import load_ply
-
-        n, pos, shs, op, cov = load_ply(scene_path)
+        n, pos, shs, op, cov = ops.loadPly(scene_path)
         self.set_arrays(pos, shs, op, cov)
         print("num_vertex = %d" % self.num_vertex)
 
