@@ -45,13 +45,17 @@ def case(request, gpu):
     return name, scene, spec, ref
 
 
+def _cap(ref):
+    return ref["pre"]["M"] + 4096
+
+
 def _referenced(ref):
     return np.unique(ref["pre"]["vals"])
 
 
 def test_preprocess_keys_values_bit_exact(case):
     name, scene, spec, ref = case
-    g = Gpu(scene, spec)
+    g = Gpu(scene, spec, capacity=_cap(ref))
     say("preprocess", name)
     M = g.preprocess()
     assert M == ref["pre"]["M"]
@@ -76,7 +80,7 @@ def test_routes_are_exercised(case):
 
 def test_sort_and_ranges_bit_exact(case):
     name, scene, spec, ref = case
-    g = Gpu(scene, spec)
+    g = Gpu(scene, spec, capacity=_cap(ref))
     g.preprocess()
     say("sort", name, g.M)
     g.sort()
@@ -92,7 +96,7 @@ def test_sort_and_ranges_bit_exact(case):
 @pytest.mark.parametrize("mode", ["plain", "persistent", "fused", "dt"])
 def test_render_fp32_parity(case, mode):
     name, scene, spec, ref = case
-    g = Gpu(scene, spec)
+    g = Gpu(scene, spec, capacity=_cap(ref))
     g.preprocess()
     g.sort()
     g.ranges_bsearch()
@@ -123,7 +127,7 @@ def test_render_fp32_parity(case, mode):
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
 def test_render_reduced_precision_features(case, prec):
     name, scene, spec, ref = case
-    g = Gpu(scene, spec, sh_type=prec, feat_prec=prec)
+    g = Gpu(scene, spec, sh_type=prec, feat_prec=prec, capacity=_cap(ref))
     g.preprocess()
     g.sort()
     g.ranges_bsearch()
@@ -139,7 +143,7 @@ def test_render_reduced_precision_features(case, prec):
 
 def test_forward_pipeline_matches_stages(case):
     name, scene, spec, ref = case
-    g = Gpu(scene, spec)
+    g = Gpu(scene, spec, capacity=_cap(ref))
     for variant, cad in ((capi.V_DEFAULT, 2), (capi.V_LITE, 8), (capi.V_FUSED, 16)):
         say("forward", name, variant)
         st, n, img = g.forward(BG, variant)
@@ -216,3 +220,27 @@ def test_all_gaussians_behind_camera(gpu):
     capi.check(st)
     assert n == 0
     assert (img[..., 0] == 255).all() and (img[..., 1] == 127).all() and (img[..., 2] == 0).all()
+
+
+def test_c2_full_size_bit_exact(gpu):
+    """Config 2 at full size: 3M Gaussians, 1920x1080, FP32."""
+    scene = scenes.make_config_scene("c2", seed=1)
+    spec = scenes.config_camera("c2")
+    say("c2 oracle")
+    ref = oracle_frame(scene, spec, bg=BG, check_every=8, fp64=True)
+    M = ref["pre"]["M"]
+    g = Gpu(scene, spec, capacity=M + 4096)
+    say("c2 gpu", M)
+    assert g.preprocess() == M
+    np.testing.assert_array_equal(g.ku[:M].cpu().numpy().view(np.uint64), ref["pre"]["keys"])
+    np.testing.assert_array_equal(g.vu[:M].cpu().numpy().view(np.uint32), ref["pre"]["vals"])
+    g.sort()
+    np.testing.assert_array_equal(g.ks[:M].cpu().numpy().view(np.uint64), ref["keys_sorted"])
+    np.testing.assert_array_equal(g.vs[:M].cpu().numpy().view(np.uint32), ref["vals_sorted"])
+    np.testing.assert_array_equal(g.ranges_bsearch(), ref["ranges"])
+    img = g.render_persistent(BG, use_ranges=True)
+    m32 = diff_metrics(img, ref["image"])
+    m64 = diff_metrics(img, ref["image64"])
+    say("c2 vs fp32", m32, "vs fp64", m64)
+    assert m32["max_abs_diff"] <= 1, m32
+    assert m64["max_abs_diff"] <= 1 and m64["psnr_db"] >= 50.0, m64
