@@ -41,11 +41,32 @@ def parse():
     ap.add_argument("--config", default="c3")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--paths", default="fp16_persistent,fp32_plain",
+                    help="comma-separated subset of fp16_persistent,fp32_plain")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe():
+    """A plain device-to-device copy of 1 GiB, five times: the reference point
+    for scaling FETCH_SIZE / WRITE_SIZE counter readings to bytes."""
+    import torch
+
+    n = 1 << 30
+    a = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n, "launches": 5}), flush=True)
 
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the rasterizer runs each frame on a single device)", file=sys.stderr)
@@ -76,10 +97,12 @@ def main():
         return polysplat.Rasterizer(scene, MAX_NUM_RENDERED=cap, MAX_NUM_TILES=T,
                                     feature_dtype=feat)
 
-    paths = {
-        "fp16_persistent": (make_rast("fp16"), "preranges_smem_persistent_lite_e2e_preranges"),
-        "fp32_plain": (make_rast("fp32"), "default"),
-    }
+    wanted = [p for p in args.paths.split(",") if p]
+    paths = {}
+    if "fp16_persistent" in wanted:
+        paths["fp16_persistent"] = (make_rast("fp16"), "preranges_smem_persistent_lite_e2e_preranges")
+    if "fp32_plain" in wanted:
+        paths["fp32_plain"] = (make_rast("fp32"), "default")
     VAR = polysplat.rasterizer.VARIANT_IDS
     PREC = {"fp16": 1, "fp32": 0, "bf16": 2}
 
@@ -119,6 +142,9 @@ def main():
         results[name] = {"ms_per_frame": ms, "num_rendered": int(M)}
         log(f"{name}: {ms:.3f} ms/frame, M={M}")
 
+    if "fp16_persistent" not in paths:
+        print(json.dumps({k: v for k, v in results.items()}), flush=True)
+        return
     # ---------------------------------------------------------- per-stage timing
     rast, _ = paths["fp16_persistent"]
     M = results["fp16_persistent"]["num_rendered"]
@@ -186,9 +212,10 @@ def main():
                         fp64=True)
         log(f"oracle fp32+fp64 frame: {time.perf_counter() - t:.1f}s")
         quality["fp16_persistent"] = diff_metrics(images["fp16_persistent"], ref32["image64"])
-        quality["fp32_plain"] = diff_metrics(images["fp32_plain"], ref32["image64"])
         quality["fp16_vs_oracle_fp16"] = diff_metrics(images["fp16_persistent"], ref16["image"])
-        quality["fp32_vs_oracle_fp32"] = diff_metrics(images["fp32_plain"], ref32["image"])
+        if "fp32_plain" in images:
+            quality["fp32_plain"] = diff_metrics(images["fp32_plain"], ref32["image64"])
+            quality["fp32_vs_oracle_fp32"] = diff_metrics(images["fp32_plain"], ref32["image"])
         assert ref16["pre"]["M"] == M, (ref16["pre"]["M"], M)
 
     # ---------------------------------------------------------- roofline
@@ -218,7 +245,7 @@ def main():
         roof["render_pixel_gaussians_per_s"] = px_evals / (stage_ms[3] * 1e-3)
 
     fp16 = results["fp16_persistent"]["ms_per_frame"]
-    fp32 = results["fp32_plain"]["ms_per_frame"]
+    fp32 = results.get("fp32_plain", {}).get("ms_per_frame", float("nan"))
     line = {
         "metric": METRIC,
         "value": round(fp16, 4),

@@ -111,11 +111,6 @@ int ps_sort_gaussian(int num_rendered, const int* d_num_rendered, int max_pairs,
                      const uint64_t* keys_unsorted, const uint32_t* values_unsorted,
                      uint64_t* keys_sorted, uint32_t* values_sorted, ps_stream_t stream);
 
-/* Device word that the sort sets non-zero if one of its bounded cross-tile
- * waits gave up (the output is then not sorted; nothing is written out of
- * bounds).  Valid for the sort space/max_pairs pair used in the call. */
-const unsigned* ps_sort_error_ptr(const void* sort_space, int max_pairs);
-
 /* Copies *d_curr_offset to the host through pinned memory and waits for it.
  * Replaces fetch_num_rendered_async (sort.cu:289-300, ops.h:103). */
 int ps_fetch_num_rendered(const int* d_curr_offset, int* out_num_rendered, ps_stream_t stream);

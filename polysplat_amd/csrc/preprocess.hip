@@ -321,9 +321,19 @@ __global__ __launch_bounds__(1024) void ps_pre_scan(int nblocks, const int* __re
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int per = (nblocks + 1023) / 1024;
   const int b0 = tid * per;
+  // Each thread sums a contiguous run of block sums, 8 loads in flight at a
+  // time (the scan is latency-bound, not bandwidth-bound).
   int local = 0;
-  for (int k = 0; k < per; ++k)
-    if (b0 + k < nblocks) local += block_sums[b0 + k];
+  for (int k0 = 0; k0 < per; k0 += 8) {
+    int v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int idx = b0 + k0 + j;
+      v[j] = (k0 + j < per && idx < nblocks) ? block_sums[idx] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) local += v[j];
+  }
   int incl = wave_incl_scan(local, lane);
   if (lane == 63) wsum[wave] = incl;
   __syncthreads();
@@ -334,10 +344,18 @@ __global__ __launch_bounds__(1024) void ps_pre_scan(int nblocks, const int* __re
     total += v;
   }
   int run = before + incl - local;
-  for (int k = 0; k < per; ++k) {
-    if (b0 + k < nblocks) {
-      block_offsets[b0 + k] = run;
-      run += block_sums[b0 + k];
+  for (int k0 = 0; k0 < per; k0 += 8) {
+    int v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int idx = b0 + k0 + j;
+      v[j] = (k0 + j < per && idx < nblocks) ? block_sums[idx] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int idx = b0 + k0 + j;
+      if (k0 + j < per && idx < nblocks) block_offsets[idx] = run;
+      run += v[j];
     }
   }
   if (tid == 0) {

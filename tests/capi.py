@@ -63,7 +63,6 @@ def lib():
             "ps_preprocess_count_ptr": ([P], P),
             "ps_preprocess_status_ptr": ([P], P),
             "ps_get_sort_buffer_size": ([I], S),
-            "ps_sort_error_ptr": ([P, I], P),
             "ps_sort_gaussian": ([I, P, I, I, I, I, I, P, S, P, P, P, P, P], I),
             "ps_fetch_num_rendered": ([P, P, P], I),
             "ps_precompute_tile_ranges": ([I, P, I, I, I, I, P, P, P], I),

@@ -2,7 +2,6 @@
 heavy ties (stability), every tile-boundary size, device-resident counts,
 and the pass counts of every configuration's key width."""
 import ctypes as C
-import os
 
 import numpy as np
 import pytest
@@ -14,18 +13,6 @@ from tests.gpu_helpers import say
 
 pytestmark = pytest.mark.gpu
 
-ALGOS = ["onesweep", "reduce_scan"]
-
-
-@pytest.fixture(params=ALGOS)
-def algo(request):
-    old = os.environ.get("POLYSPLAT_SORT")
-    os.environ["POLYSPLAT_SORT"] = request.param
-    yield request.param
-    if old is None:
-        os.environ.pop("POLYSPLAT_SORT", None)
-    else:
-        os.environ["POLYSPLAT_SORT"] = old
 
 
 def _keys(n, tiles, depth_levels, rng):
@@ -54,11 +41,6 @@ def _run_sort(keys, vals, W, H, device_count=False, cap=None):
                                   cap, W, H, 16, 16, capi.ptr(space), nbytes, capi.ptr(ku),
                                   capi.ptr(vu), capi.ptr(ks), capi.ptr(vs), capi.stream()))
     torch.cuda.synchronize()
-    err = C.cast(L.ps_sort_error_ptr(capi.ptr(space), cap), C.c_void_p).value
-    # the error word lives inside `space`; read it through a device-to-device view
-    off = err - space.data_ptr()
-    assert 0 <= off < nbytes
-    assert int(space[off:off + 4].view(torch.int32).item()) == 0, "sort gave up waiting"
     out_k = ks.cpu().numpy().view(np.uint64)
     out_v = vs.cpu().numpy().view(np.uint32)
     # inputs preserved
@@ -67,11 +49,11 @@ def _run_sort(keys, vals, W, H, device_count=False, cap=None):
 
 
 @pytest.mark.parametrize("n", [0, 1, 2, 255, 2047, 2048, 2049, 4096 * 3 + 17, 100_000])
-def test_sort_sizes_with_ties(gpu, algo, n):
+def test_sort_sizes_with_ties(gpu, n):
     rng = np.random.default_rng(n)
     keys = _keys(n, 40, 3, rng)  # very few distinct keys: stability is visible
     vals = rng.permutation(max(n, 1))[:n].astype(np.uint32)
-    say("sort n", n, algo)
+    say("sort n", n)
     k, v = _run_sort(keys, vals, 160, 64)
     rk, rv = O.sort_pairs(keys, vals)
     np.testing.assert_array_equal(k[:n], rk)
@@ -81,7 +63,7 @@ def test_sort_sizes_with_ties(gpu, algo, n):
 
 
 @pytest.mark.parametrize("W,H", [(800, 800), (1920, 1080), (3840, 2160), (9216, 6912)])
-def test_sort_key_widths(gpu, algo, W, H):
+def test_sort_key_widths(gpu, W, H):
     tiles = ((W + 15) // 16) * ((H + 15) // 16)
     rng = np.random.default_rng(W)
     n = 300_000
@@ -94,7 +76,7 @@ def test_sort_key_widths(gpu, algo, W, H):
     np.testing.assert_array_equal(v[:n], rv)
 
 
-def test_sort_large_device_count_properties(gpu, algo):
+def test_sort_large_device_count_properties(gpu):
     # 12M pairs: checked by properties (sorted, permutation, stability of the
     # value order within equal keys) instead of a full oracle run.
     n = 12_000_000
@@ -111,7 +93,7 @@ def test_sort_large_device_count_properties(gpu, algo):
     nbytes = L.ps_get_sort_buffer_size(n)
     space = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     dn = torch.tensor([n], dtype=torch.int32, device=dev)
-    say("sort large", n, algo)
+    say("sort large", n)
     capi.check(L.ps_sort_gaussian(0, capi.ptr(dn), n, 3840, 2160, 16, 16, capi.ptr(space), nbytes,
                                   capi.ptr(ku), capi.ptr(vu), capi.ptr(ks), capi.ptr(vs),
                                   capi.stream()))
