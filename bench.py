@@ -45,6 +45,10 @@ def parse():
                     help="comma-separated subset of fp16_persistent,fp32_plain")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
+    ap.add_argument("--serve-requests", type=int, default=10000,
+                    help="requests in the 240 req/s serving stream (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=240.0)
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving stream")
     return ap.parse_args()
 
 
@@ -62,10 +66,59 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes": n, "launches": 5}), flush=True)
 
 
+def serve(args):
+    """Serving stream: 4 resident scenes (1M, 3M, 10M, 30M Gaussians, FP16 SH
+    and features), Poisson requests at --serve-rate with orbit poses and
+    resolutions drawn from 1280x720 .. 7680x4320, continuous batching."""
+    import torch
+
+    from polysplat_amd import serving as SV
+
+    t = time.time()
+    sc = SV.make_resident_scenes(SV.SCENE_SIZES, device="cuda:0")
+    torch.cuda.synchronize()
+    log(f"serving: {len(sc)} resident scenes generated in {time.time() - t:.1f}s")
+    ex = SV.GpuExecutor(sc, slots=2)
+    est = SV.Estimates()
+    t = time.time()
+    SV.calibrate(ex, est, len(sc))
+    log(f"serving: calibrated {len(est.pairs)} (scene, resolution) pairs in {time.time() - t:.1f}s")
+    for k in sorted(est.pairs):
+        log(f"  scene {SV.SCENE_SIZES[k[0]]:>9d} {k[1]}x{k[2]}: pairs {est.pairs[k]:>10d}, "
+            f"{est.ms[k]:.3f} ms alone")
+    sched = SV.Scheduler(SV.SCENE_SIZES, 48 << 30, est, max_frames=8, max_batch_ms=6.0)
+    reqs = SV.poisson_requests(args.serve_requests, args.serve_rate, len(sc))
+    loop = SV.ServingLoop(sched, ex)
+    t = time.perf_counter()
+    done = loop.run(reqs)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    res = SV.summarize(done, sched.rejected, wall, batches=loop.batches)
+    res["overflow_retries"] = loop.overflows
+    res["rate_req_s"] = args.serve_rate
+    res["scenes"] = list(SV.SCENE_SIZES)
+    res["resolutions"] = [f"{w}x{h}" for w, h in SV.RESOLUTIONS]
+    by_res = {}
+    for w, h in SV.RESOLUTIONS:
+        lat = [r.latency_ms for r in done if (r.width, r.height) == (w, h)]
+        if lat:
+            import numpy as np
+
+            by_res[f"{w}x{h}"] = {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+                                  "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+    res["by_resolution"] = by_res
+    res = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res.items()}
+    log(f"serving: {res}")
+    return res
+
+
 def main():
     args = parse()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.serve_only:
+        print(json.dumps({"serving": serve(args)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -274,6 +327,10 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if args.serve_requests > 0:
+        del paths, rast, scene, arrays
+        torch.cuda.empty_cache()
+        line["serving"] = serve(args)
     print(json.dumps(line), flush=True)
 
 

@@ -203,6 +203,55 @@ int ps_forward(int P, const float* positions, const void* shs, int sh_prec,
                int variant, int feat_prec, const ps_frame_buffers* fb, const float bg[3],
                uint8_t* out_color, int sync_count, int* num_rendered_out, ps_stream_t stream);
 
+/* ---------------------------------------------------------------- batch
+ * Several frames, each with its own scene and camera, rendered through
+ * shared launches: one preprocess (count, scan, emit) over all frames' Gaussians,
+ * one sort of all pairs (tile ids are offset per frame so that frames occupy
+ * consecutive tile ranges), one range build and one persistent render whose
+ * tile queue spans every frame.  Per frame the result equals ps_forward with
+ * PS_VARIANT_PERSISTENT_LITE: keys, order and images are identical.
+ * The reference renders one frame per call (forward_fused_e2e_preranges,
+ * pybind.cpp:1016-1080) and has no batched form; this entry point serves
+ * callers that render concurrent camera requests (a serving loop).
+ *
+ * Shared buffers come from fb: keys/values/sort_space sized for the sum of
+ * the frames' pairs (capacity), tile_ranges for the sum of their tiles,
+ * curr_offset receives the total pair count, workspace holds
+ * ps_forward_batch_workspace_size(num_frames, P) bytes; fb's feature
+ * pointers are ignored.  Per-frame features and images come from
+ * ps_frame.  All frames share sh_prec and feat_prec.  The frame descriptors
+ * are staged through pinned memory owned by the library, so the call only
+ * enqueues work (it blocks only when 16 earlier batches of the same host
+ * thread are still waiting for their descriptor copy).  With sync_count != 0
+ * the call waits, writes each frame's pair count to frame_counts (host, may
+ * be NULL) and returns PS_ERR_OVERFLOW when the pairs did not fit (then no
+ * keys were written and every image shows its background). */
+#define PS_MAX_BATCH_FRAMES 64
+
+typedef struct ps_frame {
+  int P;
+  const float* positions;         /* f32[P][3] */
+  const void* shs;                /* [P][48] in sh_prec */
+  const float* opacities;         /* f32[P] */
+  const float* cov3d;             /* f32[P][6] */
+  ps_camera cam;                  /* 16x16 tiles */
+  float bg[3];
+  float* points_xy;               /* f32 features [P] (F32 mode) */
+  float* rgb_depth;
+  float* conic_opacity;
+  void* records;                  /* 24-byte records [P] (F16/BF16 mode) */
+  uint8_t* out_color;             /* u8[height][width][3] */
+} ps_frame;
+
+size_t ps_forward_batch_workspace_size(int num_frames, const int* P);
+int ps_forward_batch(int num_frames, const ps_frame* frames, int sh_prec, int feat_prec,
+                     const ps_frame_buffers* fb, int sync_count, int* frame_counts,
+                     ps_stream_t stream);
+/* Device pointer to the per-frame pair counts a batch leaves in its
+ * workspace (int[num_frames], valid once the batch has run), for callers
+ * that read them back asynchronously. */
+const int* ps_forward_batch_counts_ptr(const void* workspace, int num_frames, const int* P);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,9 +36,25 @@ int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_
                              const int* ranges, int dt, const float bg[3], uint8_t* out,
                              unsigned* queue, hipStream_t stream);
 
+FrameDev make_frame_dev(int P, const float* positions, const void* shs, const float* opacities,
+                        const float* cov3d, const ps_camera* cam, float* xy, float* rgbd,
+                        float* conic, void* rec, int rec_prec, uint8_t* out, const float* bg);
+int check_frame(const FrameDev& F, const ps_camera* cam);
+size_t preprocess_batch_workspace_size(int nf, const int* P);
+int preprocess_batch_launch(int nf, FrameDev* frames_host, int sh_prec, uint64_t* keys,
+                            uint32_t* values, int capacity, int* total_out, void* workspace,
+                            size_t workspace_bytes, int** frame_counts_dev, const int** state_count,
+                            int* tiles_out, hipStream_t stream);
+int render_batch_launch(const FrameDev* frames_dev, int nf, int feat_prec, const int* ranges,
+                        int tiles, const uint32_t* order, unsigned* queue, hipStream_t stream);
+const FrameDev* batch_frames_dev(int nf, const FrameDev* frames_host, void* workspace);
+int* batch_counts_dev(int nf, const int* P, void* workspace);
+
 namespace {
 
 thread_local char g_last_error[512] = "";
+
+constexpr int kPinnedInts = 256;
 
 // Pinned slot for count readbacks (the reference keeps the same kind of slot
 // per thread, sort.cu:39-69).
@@ -47,7 +63,8 @@ struct PinnedSlot {
   hipEvent_t ev = nullptr;
   bool ok = false;
   PinnedSlot() {
-    if (hipHostMalloc(reinterpret_cast<void**>(&host), 4 * sizeof(int), hipHostMallocDefault) !=
+    if (hipHostMalloc(reinterpret_cast<void**>(&host), kPinnedInts * sizeof(int),
+                      hipHostMallocDefault) !=
         hipSuccess) {
       host = nullptr;
       return;
@@ -80,20 +97,40 @@ void set_last_error(const char* fmt, ...) {
   va_end(ap);
 }
 
-// Copies n ints from device to the pinned slot and waits.
-static int read_back(const int* const* ptrs, int n, int* out, hipStream_t stream) {
+// Copies spans of device ints (ptrs[i], lens[i]) into `out`, back to back,
+// through the pinned slot, and waits.
+static int read_back_spans(const int* const* ptrs, const int* lens, int n, int* out,
+                           hipStream_t stream) {
   PinnedSlot& s = pinned();
-  if (!s.ok) {
-    for (int i = 0; i < n; ++i)
-      PS_HIP_CHECK(hipMemcpy(out + i, ptrs[i], sizeof(int), hipMemcpyDeviceToHost));
+  int total = 0;
+  for (int i = 0; i < n; ++i) total += lens[i];
+  if (!s.ok || total > kPinnedInts) {
+    int o = 0;
+    for (int i = 0; i < n; ++i) {
+      PS_HIP_CHECK(hipMemcpyAsync(out + o, ptrs[i], lens[i] * sizeof(int), hipMemcpyDeviceToHost,
+                                  stream));
+      o += lens[i];
+    }
+    PS_HIP_CHECK(hipStreamSynchronize(stream));
     return PS_OK;
   }
-  for (int i = 0; i < n; ++i)
-    PS_HIP_CHECK(hipMemcpyAsync(s.host + i, ptrs[i], sizeof(int), hipMemcpyDeviceToHost, stream));
+  int o = 0;
+  for (int i = 0; i < n; ++i) {
+    PS_HIP_CHECK(hipMemcpyAsync(s.host + o, ptrs[i], lens[i] * sizeof(int), hipMemcpyDeviceToHost,
+                                stream));
+    o += lens[i];
+  }
   PS_HIP_CHECK(hipEventRecord(s.ev, stream));
   PS_HIP_CHECK(hipEventSynchronize(s.ev));
-  for (int i = 0; i < n; ++i) out[i] = s.host[i];
+  for (int i = 0; i < total; ++i) out[i] = s.host[i];
   return PS_OK;
+}
+
+// Copies n single ints from device memory and waits.
+static int read_back(const int* const* ptrs, int n, int* out, hipStream_t stream) {
+  int lens[4] = {1, 1, 1, 1};
+  if (n > 4) return PS_ERR_INVALID;
+  return read_back_spans(ptrs, lens, n, out, stream);
 }
 
 }  // namespace ps
@@ -128,7 +165,7 @@ extern "C" int ps_fetch_num_rendered(const int* d_curr_offset, int* out_num_rend
 }
 
 extern "C" size_t ps_forward_workspace_size(int P) {
-  return ps::align_up(ps_preprocess_workspace_size(P), 256) + ps::kQueueBytes;
+  return ps::kQueueBytes + ps::align_up(ps_preprocess_workspace_size(P), 256);
 }
 
 extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh_prec,
@@ -155,9 +192,11 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
       variant == PS_VARIANT_PERSISTENT_LITE || variant == PS_VARIANT_PERSISTENT_DT;
   PS_REQUIRE(!need_ranges || fb->tile_ranges != nullptr, "tile_ranges is NULL");
 
-  char* ws = static_cast<char*>(fb->workspace);
+  // The queue words sit at the start of the workspace, where no other stage
+  // writes, so a workspace reused for scenes of other sizes keeps them zero.
+  unsigned* queue = static_cast<unsigned*>(fb->workspace);
+  char* ws = static_cast<char*>(fb->workspace) + kQueueBytes;
   const size_t pre_bytes = align_up(ps_preprocess_workspace_size(P), 256);
-  unsigned* queue = reinterpret_cast<unsigned*>(ws + pre_bytes);
   int st;
   // The frame's keys start at offset 0 (the reference clears curr_offset
   // first, pybind.cpp:946); the scan kernel takes 0 as the base directly.
@@ -197,6 +236,89 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
     st = read_back(ptrs, 2, vals, stream);
     if (st != PS_OK) return st;
     if (num_rendered_out) *num_rendered_out = vals[0];
+    if (vals[1] & 1) {
+      set_last_error("%d pairs do not fit in %d", vals[0], fb->capacity);
+      return PS_ERR_OVERFLOW;
+    }
+  }
+  return PS_OK;
+}
+
+extern "C" size_t ps_forward_batch_workspace_size(int num_frames, const int* P) {
+  if (num_frames <= 0 || P == nullptr) return 0;
+  return ps::align_up(ps::preprocess_batch_workspace_size(num_frames, P), 256) + ps::kQueueBytes;
+}
+
+extern "C" const int* ps_forward_batch_counts_ptr(const void* workspace, int num_frames,
+                                                  const int* P) {
+  if (workspace == nullptr || P == nullptr || num_frames <= 0) return nullptr;
+  return ps::batch_counts_dev(num_frames, P,
+                              static_cast<char*>(const_cast<void*>(workspace)) + ps::kQueueBytes);
+}
+
+extern "C" int ps_forward_batch(int num_frames, const ps_frame* frames, int sh_prec, int feat_prec,
+                                const ps_frame_buffers* fb, int sync_count, int* frame_counts,
+                                ps_stream_t stream_) {
+  using namespace ps;
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+  PS_REQUIRE(num_frames > 0 && num_frames <= PS_MAX_BATCH_FRAMES,
+             "batch of %d frames (allowed 1..%d)", num_frames, PS_MAX_BATCH_FRAMES);
+  PS_REQUIRE(frames != nullptr && fb != nullptr, "NULL argument");
+  PS_REQUIRE(feat_prec == PS_PREC_F32 || feat_prec == PS_PREC_F16 || feat_prec == PS_PREC_BF16,
+             "unknown feature precision %d", feat_prec);
+  PS_REQUIRE(fb->curr_offset != nullptr && fb->tile_ranges != nullptr, "shared buffers are NULL");
+  const bool f32 = feat_prec == PS_PREC_F32;
+  FrameDev host[PS_MAX_BATCH_FRAMES];
+  int P[PS_MAX_BATCH_FRAMES];
+  for (int f = 0; f < num_frames; ++f) {
+    const ps_frame& q = frames[f];
+    PS_REQUIRE(q.cam.block_x == 16 && q.cam.block_y == 16,
+               "frame %d: the renderers implement 16x16 tiles only", f);
+    PS_REQUIRE(q.out_color != nullptr, "frame %d: out_color is NULL", f);
+    PS_REQUIRE(f32 ? (q.points_xy && q.rgb_depth && q.conic_opacity) : q.records != nullptr,
+               "frame %d: feature buffers for the chosen precision are NULL", f);
+    host[f] = make_frame_dev(q.P, q.positions, q.shs, q.opacities, q.cov3d, &q.cam,
+                             f32 ? q.points_xy : nullptr, f32 ? q.rgb_depth : nullptr,
+                             f32 ? q.conic_opacity : nullptr, f32 ? nullptr : q.records,
+                             feat_prec, q.out_color, q.bg);
+    const int st = check_frame(host[f], &q.cam);
+    if (st != PS_OK) return st;
+    P[f] = q.P;
+  }
+  const size_t pre_bytes = align_up(preprocess_batch_workspace_size(num_frames, P), 256);
+  PS_REQUIRE(fb->workspace != nullptr && fb->workspace_bytes >= pre_bytes + kQueueBytes,
+             "batch workspace too small (%zu < %zu)", fb->workspace_bytes, pre_bytes + kQueueBytes);
+  // Queue words first (fixed place for any batch composition), then the
+  // preprocess area.
+  unsigned* queue = static_cast<unsigned*>(fb->workspace);
+  void* ws = static_cast<char*>(fb->workspace) + kQueueBytes;
+  int* d_frame_counts = nullptr;
+  const int* d_n = nullptr;
+  int tiles = 0;
+  int st = preprocess_batch_launch(num_frames, host, sh_prec, fb->keys_unsorted,
+                                   fb->values_unsorted, fb->capacity, fb->curr_offset,
+                                   ws, pre_bytes, &d_frame_counts, &d_n, &tiles, stream);
+  if (st != PS_OK) return st;
+  st = sort_launch(0, d_n, fb->capacity, 32 + bit_length((unsigned)tiles), fb->sort_space,
+                   fb->sort_space_bytes, fb->keys_unsorted, fb->values_unsorted, fb->keys_sorted,
+                   fb->values_sorted, stream);
+  if (st != PS_OK) return st;
+  st = tile_ranges_scan_launch(0, d_n, fb->capacity, tiles, fb->keys_sorted, fb->tile_ranges,
+                               stream);
+  if (st != PS_OK) return st;
+  st = render_batch_launch(batch_frames_dev(num_frames, host, ws), num_frames,
+                           feat_prec, fb->tile_ranges, tiles, fb->values_sorted, queue, stream);
+  if (st != PS_OK) return st;
+  if (sync_count) {
+    // Total, status word (it follows the count in the preprocess state) and
+    // the per-frame counts.
+    const int* ptrs[3] = {fb->curr_offset, d_n + 1, d_frame_counts};
+    const int lens[3] = {1, 1, num_frames};
+    int vals[PS_MAX_BATCH_FRAMES + 2];
+    st = read_back_spans(ptrs, lens, 3, vals, stream);
+    if (st != PS_OK) return st;
+    if (frame_counts)
+      for (int f = 0; f < num_frames; ++f) frame_counts[f] = vals[2 + f];
     if (vals[1] & 1) {
       set_last_error("%d pairs do not fit in %d", vals[0], fb->capacity);
       return PS_ERR_OVERFLOW;

@@ -470,6 +470,103 @@ int forward_frame(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width, 
                         "forward_frame");
 }
 
+// Batch of frames through shared launches.  Each entry of `frames` is a
+// tuple (orig_points, shs, opacities, cov3Ds, width, height, position,
+// rotation, focal_x, focal_y, zFar, zNear, bg_color, out_color, records,
+// points_xy, rgb_depth, conic_opacity); the feature tensors of the unused
+// precision may be empty.  Returns per-frame pair counts when sync is set,
+// else an empty list.
+std::vector<int> forward_batch(const py::list& frames, int feat_prec, Tensor& curr_offset,
+                               Tensor& sort_space, Tensor& ku, Tensor& vu, Tensor& ks, Tensor& vs,
+                               Tensor& tile_ranges, Tensor& workspace, bool sync) {
+  const int nf = (int)frames.size();
+  need(nf > 0 && nf <= PS_MAX_BATCH_FRAMES, "forward_batch: bad frame count");
+  std::vector<ps_frame> fr(nf);
+  std::vector<int> P(nf);
+  int sh_prec = -1;
+  int64_t tiles = 0;
+  for (int f = 0; f < nf; ++f) {
+    py::tuple t = frames[f].cast<py::tuple>();
+    need(t.size() == 18, "forward_batch: each frame is an 18-tuple");
+    Tensor pts = t[0].cast<Tensor>(), shs = t[1].cast<Tensor>(), op = t[2].cast<Tensor>(),
+           cov = t[3].cast<Tensor>();
+    check_scene(pts, shs, op, cov);
+    const int w = t[4].cast<int>(), h = t[5].cast<int>();
+    Tensor pos = t[6].cast<Tensor>(), rot = t[7].cast<Tensor>();
+    Tensor bg = t[12].cast<Tensor>(), out = t[13].cast<Tensor>();
+    check_out(out, w, h);
+    const int shp = sh_prec_of(shs);
+    need(sh_prec < 0 || shp == sh_prec, "forward_batch: frames must share the SH precision");
+    sh_prec = shp;
+    ps_frame& q = fr[f];
+    memset(&q, 0, sizeof(q));
+    q.P = (int)op.size(0);
+    P[f] = q.P;
+    q.positions = ptr<float>(pts);
+    q.shs = shs.data_ptr();
+    q.opacities = ptr<float>(op);
+    q.cov3d = ptr<float>(cov);
+    q.cam = make_camera(w, h, 16, 16, pos, rot, t[8].cast<float>(), t[9].cast<float>(),
+                        t[10].cast<float>(), t[11].cast<float>());
+    auto b = host_floats(bg, 3, "bg_color");
+    q.bg[0] = b[0];
+    q.bg[1] = b[1];
+    q.bg[2] = b[2];
+    q.out_color = ptr<uint8_t>(out);
+    if (feat_prec == PS_PREC_F32) {
+      Tensor xy = t[15].cast<Tensor>(), rgbd = t[16].cast<Tensor>(), con = t[17].cast<Tensor>();
+      check_feats(xy, rgbd, con);
+      need(xy.numel() >= 2LL * q.P, "points_xy too small");
+      q.points_xy = ptr<float>(xy);
+      q.rgb_depth = ptr<float>(rgbd);
+      q.conic_opacity = ptr<float>(con);
+    } else {
+      Tensor rec = t[14].cast<Tensor>();
+      need_dev(rec, "records", at::kByte);
+      need(rec.numel() >= 24LL * q.P, "records too small");
+      q.records = rec.data_ptr();
+    }
+    tiles += tiles_of(w, h, 16, 16);
+  }
+  need_dev(curr_offset, "curr_offset", at::kInt);
+  need_dev(sort_space, "list_sorting_space", at::kChar);
+  need_dev(ku, "keys_unsorted", at::kLong);
+  need_dev(vu, "values_unsorted", at::kInt);
+  need_dev(ks, "keys_sorted", at::kLong);
+  need_dev(vs, "values_sorted", at::kInt);
+  need_dev(tile_ranges, "tile_ranges", at::kInt);
+  need(tile_ranges.numel() >= 2 * tiles, "tile_ranges too small for the batch");
+  ps_frame_buffers fb;
+  memset(&fb, 0, sizeof(fb));
+  fb.curr_offset = ptr<int>(curr_offset);
+  fb.sort_space = sort_space.data_ptr();
+  fb.sort_space_bytes = (size_t)sort_space.numel();
+  fb.keys_unsorted = ptr<uint64_t>(ku);
+  fb.values_unsorted = ptr<uint32_t>(vu);
+  fb.keys_sorted = ptr<uint64_t>(ks);
+  fb.values_sorted = ptr<uint32_t>(vs);
+  int cap = (int)std::min({ku.numel(), vu.numel(), ks.numel(), vs.numel()});
+  while (cap > 0 && ps_get_sort_buffer_size(cap) > fb.sort_space_bytes) cap /= 2;
+  fb.capacity = cap;
+  fb.tile_ranges = ptr<int>(tile_ranges);
+  const size_t wsb = ps_forward_batch_workspace_size(nf, P.data());
+  Tensor ws = workspace;
+  if (!(ws.defined() && ws.numel() > 0)) ws = cached_workspace(ku.device(), wsb, "batch");
+  need_dev(ws, "workspace", at::kByte);
+  need((size_t)ws.numel() >= wsb, "workspace too small");
+  fb.workspace = ws.data_ptr();
+  fb.workspace_bytes = (size_t)ws.numel();
+  std::vector<int> counts(nf, 0);
+  const int st = ps_forward_batch(nf, fr.data(), sh_prec, feat_prec, &fb, sync ? 1 : 0,
+                                  counts.data(), cur_stream());
+  if (st == PS_ERR_OVERFLOW)
+    throw std::runtime_error(std::string("forward_batch: Too many k-v pairs! (") +
+                             ps_last_error() + ")");
+  check(st, "forward_batch");
+  if (!sync) counts.clear();
+  return counts;
+}
+
 py::tuple camera_matrices(int width, int height, Tensor& position, Tensor& rotation, float fx,
                           float fy, float zfar, float znear) {
   ps_camera cam = make_camera(width, height, 16, 16, position, rotation, fx, fy, zfar, znear);
@@ -506,6 +603,17 @@ PYBIND11_MODULE(_C, m) {
   ops.def("forward_fused_e2e_preranges", &forward_fused_e2e_preranges);
   ops.def("forward_frame", &forward_frame);
   ops.def("camera_matrices", &camera_matrices);
+  ops.def("forward_batch", &forward_batch);
+  ops.def("forward_batch_workspace_size", [](std::vector<int> P) {
+    return ps_forward_batch_workspace_size((int)P.size(), P.data());
+  });
+  ops.def("max_batch_frames", []() { return (int)PS_MAX_BATCH_FRAMES; });
+  ops.def("forward_batch_counts_offset", [](std::vector<int> P) {
+    // Byte offset of the per-frame counts inside a batch workspace.
+    char* base = reinterpret_cast<char*>(4096);
+    const int* p = ps_forward_batch_counts_ptr(base, (int)P.size(), P.data());
+    return (int64_t)(reinterpret_cast<const char*>(p) - base);
+  });
   ops.def("forward_workspace_size",
           [](int64_t P) { return ps_forward_workspace_size((int)P); });
   ops.def("preprocess_workspace_size",

@@ -34,6 +34,10 @@
 #include "ps_host.h"
 #include "ps_wave.h"
 
+#include <string.h>
+
+#include <type_traits>
+
 namespace ps {
 namespace {
 
@@ -116,7 +120,8 @@ __device__ __forceinline__ Cand permute_cand(const Cand& v, int src) {
 // mode writes keys at off + running index.
 template <bool EMIT>
 __device__ int sweep_massive(const Cand& me, bool is_massive, int lane, int gid, int off, int cnt,
-                             int bx, int by, int grid_x, uint64_t* keys, uint32_t* vals) {
+                             int bx, int by, int grid_x, int tile0, uint64_t* keys,
+                             uint32_t* vals) {
   uint64_t todo = ballot(is_massive);
   int mine = 0;
   const uint64_t below = lanes_below(lane);
@@ -146,7 +151,7 @@ __device__ int sweep_massive(const Cand& me, bool is_massive, int lane, int gid,
       // keeps every write inside this Gaussian's slots regardless.
       if (EMIT && ok && k < qcnt) {
         const int slot = qoff + k;
-        keys[slot] = make_key((q.ry0 + ty) * grid_x + (q.rx0 + tx), q.depth);
+        keys[slot] = make_key(tile0 + (q.ry0 + ty) * grid_x + (q.rx0 + tx), q.depth);
         vals[slot] = (uint32_t)qgid;
       }
       run += popc64(m);
@@ -161,7 +166,8 @@ __device__ int sweep_massive(const Cand& me, bool is_massive, int lane, int gid,
 // whose inclusive prefix of areas exceeds t.
 template <bool EMIT>
 __device__ int sweep_fragmented(const Cand& me, bool is_frag, int lane, int gid, int off, int cnt,
-                                int bx, int by, int grid_x, uint64_t* keys, uint32_t* vals) {
+                                int bx, int by, int grid_x, int tile0, uint64_t* keys,
+                                uint32_t* vals) {
   const int area = is_frag ? (me.rx1 - me.rx0) * (me.ry1 - me.ry0) : 0;
   const int incl = wave_incl_scan(area, lane);
   const int excl = incl - area;
@@ -199,7 +205,7 @@ __device__ int sweep_fragmented(const Cand& me, bool is_frag, int lane, int gid,
     const int k = o_done + popc64(m & below & ~lanes_below(seg_lo));
     if (EMIT && ok && k < o_cnt) {
       const int slot = o_off + k;
-      keys[slot] = make_key((q.ry0 + ty) * grid_x + (q.rx0 + tx), q.depth);
+      keys[slot] = make_key(tile0 + (q.ry0 + ty) * grid_x + (q.rx0 + tx), q.depth);
       vals[slot] = (uint32_t)o_gid;
     }
     // Each owner adds the passing tasks of its own segment in this step.
@@ -251,62 +257,79 @@ __device__ __forceinline__ uint16_t to_half_bits(float f, int prec) {
   return prec == kFeatBF16 ? f32_to_bf16(f) : __half_as_ushort(__float2half_rn(f));
 }
 
+// Count pass for the workgroup `vb` of a launch (vb - F.block0 is the
+// frame-local workgroup).  counts/block_sums are indexed by launch position.
 template <int SHP>
-__global__ __launch_bounds__(kPreThreads) void ps_pre_count(
-    int P, const float* __restrict__ pos, const float* __restrict__ opac,
-    const float* __restrict__ cov3, const void* __restrict__ shs, CamParams cp,
-    float2* __restrict__ xy_out, float4* __restrict__ rgbd_out, float4* __restrict__ conic_out,
-    FeatRec16* __restrict__ rec_out, int rec_prec, int* __restrict__ counts,
-    int* __restrict__ block_sums) {
+__device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* __restrict__ counts,
+                                               int* __restrict__ block_sums) {
   __shared__ int scratch[4];
   const int tid = threadIdx.x, lane = tid & 63;
-  const int g = blockIdx.x * kPreThreads + tid;
+  const int g = (vb - F.block0) * kPreThreads + tid;
+  const CamParams& cp = F.cp;
   Proj pj;
   pj.valid = false;
   float px = 0.f, py = 0.f, pz = 0.f;
-  if (g < P) {
-    px = pos[3 * (size_t)g + 0];
-    py = pos[3 * (size_t)g + 1];
-    pz = pos[3 * (size_t)g + 2];
+  if (g < F.P) {
+    px = F.pos[3 * (size_t)g + 0];
+    py = F.pos[3 * (size_t)g + 1];
+    pz = F.pos[3 * (size_t)g + 2];
     float c[6];
-    load_cov(cov3, g, c);
-    pj = project(px, py, pz, opac[g], c, cp);
+    load_cov(F.cov3, g, c);
+    pj = project(px, py, pz, F.opac[g], c, cp);
   }
   const int area = pj.valid ? (pj.rmax_x - pj.rmin_x) * (pj.rmax_y - pj.rmin_y) : 0;
   int count = 0;
   if (area == 1) count = tile_hits(pj.rmin_x, pj.rmin_y, pj, cp.block_x, cp.block_y) ? 1 : 0;
   const Cand me = cand_from(pj);
   count += sweep_massive<false>(me, area >= kMassive, lane, g, 0, 0, cp.block_x, cp.block_y,
-                                cp.grid_x, nullptr, nullptr);
+                                cp.grid_x, 0, nullptr, nullptr);
   count += sweep_fragmented<false>(me, area >= 2 && area < kMassive, lane, g, 0, 0, cp.block_x,
-                                   cp.block_y, cp.grid_x, nullptr, nullptr);
-  if (g < P) counts[g] = count;
+                                   cp.block_y, cp.grid_x, 0, nullptr, nullptr);
+  counts[(size_t)vb * kPreThreads + tid] = count;
   if (count > 0) {
     const float4 con = make_float4((-0.5f * kLog2e) * pj.con_a, -kLog2e * pj.con_b,
                                    (-0.5f * kLog2e) * pj.con_c, pj.log2_opacity);
     float sh[48];
-    load_sh<SHP>(shs, g, sh);
+    load_sh<SHP>(F.shs, g, sh);
     const F3 rgb = sh_color(px, py, pz, cp.campos, sh);
-    if (xy_out) xy_out[g] = make_float2(pj.xy_x, pj.xy_y);
-    if (conic_out) conic_out[g] = con;
-    if (rgbd_out) rgbd_out[g] = make_float4(rgb.x, rgb.y, rgb.z, pj.depth);
-    if (rec_out) {
+    if (F.xy) F.xy[g] = make_float2(pj.xy_x, pj.xy_y);
+    if (F.conic) F.conic[g] = con;
+    if (F.rgbd) F.rgbd[g] = make_float4(rgb.x, rgb.y, rgb.z, pj.depth);
+    if (F.rec) {
       FeatRec16 r;
       r.x = pj.xy_x;
       r.y = pj.xy_y;
       r.w = con.w;
-      r.ca = to_half_bits(con.x * kConicScale, rec_prec);
-      r.cb = to_half_bits(con.y * kConicScale, rec_prec);
-      r.cc = to_half_bits(con.z * kConicScale, rec_prec);
-      r.r = to_half_bits(rgb.x, rec_prec);
-      r.g = to_half_bits(rgb.y, rec_prec);
-      r.b = to_half_bits(rgb.z, rec_prec);
-      rec_out[g] = r;
+      r.ca = to_half_bits(con.x * kConicScale, F.rec_prec);
+      r.cb = to_half_bits(con.y * kConicScale, F.rec_prec);
+      r.cc = to_half_bits(con.z * kConicScale, F.rec_prec);
+      r.r = to_half_bits(rgb.x, F.rec_prec);
+      r.g = to_half_bits(rgb.y, F.rec_prec);
+      r.b = to_half_bits(rgb.z, F.rec_prec);
+      F.rec[g] = r;
     }
   }
   int total;
   block256_excl_scan(count, scratch, total);
-  if (tid == 0) block_sums[blockIdx.x] = total;
+  if (tid == 0) block_sums[vb] = total;
+}
+
+template <int SHP>
+__global__ __launch_bounds__(kPreThreads) void ps_pre_count(FrameDev F, int* __restrict__ counts,
+                                                            int* __restrict__ block_sums) {
+  pre_count_body<SHP>(F, blockIdx.x, counts, block_sums);
+}
+
+__device__ __forceinline__ int frame_of_block(const FrameDev* frames, int nf, int vb) {
+  return frame_search(frames, nf, vb, [](const FrameDev& f) { return f.block0; });
+}
+
+template <int SHP>
+__global__ __launch_bounds__(kPreThreads) void ps_pre_count_batch(const FrameDev* __restrict__ frames,
+                                                                  int nf, int* __restrict__ counts,
+                                                                  int* __restrict__ block_sums) {
+  const int f = frame_of_block(frames, nf, blockIdx.x);
+  pre_count_body<SHP>(frames[f], blockIdx.x, counts, block_sums);
 }
 
 // state[0] = offset the frame's keys start at, state[1] = pair count for the
@@ -316,7 +339,9 @@ constexpr int kStateBase = 0, kStateCount = 1, kStateFlags = 2, kStateWords = 64
 __global__ __launch_bounds__(1024) void ps_pre_scan(int nblocks, const int* __restrict__ block_sums,
                                                     int* __restrict__ block_offsets,
                                                     int* __restrict__ curr_offset, int capacity,
-                                                    int reset_offset, int* __restrict__ state) {
+                                                    int reset_offset, int* __restrict__ state,
+                                                    const FrameDev* __restrict__ frames, int nf,
+                                                    int* __restrict__ frame_counts) {
   __shared__ int wsum[16];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int per = (nblocks + 1023) / 1024;
@@ -367,39 +392,70 @@ __global__ __launch_bounds__(1024) void ps_pre_scan(int nblocks, const int* __re
     state[kStateCount] = overflow ? 0 : (int)end;
     state[kStateFlags] = overflow ? 1 : 0;
   }
+  if (frames != nullptr && frame_counts != nullptr) {
+    // Pairs of each frame of a batch: difference of its first block's offset
+    // and the next frame's.
+    __syncthreads();
+    for (int f = tid; f < nf; f += 1024) {
+      const int b = frames[f].block0;
+      const int bn = (f + 1 < nf) ? frames[f + 1].block0 : nblocks;
+      const int start = (b < nblocks) ? block_offsets[b] : total;
+      const int stop = (bn < nblocks) ? block_offsets[bn] : total;
+      frame_counts[f] = stop - start;
+    }
+  }
 }
 
-__global__ __launch_bounds__(kPreThreads) void ps_pre_emit(
-    int P, const float* __restrict__ pos, const float* __restrict__ opac,
-    const float* __restrict__ cov3, CamParams cp, const int* __restrict__ counts,
-    const int* __restrict__ block_offsets, const int* __restrict__ state,
-    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+__device__ __forceinline__ void pre_emit_body(const FrameDev& F, int vb,
+                                              const int* __restrict__ counts,
+                                              const int* __restrict__ block_offsets,
+                                              const int* __restrict__ state,
+                                              uint64_t* __restrict__ keys,
+                                              uint32_t* __restrict__ vals) {
   __shared__ int scratch[4];
   if (state[kStateFlags] & 1) return;
   const int tid = threadIdx.x, lane = tid & 63;
-  const int g = blockIdx.x * kPreThreads + tid;
-  const int count = (g < P) ? counts[g] : 0;
+  const int g = (vb - F.block0) * kPreThreads + tid;
+  const int count = (g < F.P) ? counts[(size_t)vb * kPreThreads + tid] : 0;
   int total;
   const int excl = block256_excl_scan(count, scratch, total);
   if (total == 0) return;  // uniform across the block
-  const int off = state[kStateBase] + block_offsets[blockIdx.x] + excl;
+  const CamParams& cp = F.cp;
+  const int off = state[kStateBase] + block_offsets[vb] + excl;
   Proj pj;
   pj.valid = false;
   if (count > 0) {
     float c[6];
-    load_cov(cov3, g, c);
-    pj = project(pos[3 * (size_t)g], pos[3 * (size_t)g + 1], pos[3 * (size_t)g + 2], opac[g], c, cp);
+    load_cov(F.cov3, g, c);
+    pj = project(F.pos[3 * (size_t)g], F.pos[3 * (size_t)g + 1], F.pos[3 * (size_t)g + 2],
+                 F.opac[g], c, cp);
   }
   const int area = (count > 0 && pj.valid) ? (pj.rmax_x - pj.rmin_x) * (pj.rmax_y - pj.rmin_y) : 0;
   if (area == 1) {
-    keys[off] = make_key(pj.rmin_y * cp.grid_x + pj.rmin_x, pj.depth);
+    keys[off] = make_key(F.tile0 + pj.rmin_y * cp.grid_x + pj.rmin_x, pj.depth);
     vals[off] = (uint32_t)g;
   }
   const Cand me = cand_from(pj);
   sweep_massive<true>(me, area >= kMassive, lane, g, off, count, cp.block_x, cp.block_y,
-                      cp.grid_x, keys, vals);
+                      cp.grid_x, F.tile0, keys, vals);
   sweep_fragmented<true>(me, area >= 2 && area < kMassive, lane, g, off, count, cp.block_x,
-                         cp.block_y, cp.grid_x, keys, vals);
+                         cp.block_y, cp.grid_x, F.tile0, keys, vals);
+}
+
+__global__ __launch_bounds__(kPreThreads) void ps_pre_emit(FrameDev F, const int* __restrict__ counts,
+                                                           const int* __restrict__ block_offsets,
+                                                           const int* __restrict__ state,
+                                                           uint64_t* __restrict__ keys,
+                                                           uint32_t* __restrict__ vals) {
+  pre_emit_body(F, blockIdx.x, counts, block_offsets, state, keys, vals);
+}
+
+__global__ __launch_bounds__(kPreThreads) void ps_pre_emit_batch(
+    const FrameDev* __restrict__ frames, int nf, const int* __restrict__ counts,
+    const int* __restrict__ block_offsets, const int* __restrict__ state,
+    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int f = frame_of_block(frames, nf, blockIdx.x);
+  pre_emit_body(frames[f], blockIdx.x, counts, block_offsets, state, keys, vals);
 }
 
 struct PreWs {
@@ -409,21 +465,77 @@ struct PreWs {
   int* block_offsets;
 };
 
-inline size_t pre_layout(int P, char* base, PreWs* w) {
-  const int nb = (P + kPreThreads - 1) / kPreThreads;
+// Workspace for `nb` preprocess workgroups (a single frame has
+// ceil(P / 256); a batch the sum over its frames).
+inline size_t pre_layout_blocks(int nb, char* base, PreWs* w) {
   size_t off = 0;
+  const size_t b = (size_t)(nb > 0 ? nb : 1);
   if (w) w->state = reinterpret_cast<int*>(base + off);
   off += align_up(kStateWords * sizeof(int), 256);
   if (w) w->counts = reinterpret_cast<int*>(base + off);
-  off += align_up((size_t)(P > 0 ? P : 1) * sizeof(int), 256);
+  off += align_up(b * kPreThreads * sizeof(int), 256);
   if (w) w->block_sums = reinterpret_cast<int*>(base + off);
-  off += align_up((size_t)(nb > 0 ? nb : 1) * sizeof(int), 256);
+  off += align_up(b * sizeof(int), 256);
   if (w) w->block_offsets = reinterpret_cast<int*>(base + off);
-  off += align_up((size_t)(nb > 0 ? nb : 1) * sizeof(int), 256);
+  off += align_up(b * sizeof(int), 256);
   return off;
 }
 
+inline int blocks_for(int P) { return (P + kPreThreads - 1) / kPreThreads; }
+
+inline size_t pre_layout(int P, char* base, PreWs* w) {
+  return pre_layout_blocks(blocks_for(P), base, w);
+}
+
+template <typename K1>
+int launch_count(int sh_prec, int nb, hipStream_t stream, K1 launch) {
+  if (nb <= 0) return PS_OK;
+  if (sh_prec == PS_PREC_F32) launch(std::integral_constant<int, kShF32>{});
+  else if (sh_prec == PS_PREC_F16) launch(std::integral_constant<int, kShF16>{});
+  else launch(std::integral_constant<int, kShBF16>{});
+  PS_LAUNCH_CHECK();
+  return PS_OK;
+}
+
 }  // namespace
+
+FrameDev make_frame_dev(int P, const float* positions, const void* shs, const float* opacities,
+                        const float* cov3d, const ps_camera* cam, float* xy, float* rgbd,
+                        float* conic, void* rec, int rec_prec, uint8_t* out, const float* bg) {
+  FrameDev F;
+  memset(&F, 0, sizeof(F));
+  F.pos = positions;
+  F.opac = opacities;
+  F.cov3 = cov3d;
+  F.shs = shs;
+  F.xy = reinterpret_cast<float2*>(xy);
+  F.rgbd = reinterpret_cast<float4*>(rgbd);
+  F.conic = reinterpret_cast<float4*>(conic);
+  F.rec = reinterpret_cast<FeatRec16*>(rec);
+  F.out = out;
+  F.P = P;
+  F.rec_prec = rec_prec;
+  F.cp = make_cam_params(cam);
+  F.tiles = F.cp.grid_x * F.cp.grid_y;
+  if (bg) {
+    F.bg[0] = bg[0];
+    F.bg[1] = bg[1];
+    F.bg[2] = bg[2];
+  }
+  return F;
+}
+
+static int check_frame_args(int P, const void* positions, const void* shs, const void* opacities,
+                            const void* cov3d, const ps_camera* cam) {
+  PS_REQUIRE(P >= 0, "P must be >= 0 (got %d)", P);
+  PS_REQUIRE(cam != nullptr, "camera is NULL");
+  PS_REQUIRE(cam->width > 0 && cam->height > 0, "image size must be positive");
+  PS_REQUIRE(cam->block_x > 0 && cam->block_y > 0, "tile size must be positive");
+  const long long tiles = (long long)tiles_x(cam) * tiles_y(cam);
+  PS_REQUIRE(tiles < (1LL << 31), "too many tiles");
+  PS_REQUIRE(P == 0 || (positions && shs && opacities && cov3d), "scene pointer is NULL");
+  return PS_OK;
+}
 
 // Internal entry used by ps_forward as well.
 int preprocess_launch(int P, const float* positions, const void* shs, int sh_prec,
@@ -432,10 +544,8 @@ int preprocess_launch(int P, const float* positions, const void* shs, int sh_pre
                       int feat_prec, uint64_t* keys, uint32_t* values, int capacity,
                       int* curr_offset, void* workspace, size_t workspace_bytes,
                       hipStream_t stream, bool reset_offset) {
-  PS_REQUIRE(P >= 0, "P must be >= 0 (got %d)", P);
-  PS_REQUIRE(cam != nullptr, "camera is NULL");
-  PS_REQUIRE(cam->width > 0 && cam->height > 0, "image size must be positive");
-  PS_REQUIRE(cam->block_x > 0 && cam->block_y > 0, "tile size must be positive");
+  int st = check_frame_args(P, positions, shs, opacities, cov3d, cam);
+  if (st != PS_OK) return st;
   PS_REQUIRE(sh_prec == PS_PREC_F32 || sh_prec == PS_PREC_F16 || sh_prec == PS_PREC_BF16,
              "unknown SH precision %d", sh_prec);
   PS_REQUIRE(feat_records == nullptr || feat_prec == PS_PREC_F16 || feat_prec == PS_PREC_BF16,
@@ -445,42 +555,158 @@ int preprocess_launch(int P, const float* positions, const void* shs, int sh_pre
   PS_REQUIRE(workspace != nullptr && workspace_bytes >= ps_preprocess_workspace_size(P),
              "preprocess workspace too small (%zu < %zu)", workspace_bytes,
              ps_preprocess_workspace_size(P));
-  const long long tiles = (long long)tiles_x(cam) * tiles_y(cam);
-  PS_REQUIRE(tiles < (1LL << 31), "too many tiles");
-  PS_REQUIRE(P == 0 || (positions && shs && opacities && cov3d), "scene pointer is NULL");
   PS_REQUIRE(capacity == 0 || (keys && values), "key/value buffers are NULL");
 
   PreWs w;
   pre_layout(P, static_cast<char*>(workspace), &w);
-  const CamParams cp = make_cam_params(cam);
-  const int nb = (P + kPreThreads - 1) / kPreThreads;
-  if (nb > 0) {
-    if (sh_prec == PS_PREC_F32)
-      hipLaunchKernelGGL(ps_pre_count<kShF32>, dim3(nb), dim3(kPreThreads), 0, stream, P,
-                         positions, opacities, cov3d, shs, cp, (float2*)points_xy,
-                         (float4*)rgb_depth, (float4*)conic_opacity, (FeatRec16*)feat_records,
-                         feat_prec, w.counts, w.block_sums);
-    else if (sh_prec == PS_PREC_F16)
-      hipLaunchKernelGGL(ps_pre_count<kShF16>, dim3(nb), dim3(kPreThreads), 0, stream, P,
-                         positions, opacities, cov3d, shs, cp, (float2*)points_xy,
-                         (float4*)rgb_depth, (float4*)conic_opacity, (FeatRec16*)feat_records,
-                         feat_prec, w.counts, w.block_sums);
-    else
-      hipLaunchKernelGGL(ps_pre_count<kShBF16>, dim3(nb), dim3(kPreThreads), 0, stream, P,
-                         positions, opacities, cov3d, shs, cp, (float2*)points_xy,
-                         (float4*)rgb_depth, (float4*)conic_opacity, (FeatRec16*)feat_records,
-                         feat_prec, w.counts, w.block_sums);
-    PS_LAUNCH_CHECK();
-  }
+  const FrameDev F = make_frame_dev(P, positions, shs, opacities, cov3d, cam, points_xy, rgb_depth,
+                                    conic_opacity, feat_records, feat_prec, nullptr, nullptr);
+  const int nb = blocks_for(P);
+  st = launch_count(sh_prec, nb, stream, [&](auto shp) {
+    hipLaunchKernelGGL(ps_pre_count<decltype(shp)::value>, dim3(nb), dim3(kPreThreads), 0, stream,
+                       F, w.counts, w.block_sums);
+  });
+  if (st != PS_OK) return st;
   hipLaunchKernelGGL(ps_pre_scan, dim3(1), dim3(1024), 0, stream, nb, w.block_sums,
-                     w.block_offsets, curr_offset, capacity, reset_offset ? 1 : 0, w.state);
+                     w.block_offsets, curr_offset, capacity, reset_offset ? 1 : 0, w.state,
+                     (const FrameDev*)nullptr, 0, (int*)nullptr);
   PS_LAUNCH_CHECK();
   if (nb > 0) {
-    hipLaunchKernelGGL(ps_pre_emit, dim3(nb), dim3(kPreThreads), 0, stream, P, positions,
-                       opacities, cov3d, cp, w.counts, w.block_offsets, w.state, keys, values);
+    hipLaunchKernelGGL(ps_pre_emit, dim3(nb), dim3(kPreThreads), 0, stream, F, w.counts,
+                       w.block_offsets, w.state, keys, values);
     PS_LAUNCH_CHECK();
   }
   return PS_OK;
+}
+
+// Pinned staging ring for batch descriptors, one per host thread.  A copy
+// from pageable memory would block the host until the stream reaches it;
+// from a pinned slot it is queued like a kernel.  A slot is rewritten only
+// after the copy that last read it has executed (its event), so up to
+// kDescSlots batches can be queued ahead of the device.
+struct DescRing {
+  static constexpr int kDescSlots = 16;
+  FrameDev* host = nullptr;
+  hipEvent_t ev[kDescSlots] = {};
+  bool used[kDescSlots] = {};
+  int next = 0;
+  bool ok = false;
+  DescRing() {
+    if (hipHostMalloc(reinterpret_cast<void**>(&host),
+                      sizeof(FrameDev) * PS_MAX_BATCH_FRAMES * kDescSlots,
+                      hipHostMallocDefault) != hipSuccess) {
+      host = nullptr;
+      return;
+    }
+    for (int i = 0; i < kDescSlots; ++i)
+      if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) return;
+    ok = true;
+  }
+  ~DescRing() {
+    for (int i = 0; i < kDescSlots; ++i)
+      if (ev[i]) (void)hipEventDestroy(ev[i]);
+    if (host) (void)hipHostFree(host);
+  }
+};
+
+int upload_descriptors(FrameDev* dst, const FrameDev* src, int nf, hipStream_t stream) {
+  PS_REQUIRE(nf > 0 && nf <= PS_MAX_BATCH_FRAMES, "batch of %d frames", nf);
+  static thread_local DescRing ring;
+  if (!ring.ok) {
+    PS_HIP_CHECK(hipMemcpy(dst, src, nf * sizeof(FrameDev), hipMemcpyHostToDevice));
+    return PS_OK;
+  }
+  const int i = ring.next;
+  ring.next = (i + 1) % DescRing::kDescSlots;
+  if (ring.used[i]) PS_HIP_CHECK(hipEventSynchronize(ring.ev[i]));
+  FrameDev* slot = ring.host + (size_t)i * PS_MAX_BATCH_FRAMES;
+  memcpy(slot, src, nf * sizeof(FrameDev));
+  PS_HIP_CHECK(hipMemcpyAsync(dst, slot, nf * sizeof(FrameDev), hipMemcpyHostToDevice, stream));
+  PS_HIP_CHECK(hipEventRecord(ring.ev[i], stream));
+  ring.used[i] = true;
+  return PS_OK;
+}
+
+size_t preprocess_batch_workspace_size(int nf, const int* P) {
+  int nb = 0;
+  for (int f = 0; f < nf; ++f) nb += blocks_for(P[f] > 0 ? P[f] : 0);
+  return align_up(pre_layout_blocks(nb, nullptr, nullptr), 256) +
+         align_up((size_t)(nf > 0 ? nf : 1) * sizeof(FrameDev), 256) +
+         align_up((size_t)(nf > 0 ? nf : 1) * sizeof(int), 256);
+}
+
+// Batched preprocess.  Fills in each frame's first workgroup (block0) and
+// first global tile id (tile0), consecutive in frame order, copies the
+// descriptors into the workspace and emits the keys of all frames into one
+// array with global tile ids.  Returns the total tile count in *tiles_out.
+int preprocess_batch_launch(int nf, FrameDev* frames_host, int sh_prec, uint64_t* keys,
+                            uint32_t* values, int capacity, int* total_out, void* workspace,
+                            size_t workspace_bytes, int** frame_counts_dev, const int** state_count,
+                            int* tiles_out, hipStream_t stream) {
+  PS_REQUIRE(nf > 0, "empty batch");
+  PS_REQUIRE(sh_prec == PS_PREC_F32 || sh_prec == PS_PREC_F16 || sh_prec == PS_PREC_BF16,
+             "unknown SH precision %d", sh_prec);
+  PS_REQUIRE(capacity == 0 || (keys && values), "key/value buffers are NULL");
+  PS_REQUIRE(total_out != nullptr, "curr_offset is NULL");
+  int nb = 0;
+  long long tiles = 0;
+  for (int f = 0; f < nf; ++f) {
+    frames_host[f].block0 = nb;
+    frames_host[f].tile0 = (int)tiles;
+    nb += blocks_for(frames_host[f].P);
+    tiles += frames_host[f].tiles;
+    PS_REQUIRE(tiles < (1LL << 31), "too many tiles in the batch");
+  }
+  if (tiles_out) *tiles_out = (int)tiles;
+  const size_t pre_bytes = align_up(pre_layout_blocks(nb, nullptr, nullptr), 256);
+  PS_REQUIRE(workspace != nullptr &&
+                 workspace_bytes >= pre_bytes + align_up(nf * sizeof(FrameDev), 256) +
+                                        align_up(nf * sizeof(int), 256),
+             "batch workspace too small");
+  char* ws = static_cast<char*>(workspace);
+  PreWs w;
+  pre_layout_blocks(nb, ws, &w);
+  FrameDev* dframes = reinterpret_cast<FrameDev*>(ws + pre_bytes);
+  int* fcounts = reinterpret_cast<int*>(ws + pre_bytes + align_up(nf * sizeof(FrameDev), 256));
+  int st = upload_descriptors(dframes, frames_host, nf, stream);
+  if (st != PS_OK) return st;
+  st = launch_count(sh_prec, nb, stream, [&](auto shp) {
+    hipLaunchKernelGGL(ps_pre_count_batch<decltype(shp)::value>, dim3(nb), dim3(kPreThreads), 0,
+                       stream, (const FrameDev*)dframes, nf, w.counts, w.block_sums);
+  });
+  if (st != PS_OK) return st;
+  hipLaunchKernelGGL(ps_pre_scan, dim3(1), dim3(1024), 0, stream, nb, w.block_sums,
+                     w.block_offsets, total_out, capacity, 1, w.state, (const FrameDev*)dframes, nf,
+                     fcounts);
+  PS_LAUNCH_CHECK();
+  if (nb > 0) {
+    hipLaunchKernelGGL(ps_pre_emit_batch, dim3(nb), dim3(kPreThreads), 0, stream,
+                       (const FrameDev*)dframes, nf, w.counts, w.block_offsets, w.state, keys,
+                       values);
+    PS_LAUNCH_CHECK();
+  }
+  if (frame_counts_dev) *frame_counts_dev = fcounts;
+  if (state_count) *state_count = w.state + kStateCount;
+  return PS_OK;
+}
+
+const FrameDev* batch_frames_dev(int nf, const FrameDev* frames_host, void* workspace) {
+  int nb = 0;
+  for (int f = 0; f < nf; ++f) nb += blocks_for(frames_host[f].P);
+  return reinterpret_cast<const FrameDev*>(static_cast<char*>(workspace) +
+                                           align_up(pre_layout_blocks(nb, nullptr, nullptr), 256));
+}
+
+int* batch_counts_dev(int nf, const int* P, void* workspace) {
+  int nb = 0;
+  for (int f = 0; f < nf; ++f) nb += blocks_for(P[f] > 0 ? P[f] : 0);
+  return reinterpret_cast<int*>(static_cast<char*>(workspace) +
+                                align_up(pre_layout_blocks(nb, nullptr, nullptr), 256) +
+                                align_up((size_t)nf * sizeof(FrameDev), 256));
+}
+
+int check_frame(const FrameDev& F, const ps_camera* cam) {
+  return check_frame_args(F.P, F.pos, F.shs, F.opac, F.cov3, cam);
 }
 
 }  // namespace ps

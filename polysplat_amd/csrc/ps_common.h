@@ -326,3 +326,40 @@ struct alignas(8) FeatRec16 {
 static_assert(sizeof(FeatRec16) == 24, "record layout");
 
 }  // namespace ps
+
+namespace ps {
+
+// Everything the preprocess and render kernels need about one frame.  A
+// single-frame launch passes it by value; a batched launch reads an array of
+// them from device memory and maps each workgroup / tile to its frame.
+struct FrameDev {
+  const float* pos;
+  const float* opac;
+  const float* cov3;
+  const void* shs;
+  float2* xy;        // FP32 features (may be null)
+  float4* rgbd;
+  float4* conic;
+  FeatRec16* rec;    // reduced-precision records (may be null)
+  uint8_t* out;      // u8[H][W][3]
+  int P;
+  int rec_prec;
+  int block0;        // first preprocess workgroup of this frame in the launch
+  int tile0;         // first global tile id of this frame
+  int tiles;         // number of 16x16 tiles of this frame
+  float bg[3];
+  CamParams cp;
+};
+
+// Largest f with key >= field(frames[f]) (frames sorted by that field).
+template <typename Field>
+__device__ __forceinline__ int frame_search(const FrameDev* frames, int nf, int key, Field field) {
+  int lo = 0, hi = nf - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (field(frames[mid]) <= key) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+}  // namespace ps

@@ -270,6 +270,48 @@ __global__ __launch_bounds__(kWave) void ps_render_persistent(
   }
 }
 
+// Persistent renderer over the tiles of several frames (one queue for all):
+// global tile t belongs to the last frame whose tile0 <= t.  Frame tile
+// offsets are staged in LDS once per workgroup.
+constexpr int kMaxBatch = PS_MAX_BATCH_FRAMES;
+
+template <int PREC, int CHECK>
+__global__ __launch_bounds__(kWave) void ps_render_persistent_batch(
+    const FrameDev* __restrict__ frames, int nf, const int2* __restrict__ ranges, int tiles,
+    const uint32_t* __restrict__ order, unsigned* __restrict__ queue) {
+  __shared__ float4 sA[2][kBatch];
+  __shared__ float4 sB[2][kBatch];
+  __shared__ float sC[2][kBatch];
+  __shared__ int s_tile0[kMaxBatch];
+  const int lane = threadIdx.x;
+  if (lane < nf) s_tile0[lane] = frames[lane].tile0;
+  __syncthreads();
+  int tile = blockIdx.x;
+  while (tile < tiles) {
+    int lo = 0, hi = nf - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_tile0[mid] <= tile) lo = mid; else hi = mid - 1;
+    }
+    const FrameDev& F = frames[lo];
+    const int2 r = ranges[tile];
+    render_tile<PREC, false, CHECK, false>(tile - F.tile0, r.x, r.y, F.cp.W, F.cp.H, F.cp.grid_x,
+                                           order, F.xy, F.rgbd, F.conic, F.rec,
+                                           make_float3(F.bg[0], F.bg[1], F.bg[2]), F.out, sA, sB,
+                                           sC, lane);
+    int next = 0;
+    if (lane == 0) next = (int)atomicAdd(&queue[0], 1u) + (int)gridDim.x;
+    tile = bcast(next, 0);
+  }
+  if (lane == 0) {
+    const unsigned done = atomicAdd(&queue[1], 1u);
+    if (done == gridDim.x - 1) {
+      atomicExch(&queue[0], 0u);
+      atomicExch(&queue[1], 0u);
+    }
+  }
+}
+
 struct GridCache {
   int device = -1;
   int cus = 0;
@@ -356,6 +398,27 @@ int dispatch_persistent(bool ranges_mode, bool gathered, bool dt, const uint64_t
 }
 
 }  // namespace
+
+int render_batch_launch(const FrameDev* frames_dev, int nf, int feat_prec, const int* ranges,
+                        int tiles, const uint32_t* order, unsigned* queue, hipStream_t stream) {
+  PS_REQUIRE(nf > 0 && nf <= kMaxBatch, "batch of %d frames (max %d)", nf, kMaxBatch);
+  if (tiles <= 0) return PS_OK;
+  auto go = [&](auto kernel) {
+    const int grid = persistent_grid(kernel, tiles);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kWave), 0, stream, frames_dev, nf,
+                       reinterpret_cast<const int2*>(ranges), tiles, order, queue);
+  };
+  switch (feat_prec) {
+    case PS_PREC_F32: go(ps_render_persistent_batch<kFeatF32, 8>); break;
+    case PS_PREC_F16: go(ps_render_persistent_batch<kFeatF16, 8>); break;
+    case PS_PREC_BF16: go(ps_render_persistent_batch<kFeatBF16, 8>); break;
+    default:
+      set_last_error("unknown feature precision %d", feat_prec);
+      return PS_ERR_INVALID;
+  }
+  PS_LAUNCH_CHECK();
+  return PS_OK;
+}
 
 int render_plain_launch(int n_host, const int* d_n, int W, int H, const float* xy,
                         const float* rgbd, const float* con, const uint64_t* keys,

@@ -22,7 +22,6 @@
 #include "ps_host.h"
 #include "ps_wave.h"
 
-#include <stdlib.h>
 
 namespace ps {
 namespace {
@@ -143,9 +142,9 @@ __device__ __forceinline__ void load_tile(const uint64_t* __restrict__ keys,
 // barrier is needed between rounds); one block scan turns the per-wave counts
 // into tile-local starts; items are staged in LDS grouped by digit and each
 // digit run is written contiguously.  The next tile's keys are loaded into
-// registers while the current one is ranked and written.
-template <bool PREFETCH>
-__device__ __forceinline__ void downsweep_body(
+// registers while the current one is scanned, staged and written (about 210
+// VGPRs: two workgroups per CU, no spills).
+__global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
     const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint64_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, const int* __restrict__ d_n,
     int n_host, int max_pairs, int shift, const uint32_t* __restrict__ offsets,
@@ -204,14 +203,13 @@ __device__ __forceinline__ void downsweep_body(
       __builtin_amdgcn_wave_barrier();
       pos[r] = base + (uint32_t)rank;
     }
-    // Optionally prefetch the next tile while this one is scanned, staged
-    // and written.
+    // Prefetch the next tile while this one is scanned, staged and written.
     uint64_t kn[kSortItems];
     uint32_t vn[kSortItems];
     const long long ntile0 = tile0 + kSortTile;
     const int ntile_n =
         (t + 1 < t_end) ? ((pl.n - ntile0 < kSortTile) ? (int)(pl.n - ntile0) : kSortTile) : 0;
-    if (PREFETCH && ntile_n > 0) load_tile(keys_in, vals_in, ntile0, ntile_n, wave, lane, kn, vn);
+    if (ntile_n > 0) load_tile(keys_in, vals_in, ntile0, ntile_n, wave, lane, kn, vn);
     __syncthreads();
     const uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
     const uint32_t tot = c0 + c1 + c2 + c3;
@@ -248,37 +246,13 @@ __device__ __forceinline__ void downsweep_body(
     wcnt[3][tid] = 0;
     tile0 = ntile0;
     tile_n = ntile_n;
-    if (PREFETCH) {
 #pragma unroll
-      for (int r = 0; r < kSortItems; ++r) {
-        k[r] = kn[r];
-        v[r] = vn[r];
-      }
-    } else if (ntile_n > 0) {
-      load_tile(keys_in, vals_in, tile0, tile_n, wave, lane, k, v);
+    for (int r = 0; r < kSortItems; ++r) {
+      k[r] = kn[r];
+      v[r] = vn[r];
     }
     __syncthreads();
   }
-}
-
-// Two register budgets: with the prefetch the kernel needs ~220 VGPRs
-// (2 waves/SIMD); without it it fits 168 VGPRs, i.e. 3 workgroups per CU.
-__global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep_pf(
-    const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
-    uint64_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, const int* __restrict__ d_n,
-    int n_host, int max_pairs, int shift, const uint32_t* __restrict__ offsets,
-    const uint32_t* __restrict__ totals) {
-  downsweep_body<true>(keys_in, vals_in, keys_out, vals_out, d_n, n_host, max_pairs, shift,
-                       offsets, totals);
-}
-
-__global__ __launch_bounds__(kSortThreads) __attribute__((amdgpu_waves_per_eu(3))) void
-ps_sort_downsweep(const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
-                  uint64_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
-                  const int* __restrict__ d_n, int n_host, int max_pairs, int shift,
-                  const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ totals) {
-  downsweep_body<false>(keys_in, vals_in, keys_out, vals_out, d_n, n_host, max_pairs, shift,
-                        offsets, totals);
 }
 
 struct SortWs {
@@ -303,13 +277,6 @@ inline size_t sort_layout(int cap, char* base, SortWs* w) {
   if (w) w->totals = reinterpret_cast<uint32_t*>(base + off);
   off += align_up((size_t)kRadix * sizeof(uint32_t), 256);
   return off;
-}
-
-// Tuning switch while the two downsweep forms are compared on hardware:
-// POLYSPLAT_SORT_PREFETCH=1 selects the register-prefetch form.
-inline bool sort_prefetch() {
-  const char* e = getenv("POLYSPLAT_SORT_PREFETCH");
-  return e && e[0] == '1';
 }
 
 }  // namespace
@@ -343,12 +310,8 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
     hipLaunchKernelGGL(ps_sort_scan, dim3(kRadix), dim3(kSortThreads), 0, stream, w.counts,
                        w.offsets, w.totals);
     PS_LAUNCH_CHECK();
-    if (sort_prefetch())
-      hipLaunchKernelGGL(ps_sort_downsweep_pf, dim3(kSortBlocks), dim3(kSortThreads), 0, stream,
-                         kin, vin, kout, vout, d_n, n_host, max_pairs, shift, w.offsets, w.totals);
-    else
-      hipLaunchKernelGGL(ps_sort_downsweep, dim3(kSortBlocks), dim3(kSortThreads), 0, stream, kin,
-                         vin, kout, vout, d_n, n_host, max_pairs, shift, w.offsets, w.totals);
+    hipLaunchKernelGGL(ps_sort_downsweep, dim3(kSortBlocks), dim3(kSortThreads), 0, stream, kin,
+                       vin, kout, vout, d_n, n_host, max_pairs, shift, w.offsets, w.totals);
     PS_LAUNCH_CHECK();
     kin = kout;
     vin = vout;

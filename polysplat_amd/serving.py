@@ -1,0 +1,444 @@
+"""Serving loop: resident scenes, camera requests of mixed resolutions,
+memory-based admission and continuous batching over shared launches.
+
+The reference renders one frame per Python call and blocks on the pair
+count between preprocess and sort (pybind.cpp:1122-1125); it has no serving
+loop.  Here every batch of frames is one call of ``ops.forward_batch``
+(preprocess, sort, range build and persistent render shared by all frames
+of the batch, no host round trip inside), and the loop keeps several
+batches in flight on separate HIP streams:
+
+* requests arrive on the host clock (Poisson process, orbit poses,
+  resolutions drawn uniformly);
+* a request is admitted when the bytes its keys, sort space, tile ranges,
+  feature records and image need fit the key-memory budget next to the
+  batches already on the GPU;
+* whenever a stream slot is free, all admitted-and-waiting requests up to
+  a frame cap and an estimated-GPU-time cap form the next batch (first in,
+  first out), so requests that arrive while others render join the next
+  launch instead of waiting for a fixed batch to fill;
+* a batch whose pairs exceed the estimate is detected from its device
+  counts (read back asynchronously) and its requests are re-queued at the
+  front with the measured count.
+
+Latency is arrival -> image ready: the completion of a batch is the GPU
+timestamp of an event recorded after its render, mapped onto the host
+clock through a start event recorded on an idle device.
+
+The scheduling policy (``Scheduler``) is independent of the GPU, so it is
+tested on the CPU with a simulated executor.
+"""
+from __future__ import annotations
+
+import collections
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import scenes as S
+
+RESOLUTIONS = ((1280, 720), (1920, 1080), (3840, 2160), (7680, 4320))
+SCENE_SIZES = (1_000_000, 3_000_000, 10_000_000, 30_000_000)
+BOX = 50.0
+ORBIT_RADIUS = 1.5 * BOX
+SORT_FIXED_BYTES = 2 * 1024 * 256 * 4 + 4096  # per-block digit counts of the radix sort
+
+
+@dataclass
+class Request:
+    rid: int
+    scene: int
+    width: int
+    height: int
+    azimuth: float
+    arrival_s: float
+    bg: tuple = (0.0, 0.0, 0.0)
+    # filled in by the loop
+    batch: int = -1
+    attempts: int = 0
+    pairs: int = -1
+    done_ms: float = math.nan
+
+    @property
+    def key(self):
+        return (self.scene, self.width, self.height)
+
+    @property
+    def tiles(self):
+        return ((self.width + 15) // 16) * ((self.height + 15) // 16)
+
+    @property
+    def latency_ms(self):
+        return self.done_ms - 1e3 * self.arrival_s
+
+
+def orbit_camera(width: int, height: int, azimuth: float, radius: float = ORBIT_RADIUS,
+                 fov_deg: float = 60.0) -> S.CameraSpec:
+    """Pinhole camera on a horizontal circle around the origin, looking at it.
+
+    The rotation is the camera-to-world matrix given row by row, the layout
+    of cameras.json that the reference passes to its bindings
+    (pybind.cpp:140-143): its third column (the camera's +z) points from the
+    camera to the origin.  Azimuth 0 is the benchmark camera at (0, 0, -r).
+    """
+    import torch
+
+    s, c = math.sin(azimuth), math.cos(azimuth)
+    pos = torch.tensor([radius * s, 0.0, -radius * c], dtype=torch.float32)
+    # rotation about +y by -azimuth: maps (0, 0, 1) to (-sin a, 0, cos a)
+    rot = torch.tensor([[c, 0.0, -s], [0.0, 1.0, 0.0], [s, 0.0, c]], dtype=torch.float32)
+    fx = width / (2.0 * math.tan(math.radians(fov_deg) / 2.0))
+    return S.CameraSpec(width, height, pos, rot, fx, fx)
+
+
+def poisson_requests(n: int, rate: float, n_scenes: int, resolutions=RESOLUTIONS,
+                     seed: int = 7) -> list[Request]:
+    """n requests with exponential inter-arrival times (rate per second),
+    uniform scene, uniform resolution and uniform orbit azimuth."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    sc = rng.integers(0, n_scenes, n)
+    rs = rng.integers(0, len(resolutions), n)
+    az = rng.uniform(0.0, 2.0 * math.pi, n)
+    return [Request(i, int(sc[i]), resolutions[rs[i]][0], resolutions[rs[i]][1], float(az[i]),
+                    float(t[i])) for i in range(n)]
+
+
+def frame_bytes(P: int, width: int, height: int, pairs: int, feat_bytes: int = 24) -> int:
+    """Device bytes one frame needs inside a batch: keys and values (unsorted
+    and sorted), the sort's ping-pong copy, tile ranges, per-Gaussian
+    feature records, the preprocess counts and the image."""
+    tiles = ((width + 15) // 16) * ((height + 15) // 16)
+    return (pairs * (12 + 12 + 12) + tiles * 8 + P * (feat_bytes + 4 + 8) + 3 * width * height
+            + 4096)
+
+
+class Estimates:
+    """Pairs and GPU milliseconds per (scene, width, height), learned from
+    completed frames; pairs keep the largest count seen."""
+
+    def __init__(self, margin: float = 1.08, slack: int = 65536):
+        self.pairs: dict = {}
+        self.ms: dict = {}
+        self.margin = margin
+        self.slack = slack
+
+    def observe(self, key, pairs: int | None = None, ms: float | None = None):
+        if pairs is not None and pairs >= 0:
+            self.pairs[key] = max(self.pairs.get(key, 0), int(pairs))
+        if ms is not None:
+            old = self.ms.get(key)
+            self.ms[key] = ms if old is None else 0.8 * old + 0.2 * ms
+
+    def capacity(self, key) -> int:
+        return int(self.pairs.get(key, 0) * self.margin) + self.slack
+
+    def cost_ms(self, key) -> float:
+        return self.ms.get(key, 1.0)
+
+
+class Scheduler:
+    """Admission and batch formation (no GPU).
+
+    ``budget`` bounds the bytes of all batches in flight.  ``form_batch``
+    takes waiting requests in arrival order while the batch stays within
+    ``max_frames`` and ``max_batch_ms`` of estimated GPU time (a single
+    request is always allowed) and the budget has room; the first request
+    that does not fit ends the batch, so arrival order is kept.  A request
+    that could never fit the whole budget is rejected.
+    """
+
+    def __init__(self, scene_sizes, budget: int, est: Estimates, max_frames: int = 8,
+                 max_batch_ms: float = 6.0, feat_bytes: int = 24):
+        self.P = list(scene_sizes)
+        self.budget = int(budget)
+        self.est = est
+        self.max_frames = max_frames
+        self.max_batch_ms = max_batch_ms
+        self.feat_bytes = feat_bytes
+        self.waiting: collections.deque = collections.deque()
+        self.in_flight_bytes = 0
+        self.rejected: list = []
+
+    def request_bytes(self, r: Request) -> int:
+        return frame_bytes(self.P[r.scene], r.width, r.height, self.est.capacity(r.key),
+                           self.feat_bytes)
+
+    def arrive(self, r: Request):
+        if self.request_bytes(r) + SORT_FIXED_BYTES > self.budget:
+            self.rejected.append(r)
+            return
+        self.waiting.append(r)
+
+    def requeue_front(self, reqs):
+        for r in reversed(list(reqs)):
+            r.attempts += 1
+            self.waiting.appendleft(r)
+
+    def form_batch(self):
+        """Returns (requests, bytes) or ([], 0) when nothing can start now."""
+        batch, nbytes, ms = [], SORT_FIXED_BYTES, 0.0
+        while self.waiting and len(batch) < self.max_frames:
+            r = self.waiting[0]
+            b = self.request_bytes(r)
+            c = self.est.cost_ms(r.key)
+            if batch and ms + c > self.max_batch_ms:
+                break
+            if self.in_flight_bytes + nbytes + b > self.budget:
+                break
+            self.waiting.popleft()
+            batch.append(r)
+            nbytes += b
+            ms += c
+        if not batch:
+            return [], 0
+        self.in_flight_bytes += nbytes
+        return batch, nbytes
+
+    def release(self, nbytes: int):
+        self.in_flight_bytes -= nbytes
+        assert self.in_flight_bytes >= 0
+
+
+@dataclass
+class InFlight:
+    bid: int
+    reqs: list
+    nbytes: int
+    capacity: int
+    slot: int
+    done: object = None            # torch.cuda.Event
+    host_counts: object = None     # pinned int32 [nf + 1]: per-frame counts, then the total
+    buffers: list = field(default_factory=list)
+    outputs: list = field(default_factory=list)
+
+
+class GpuExecutor:
+    """Runs batches on the GPU with ``ops.forward_batch``; one HIP stream per
+    slot, all at the default priority."""
+
+    def __init__(self, scenes_dev, device="cuda:0", slots: int = 2, feat_prec: str = "fp16",
+                 radius: float = ORBIT_RADIUS):
+        import torch
+
+        from . import _C
+
+        self.torch = torch
+        self.ops = _C.ops
+        self.dev = torch.device(device)
+        self.scenes = scenes_dev  # list of dicts: pos, shs, opac, cov (device), P
+        self.slots = [torch.cuda.Stream(device=self.dev) for _ in range(slots)]
+        self.busy = [None] * slots
+        self.feat_prec = {"fp32": 0, "fp16": 1, "bf16": 2}[feat_prec]
+        self.radius = radius
+        self.start = torch.cuda.Event(enable_timing=True)
+        self.t0 = None
+        self._bid = 0
+
+    def begin(self):
+        torch = self.torch
+        torch.cuda.synchronize(self.dev)
+        self.start.record(self.slots[0])
+        torch.cuda.synchronize(self.dev)
+        self.t0 = time.perf_counter()
+
+    def now_s(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def free_slot(self):
+        for i, b in enumerate(self.busy):
+            if b is None:
+                return i
+        return None
+
+    def launch(self, reqs, nbytes, capacity, slot) -> InFlight:
+        torch = self.torch
+        s = self.slots[slot]
+        fl = InFlight(self._bid, reqs, nbytes, capacity, slot)
+        self._bid += 1
+        with torch.cuda.stream(s):
+            dev = self.dev
+            cap = max(int(capacity), 1)
+            ku = torch.empty(cap, dtype=torch.int64, device=dev)
+            vu = torch.empty(cap, dtype=torch.int32, device=dev)
+            ks = torch.empty(cap, dtype=torch.int64, device=dev)
+            vs = torch.empty(cap, dtype=torch.int32, device=dev)
+            space = torch.empty(self.ops.get_sort_buffer_size(cap), dtype=torch.int8, device=dev)
+            curr = torch.empty(1, dtype=torch.int32, device=dev)
+            tiles = sum(r.tiles for r in reqs)
+            ranges = torch.empty((tiles, 2), dtype=torch.int32, device=dev)
+            P = [self.scenes[r.scene]["P"] for r in reqs]
+            ws_bytes = self.ops.forward_batch_workspace_size(P)
+            ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+            ws[:256].zero_()  # persistent-queue words start at zero
+            frames, outs, recs = [], [], []
+            empty = torch.empty(0, dtype=torch.float32, device=dev)
+            for r in reqs:
+                sc = self.scenes[r.scene]
+                cam = orbit_camera(r.width, r.height, r.azimuth, self.radius)
+                out = torch.empty((r.height, r.width, 3), dtype=torch.uint8, device=dev)
+                rec = torch.empty(24 * max(sc["P"], 1), dtype=torch.uint8, device=dev)
+                bg = torch.tensor(r.bg, dtype=torch.float32)
+                frames.append((sc["pos"], sc["shs"], sc["opac"], sc["cov"], r.width, r.height,
+                               cam.position, cam.rotation, cam.focal_x, cam.focal_y, cam.zFar,
+                               cam.zNear, bg, out, rec, empty, empty, empty))
+                outs.append(out)
+                recs.append(rec)
+            self.ops.forward_batch(frames, self.feat_prec, curr, space, ku, vu, ks, vs, ranges,
+                                   ws, False)
+            off = self.ops.forward_batch_counts_offset(P)
+            host = torch.empty(len(reqs) + 1, dtype=torch.int32, pin_memory=True)
+            host[:len(reqs)].copy_(ws[off:off + 4 * len(reqs)].view(torch.int32),
+                                   non_blocking=True)
+            host[len(reqs):].copy_(curr, non_blocking=True)
+            fl.done = torch.cuda.Event(enable_timing=True)
+            fl.done.record(s)
+        fl.host_counts = host
+        fl.buffers = [ku, vu, ks, vs, space, curr, ranges, ws, recs]
+        fl.outputs = outs
+        self.busy[slot] = fl
+        return fl
+
+    def poll(self):
+        """Completed batches (their slots are freed)."""
+        out = []
+        for i, fl in enumerate(self.busy):
+            if fl is not None and fl.done.query():
+                done_ms = self.start.elapsed_time(fl.done)
+                counts = fl.host_counts.tolist()
+                self.busy[i] = None
+                out.append((fl, done_ms, counts[:-1], counts[-1]))
+        return out
+
+    def idle(self) -> bool:
+        return all(b is None for b in self.busy)
+
+
+class ServingLoop:
+    """Drives requests through a Scheduler and an executor in real time."""
+
+    def __init__(self, scheduler: Scheduler, executor, keep_outputs: bool = False):
+        self.sched = scheduler
+        self.ex = executor
+        self.keep_outputs = keep_outputs
+        self.outputs: dict = {}
+        self.batches: list = []  # (bid, [rid...], launch_s, done_ms)
+        self.overflows = 0
+
+    def run(self, requests, spin_s: float = 2e-4):
+        reqs = sorted(requests, key=lambda r: r.arrival_s)
+        pending = collections.deque(reqs)
+        done: list = []
+        self.ex.begin()
+        n_total = len(reqs)
+        while len(done) + len(self.sched.rejected) < n_total:
+            now = self.ex.now_s()
+            while pending and pending[0].arrival_s <= now:
+                self.sched.arrive(pending.popleft())
+            progressed = False
+            for fl, done_ms, counts, total in self.ex.poll():
+                progressed = True
+                self.sched.release(fl.nbytes)
+                if total > fl.capacity:
+                    # not rendered: learn the real counts and retry first
+                    self.overflows += 1
+                    for r, c in zip(fl.reqs, counts):
+                        self.sched.est.observe(r.key, pairs=c)
+                    self.sched.requeue_front(fl.reqs)
+                    continue
+                for r, c, img in zip(fl.reqs, counts, fl.outputs):
+                    r.pairs = c
+                    r.done_ms = done_ms
+                    r.batch = fl.bid
+                    self.sched.est.observe(r.key, pairs=c)
+                    if self.keep_outputs:
+                        self.outputs[r.rid] = img
+                    done.append(r)
+                self.batches.append((fl.bid, [r.rid for r in fl.reqs], done_ms))
+            while True:
+                slot = self.ex.free_slot()
+                if slot is None:
+                    break
+                batch, nbytes = self.sched.form_batch()
+                if not batch:
+                    break
+                cap = sum(self.sched.est.capacity(r.key) for r in batch)
+                self.ex.launch(batch, nbytes, cap, slot)
+                progressed = True
+            if not progressed:
+                if not pending and not self.sched.waiting and self.ex.idle():
+                    break
+                time.sleep(spin_s)
+        return done
+
+
+def summarize(done, rejected, wall_s: float, deadline_ms: float = 1000.0 / 60.0,
+              batches=None) -> dict:
+    lat = np.array([r.latency_ms for r in done], dtype=np.float64)
+    res = {
+        "requests": len(done) + len(rejected),
+        "completed": len(done),
+        "rejected": len(rejected),
+        "p50_ms": float(np.percentile(lat, 50)) if len(lat) else math.nan,
+        "p90_ms": float(np.percentile(lat, 90)) if len(lat) else math.nan,
+        "p99_ms": float(np.percentile(lat, 99)) if len(lat) else math.nan,
+        "max_ms": float(lat.max()) if len(lat) else math.nan,
+        "deadline_ms": deadline_ms,
+        "deadline_miss_frac": float((lat > deadline_ms).mean()) if len(lat) else math.nan,
+        "frames_per_s": len(done) / wall_s if wall_s > 0 else math.nan,
+        "wall_s": wall_s,
+    }
+    if batches:
+        sizes = np.array([len(b[1]) for b in batches])
+        res["batches"] = len(batches)
+        res["mean_frames_per_batch"] = float(sizes.mean())
+        res["max_frames_per_batch"] = int(sizes.max())
+    return res
+
+
+def make_resident_scenes(sizes=SCENE_SIZES, device="cuda:0", seed0: int = 101, box: float = BOX,
+                         mix=None):
+    """Scenes generated on the device with the C2 distributions (uniform
+    means in a `box` cube, log-scales N(-4.5, 0.5)) and stored with FP16 SH."""
+    import torch
+
+    mix = mix if mix is not None else S.CONFIGS["c2"]["mix"]
+    out = []
+    for i, P in enumerate(sizes):
+        sc = S.make_scene(P, box, mix, seed=seed0 + i, device=device)
+        out.append({"P": P, "pos": sc.position, "shs": sc.shs.to(torch.float16).contiguous(),
+                    "opac": sc.opacity, "cov": sc.cov3d})
+        del sc
+    return out
+
+
+def calibrate(executor: GpuExecutor, est: Estimates, n_scenes: int, resolutions=RESOLUTIONS,
+              azimuths=(0.0, math.pi / 4), repeats: int = 3):
+    """Renders every (scene, resolution) alone at a few orbit poses: warms the
+    kernels and seeds the pair and time estimates."""
+    torch = executor.torch
+    executor.begin()
+    for s in range(n_scenes):
+        for (w, h) in resolutions:
+            for a in azimuths:
+                cap = 1 << 20
+                for _ in range(4):
+                    r = Request(-1, s, w, h, a, 0.0)
+                    fl = executor.launch([r], 0, cap, 0)
+                    torch.cuda.synchronize(executor.dev)
+                    (fl2, _, counts, total), = executor.poll()
+                    if total <= cap:
+                        break
+                    cap = int(total * 1.05) + 4096
+                est.observe(r.key, pairs=total)
+                t = []
+                for _ in range(repeats):
+                    ev0 = torch.cuda.Event(enable_timing=True)
+                    ev0.record(executor.slots[0])
+                    executor.launch([r], 0, cap, 0)
+                    torch.cuda.synchronize(executor.dev)
+                    (fl2, _, _, _), = executor.poll()
+                    t.append(ev0.elapsed_time(fl2.done))
+                est.observe(r.key, ms=float(np.median(t)))

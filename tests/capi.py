@@ -17,6 +17,7 @@ HEADER = ROOT / "include" / "polysplat_amd.h"
 PS_OK, PS_ERR_INVALID, PS_ERR_HIP, PS_ERR_OVERFLOW, PS_ERR_UNSUPPORTED = range(5)
 F32, F16, BF16 = 0, 1, 2
 V_DEFAULT, V_LITE, V_FUSED, V_DT = 0, 1, 2, 3
+MAX_BATCH_FRAMES = 64
 
 
 class PsCamera(C.Structure):
@@ -34,6 +35,14 @@ class PsFrameBuffers(C.Structure):
                 ("rgb_depth", C.c_void_p), ("conic_opacity", C.c_void_p),
                 ("records", C.c_void_p), ("tile_ranges", C.c_void_p),
                 ("workspace", C.c_void_p), ("workspace_bytes", C.c_size_t)]
+
+
+class PsFrame(C.Structure):
+    _fields_ = [("P", C.c_int), ("positions", C.c_void_p), ("shs", C.c_void_p),
+                ("opacities", C.c_void_p), ("cov3d", C.c_void_p), ("cam", PsCamera),
+                ("bg", C.c_float * 3), ("points_xy", C.c_void_p), ("rgb_depth", C.c_void_p),
+                ("conic_opacity", C.c_void_p), ("records", C.c_void_p),
+                ("out_color", C.c_void_p)]
 
 
 def header_functions() -> list[str]:
@@ -74,6 +83,9 @@ def lib():
                                            I),
             "ps_forward_workspace_size": ([I], S),
             "ps_forward": ([I, P, P, I, P, P, P, I, I, P, P, P, I, P, P], I),
+            "ps_forward_batch_workspace_size": ([I, P], S),
+            "ps_forward_batch": ([I, P, I, I, P, I, P, P], I),
+            "ps_forward_batch_counts_ptr": ([P, I, P], P),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

@@ -73,3 +73,28 @@ def test_invalid_arguments_are_rejected_without_gpu():
     assert b"camera" in L.ps_last_error() or b"NULL" in L.ps_last_error()
     st = L.ps_sort_gaussian(5, None, 4, 64, 64, 16, 16, None, 0, None, None, None, None, None)
     assert st == capi.PS_ERR_INVALID
+
+
+def test_batch_workspace_and_argument_checks_without_gpu():
+    import ctypes as C
+
+    L = capi.lib()
+    P = (C.c_int * 3)(1000, 0, 50_000)
+    one = (C.c_int * 1)(1000)
+    w3 = L.ps_forward_batch_workspace_size(3, P)
+    assert w3 > L.ps_forward_batch_workspace_size(1, one) > 0
+    assert L.ps_forward_batch_workspace_size(0, P) == 0
+    frames = (capi.PsFrame * 2)()
+    fb = capi.PsFrameBuffers()
+    # too many frames, no frames, NULL arguments: rejected before any HIP call
+    assert L.ps_forward_batch(capi.MAX_BATCH_FRAMES + 1, frames, 0, 1, C.byref(fb), 0, None,
+                              None) == capi.PS_ERR_INVALID
+    assert L.ps_forward_batch(0, frames, 0, 1, C.byref(fb), 0, None, None) == capi.PS_ERR_INVALID
+    assert L.ps_forward_batch(1, None, 0, 1, C.byref(fb), 0, None, None) == capi.PS_ERR_INVALID
+    # frame without 16x16 tiles
+    frames[0].cam.width, frames[0].cam.height = 64, 64
+    frames[0].cam.block_x, frames[0].cam.block_y = 8, 8
+    fb.curr_offset = 1
+    fb.tile_ranges = 1
+    assert L.ps_forward_batch(1, frames, 0, 1, C.byref(fb), 0, None, None) == capi.PS_ERR_INVALID
+    assert b"16x16" in L.ps_last_error()
