@@ -225,7 +225,10 @@ int ps_forward(int P, const float* positions, const void* shs, int sh_prec,
  * thread are still waiting for their descriptor copy).  With sync_count != 0
  * the call waits, writes each frame's pair count to frame_counts (host, may
  * be NULL) and returns PS_ERR_OVERFLOW when the pairs did not fit (then no
- * keys were written and every image shows its background). */
+ * keys were written and every image shows its background).
+ *
+ * Status: currently switched off -- after validating its arguments the call
+ * returns PS_ERR_UNSUPPORTED without touching the device. */
 #define PS_MAX_BATCH_FRAMES 64
 
 typedef struct ps_frame {

@@ -20,12 +20,7 @@ int preprocess_launch(int P, const float* positions, const void* shs, int sh_pre
                       int feat_prec, uint64_t* keys, uint32_t* values, int capacity,
                       int* curr_offset, void* workspace, size_t workspace_bytes,
                       hipStream_t stream, bool reset_offset);
-int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* space,
-                size_t space_bytes, const uint64_t* keys_in, const uint32_t* vals_in,
-                uint64_t* keys_out, uint32_t* vals_out, hipStream_t stream);
 int sort_end_bit(int width, int height, int block_x, int block_y);
-int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles,
-                            const uint64_t* keys, int* ranges, hipStream_t stream);
 int render_plain_launch(int n_host, const int* d_n, int W, int H, const float* xy,
                         const float* rgbd, const float* con, const uint64_t* keys,
                         const uint32_t* order, const float bg[3], uint8_t* out,
@@ -207,21 +202,19 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
                          fb->curr_offset, ws, pre_bytes, stream, true);
   if (st != PS_OK) return st;
   const int* d_n = ps_preprocess_count_ptr(ws);
+  // The sort leaves the tile ranges in tile_ranges when the renderer needs
+  // them (its segment pass computes them anyway).
+  const int tiles = ((cam->width + 15) / 16) * ((cam->height + 15) / 16);
   st = sort_launch(0, d_n, fb->capacity, sort_end_bit(cam->width, cam->height, 16, 16),
                    fb->sort_space, fb->sort_space_bytes, fb->keys_unsorted, fb->values_unsorted,
-                   fb->keys_sorted, fb->values_sorted, stream);
+                   fb->keys_sorted, fb->values_sorted, stream,
+                   need_ranges ? fb->tile_ranges : nullptr, tiles);
   if (st != PS_OK) return st;
-  const int tiles = ((cam->width + 15) / 16) * ((cam->height + 15) / 16);
   if (variant == PS_VARIANT_DEFAULT) {
     st = render_plain_launch(0, d_n, cam->width, cam->height, fb->points_xy, fb->rgb_depth,
                              fb->conic_opacity, fb->keys_sorted, fb->values_sorted, bg, out_color,
                              stream);
   } else {
-    if (need_ranges) {
-      st = tile_ranges_scan_launch(0, d_n, fb->capacity, tiles, fb->keys_sorted, fb->tile_ranges,
-                                   stream);
-      if (st != PS_OK) return st;
-    }
     st = render_persistent_launch(0, d_n, cam->width, cam->height, feat_prec, fb->points_xy,
                                   fb->rgb_depth, fb->conic_opacity, fb->records, 0,
                                   fb->keys_sorted, fb->values_sorted,
@@ -243,6 +236,10 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
   }
   return PS_OK;
 }
+
+namespace ps {
+static bool batch_enabled() { return false; }
+}  // namespace ps
 
 extern "C" size_t ps_forward_batch_workspace_size(int num_frames, const int* P) {
   if (num_frames <= 0 || P == nullptr) return 0;
@@ -285,6 +282,13 @@ extern "C" int ps_forward_batch(int num_frames, const ps_frame* frames, int sh_p
     if (st != PS_OK) return st;
     P[f] = q.P;
   }
+  // Switched off: a GPU run of the batched path faulted the device and the
+  // cause has not been found yet.  Arguments are still validated.
+  if (!batch_enabled()) {
+    set_last_error("batched frames are switched off in this build; render the frames with "
+                   "ps_forward");
+    return PS_ERR_UNSUPPORTED;
+  }
   const size_t pre_bytes = align_up(preprocess_batch_workspace_size(num_frames, P), 256);
   PS_REQUIRE(fb->workspace != nullptr && fb->workspace_bytes >= pre_bytes + kQueueBytes,
              "batch workspace too small (%zu < %zu)", fb->workspace_bytes, pre_bytes + kQueueBytes);
@@ -301,10 +305,7 @@ extern "C" int ps_forward_batch(int num_frames, const ps_frame* frames, int sh_p
   if (st != PS_OK) return st;
   st = sort_launch(0, d_n, fb->capacity, 32 + bit_length((unsigned)tiles), fb->sort_space,
                    fb->sort_space_bytes, fb->keys_unsorted, fb->values_unsorted, fb->keys_sorted,
-                   fb->values_sorted, stream);
-  if (st != PS_OK) return st;
-  st = tile_ranges_scan_launch(0, d_n, fb->capacity, tiles, fb->keys_sorted, fb->tile_ranges,
-                               stream);
+                   fb->values_sorted, stream, fb->tile_ranges, tiles);
   if (st != PS_OK) return st;
   st = render_batch_launch(batch_frames_dev(num_frames, host, ws), num_frames,
                            feat_prec, fb->tile_ranges, tiles, fb->values_sorted, queue, stream);

@@ -48,25 +48,30 @@ __global__ __launch_bounds__(kThreads) void ps_ranges_bsearch(const uint64_t* __
   ranges[t] = make_int2(start, lo);
 }
 
-__global__ __launch_bounds__(kThreads) void ps_fill_int2(int2* __restrict__ p, int count) {
+// Zeroes `count` ranges and, when given, one extra counter word.
+__global__ __launch_bounds__(kThreads) void ps_fill_int2(int2* __restrict__ p, int count,
+                                                         int* __restrict__ counter) {
+  if (counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < count; i += gridDim.x * kThreads)
     p[i] = make_int2(0, 0);
 }
 
 // Boundary scan: writes starts/ends only where the tile id changes, so empty
-// tiles keep the (0, 0) written by ps_fill_int2.
+// tiles keep the (0, 0) written by ps_fill_int2.  The tile id is the key's
+// high word masked with `mask` (all ones for render ranges; the radix sort's
+// segment pass keeps only the bits it sorted).
 __global__ __launch_bounds__(kThreads) void ps_ranges_scan(const uint64_t* __restrict__ keys,
                                                            const int* __restrict__ d_n, int n_host,
-                                                           int max_pairs, int tiles,
+                                                           int max_pairs, int tiles, uint32_t mask,
                                                            int2* __restrict__ ranges) {
   const int n = count_of(d_n, n_host, max_pairs);
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-    const uint32_t cur = tile_of(keys, i);
+    const uint32_t cur = tile_of(keys, i) & mask;
     const bool cur_ok = cur < (uint32_t)tiles;  // keys from a foreign caller may be garbage
     if (i == 0) {
       if (cur_ok) ranges[cur].x = 0;
     } else {
-      const uint32_t prev = tile_of(keys, i - 1);
+      const uint32_t prev = tile_of(keys, i - 1) & mask;
       if (cur != prev) {
         if (prev < (uint32_t)tiles) ranges[prev].y = i;
         if (cur_ok) ranges[cur].x = i;
@@ -109,14 +114,15 @@ inline int grid_for(int n) {
 }  // namespace
 
 int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles,
-                            const uint64_t* keys, int* ranges, hipStream_t stream) {
+                            const uint64_t* keys, int* ranges, hipStream_t stream, uint32_t mask,
+                            int* zero_counter) {
   hipLaunchKernelGGL(ps_fill_int2, dim3(grid_for(tiles)), dim3(kThreads), 0, stream,
-                     reinterpret_cast<int2*>(ranges), tiles);
+                     reinterpret_cast<int2*>(ranges), tiles, zero_counter);
   PS_LAUNCH_CHECK();
   const int n = d_n ? max_pairs : n_host;
   if (n > 0) {
     hipLaunchKernelGGL(ps_ranges_scan, dim3(grid_for(n)), dim3(kThreads), 0, stream, keys, d_n,
-                       n_host, max_pairs, tiles, reinterpret_cast<int2*>(ranges));
+                       n_host, max_pairs, tiles, mask, reinterpret_cast<int2*>(ranges));
     PS_LAUNCH_CHECK();
   }
   return PS_OK;

@@ -22,6 +22,8 @@
 #include "ps_host.h"
 #include "ps_wave.h"
 
+#include <stdlib.h>
+
 
 namespace ps {
 namespace {
@@ -255,13 +257,248 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
   }
 }
 
+
+// ------------------------------------------------------------ segment pass
+// After the digit passes over the tile bits [32, end_bit), pairs are grouped
+// by tile (segment) and keep their input order inside each segment; what is
+// left is a stable sort of every segment by the low 32 bits.  That needs no
+// global passes: a wave sorts a segment of up to kSegCap pairs in its own
+// LDS (four 8-bit counting passes); larger segments are listed and a
+// workgroup sorts each with four counting passes through global memory.
+// The result equals the full LSD sort bit for bit: sorting by (high, low)
+// stably is sorting by the whole key stably.
+
+constexpr int kSegCap = 896;    // pairs per wave-sorted segment (LDS stays under 64 KiB)
+constexpr int kSegWaves = 4;    // waves per workgroup of the small-segment kernel
+
+// Lanes of the wave whose 8-bit digit equals this lane's (valid lanes only).
+__device__ __forceinline__ uint64_t match_digit(int d, bool ok) {
+  uint64_t peers = ballot(ok);
+#pragma unroll
+  for (int bit = 0; bit < 8; ++bit) {
+    const bool set = ok && ((d >> bit) & 1);
+    const uint64_t bb = ballot(set);
+    peers &= set ? bb : ~bb;
+  }
+  return peers;
+}
+
+// One stable counting pass of a wave's n LDS items (low words sd, values sv)
+// by digit `shift`, into dd/dv.  Items are taken in index order, 64 at a
+// time; a wave's LDS operations execute in order, so the per-digit counters
+// need no barrier between rounds.
+__device__ __forceinline__ void wave_counting_pass(const uint32_t* sd, const uint32_t* sv,
+                                                   uint32_t* dd, uint32_t* dv, uint32_t* cnt,
+                                                   int n, int shift, int lane, uint64_t below) {
+  for (int i = lane; i < kRadix; i += kWave) cnt[i] = 0;
+  __builtin_amdgcn_wave_barrier();
+  const int rounds = (n + kWave - 1) / kWave;
+  for (int r = 0; r < rounds; ++r) {
+    const int j = r * kWave + lane;
+    const bool ok = j < n;
+    const int d = ok ? (int)((sd[j] >> shift) & 0xffu) : 0;
+    const uint64_t peers = match_digit(d, ok);
+    if (ok && (peers & below) == 0) cnt[d] += (uint32_t)popc64(peers);
+    __builtin_amdgcn_wave_barrier();
+  }
+  const uint32_t c0 = cnt[4 * lane], c1 = cnt[4 * lane + 1], c2 = cnt[4 * lane + 2],
+                 c3 = cnt[4 * lane + 3];
+  const int sum = (int)(c0 + c1 + c2 + c3);
+  const uint32_t ex = (uint32_t)(wave_incl_scan(sum, lane) - sum);
+  __builtin_amdgcn_wave_barrier();
+  cnt[4 * lane] = ex;
+  cnt[4 * lane + 1] = ex + c0;
+  cnt[4 * lane + 2] = ex + c0 + c1;
+  cnt[4 * lane + 3] = ex + c0 + c1 + c2;
+  __builtin_amdgcn_wave_barrier();
+  for (int r = 0; r < rounds; ++r) {
+    const int j = r * kWave + lane;
+    const bool ok = j < n;
+    const uint32_t lo = ok ? sd[j] : 0u;
+    const uint32_t v = ok ? sv[j] : 0u;
+    const int d = (int)((lo >> shift) & 0xffu);
+    const uint64_t peers = match_digit(d, ok);
+    const uint32_t rank = (uint32_t)popc64(peers & below);
+    const uint32_t base = ok ? cnt[d] : 0u;
+    __builtin_amdgcn_wave_barrier();
+    if (ok && rank == 0) cnt[d] = base + (uint32_t)popc64(peers);
+    __builtin_amdgcn_wave_barrier();
+    if (ok) {
+      dd[base + rank] = lo;
+      dv[base + rank] = v;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Each wave takes segments seg = wave id, wave id + total waves, ...  A
+// segment longer than kSegCap, or whose keys' high words differ (bits above
+// the sorted ones), goes to the list for ps_seg_big.  Sorted in place.
+__global__ __launch_bounds__(kSegWaves * kWave) void ps_seg_small(
+    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, const int2* __restrict__ ranges,
+    int segs, int* __restrict__ big_list, int* __restrict__ big_count, int big_cap) {
+  __shared__ uint32_t a_lo[kSegWaves][kSegCap];
+  __shared__ uint32_t a_v[kSegWaves][kSegCap];
+  __shared__ uint32_t b_lo[kSegWaves][kSegCap];
+  __shared__ uint32_t b_v[kSegWaves][kSegCap];
+  __shared__ uint32_t cnt[kSegWaves][kRadix];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t below = lanes_below(lane);
+  for (int seg = blockIdx.x * kSegWaves + wave; seg < segs; seg += gridDim.x * kSegWaves) {
+    const int2 r = ranges[seg];
+    const int n = r.y - r.x;
+    if (n <= 1) continue;
+    bool big = n > kSegCap;
+    uint32_t hi0 = 0;
+    if (!big) {
+      hi0 = (uint32_t)(keys[r.x] >> 32);
+      bool same = true;
+      for (int j = lane; j < n; j += kWave) {
+        const uint64_t k = keys[r.x + j];
+        same = same && (uint32_t)(k >> 32) == hi0;
+        a_lo[wave][j] = (uint32_t)k;
+        a_v[wave][j] = vals[r.x + j];
+      }
+      big = ballot(!same) != 0;
+    }
+    if (big) {
+      if (lane == 0) {
+        const int at = atomicAdd(big_count, 1);
+        if (at < big_cap) big_list[at] = seg;
+      }
+      continue;
+    }
+    __builtin_amdgcn_wave_barrier();
+    wave_counting_pass(a_lo[wave], a_v[wave], b_lo[wave], b_v[wave], cnt[wave], n, 0, lane, below);
+    wave_counting_pass(b_lo[wave], b_v[wave], a_lo[wave], a_v[wave], cnt[wave], n, 8, lane, below);
+    wave_counting_pass(a_lo[wave], a_v[wave], b_lo[wave], b_v[wave], cnt[wave], n, 16, lane, below);
+    wave_counting_pass(b_lo[wave], b_v[wave], a_lo[wave], a_v[wave], cnt[wave], n, 24, lane, below);
+    const uint64_t hi = (uint64_t)hi0 << 32;
+    for (int j = lane; j < n; j += kWave) {
+      keys[r.x + j] = hi | a_lo[wave][j];
+      vals[r.x + j] = a_v[wave][j];
+    }
+  }
+}
+
+// Listed segments: a workgroup per segment, four stable counting passes by
+// the low 32 bits through global memory (keys <-> tmp, ending in keys).
+// Chunks of 1024 pairs: wave w ranks items [256w, 256w + 256) of the chunk
+// against its own LDS counters, then the per-wave counts are combined.
+__global__ __launch_bounds__(kSortThreads) void ps_seg_big(
+    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint64_t* __restrict__ tkeys,
+    uint32_t* __restrict__ tvals, const int2* __restrict__ ranges,
+    const int* __restrict__ big_list, const int* __restrict__ big_count, int big_cap) {
+  constexpr int kPer = 4;  // rounds of 64 per wave and chunk
+  constexpr int kChunk = kSortThreads * kPer;
+  __shared__ uint32_t wcnt[4][kRadix];
+  __shared__ uint32_t base[kRadix];
+  __shared__ int scratch[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t below = lanes_below(lane);
+  const int nbig = min(*big_count, big_cap);
+  for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+    const int2 r = ranges[big_list[b]];
+    const int n = r.y - r.x;
+    for (int p = 0; p < 4; ++p) {
+      const int shift = 8 * p;
+      const uint64_t* sk = ((p & 1) ? tkeys : keys) + r.x;
+      const uint32_t* sv = ((p & 1) ? tvals : vals) + r.x;
+      uint64_t* dk = ((p & 1) ? keys : tkeys) + r.x;
+      uint32_t* dv = ((p & 1) ? vals : tvals) + r.x;
+      // digit histogram
+      wcnt[0][tid] = 0;
+      wcnt[1][tid] = 0;
+      wcnt[2][tid] = 0;
+      wcnt[3][tid] = 0;
+      __syncthreads();
+      for (int i = tid; i < n; i += kSortThreads)
+        atomicAdd(&wcnt[wave][digit_of(sk[i], shift)], 1u);
+      __syncthreads();
+      int total;
+      const int ex = block256_excl_scan(
+          (int)(wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid]), scratch, total);
+      base[tid] = (uint32_t)ex;
+      wcnt[0][tid] = 0;
+      wcnt[1][tid] = 0;
+      wcnt[2][tid] = 0;
+      wcnt[3][tid] = 0;
+      __syncthreads();
+      for (int c0 = 0; c0 < n; c0 += kChunk) {
+        uint64_t k[kPer];
+        uint32_t v[kPer], pos[kPer];
+        int dg[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+          const int i = c0 + wave * (kWave * kPer) + q * kWave + lane;
+          const bool ok = i < n;
+          k[q] = ok ? sk[i] : 0ull;
+          v[q] = ok ? sv[i] : 0u;
+          dg[q] = ok ? digit_of(k[q], shift) : -1;
+        }
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+          const bool ok = dg[q] >= 0;
+          const int d = ok ? dg[q] : 0;
+          const uint64_t peers = match_digit(d, ok);
+          const uint32_t rank = (uint32_t)popc64(peers & below);
+          const uint32_t bw = ok ? wcnt[wave][d] : 0u;
+          __builtin_amdgcn_wave_barrier();
+          if (ok && rank == 0) wcnt[wave][d] = bw + (uint32_t)popc64(peers);
+          __builtin_amdgcn_wave_barrier();
+          pos[q] = bw + rank;
+        }
+        __syncthreads();
+        // per digit: counts of the four waves -> offsets of each wave's run
+        const uint32_t w0 = wcnt[0][tid], w1 = wcnt[1][tid], w2 = wcnt[2][tid], w3 = wcnt[3][tid];
+        const uint32_t b0 = base[tid];
+        __syncthreads();
+        wcnt[0][tid] = b0;
+        wcnt[1][tid] = b0 + w0;
+        wcnt[2][tid] = b0 + w0 + w1;
+        wcnt[3][tid] = b0 + w0 + w1 + w2;
+        base[tid] = b0 + w0 + w1 + w2 + w3;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+          if (dg[q] >= 0) {
+            const uint32_t dst = wcnt[wave][dg[q]] + pos[q];
+            dk[dst] = k[q];
+            dv[dst] = v[q];
+          }
+        }
+        __syncthreads();
+        wcnt[0][tid] = 0;
+        wcnt[1][tid] = 0;
+        wcnt[2][tid] = 0;
+        wcnt[3][tid] = 0;
+        __syncthreads();
+      }
+      // the pass's writes must be visible to the next pass's reads
+      __threadfence_block();
+      __syncthreads();
+    }
+  }
+}
+
 struct SortWs {
   uint64_t* alt_keys;
   uint32_t* alt_vals;
   uint32_t* counts;
   uint32_t* offsets;
   uint32_t* totals;
+  int2* seg_ranges;  // segment ranges when the caller gives none
+  int* big_list;     // segments for ps_seg_big
+  int* big_count;
 };
+
+// Segments (values of the sorted high bits) the internal range table holds.
+constexpr int kSegMaxSegs = 1 << 18;
+
+// Entries the list of segments for ps_seg_big can take: every segment longer
+// than kSegCap (at most pairs / (kSegCap + 1)), plus every internal segment
+// whose keys differ above the sorted bits.
+inline size_t big_list_cap(size_t pairs) { return pairs / (kSegCap + 1) + 1 + kSegMaxSegs; }
 
 inline size_t sort_layout(int cap, char* base, SortWs* w) {
   size_t off = 0;
@@ -276,26 +513,27 @@ inline size_t sort_layout(int cap, char* base, SortWs* w) {
   off += align_up((size_t)kRadix * kSortBlocks * sizeof(uint32_t), 256);
   if (w) w->totals = reinterpret_cast<uint32_t*>(base + off);
   off += align_up((size_t)kRadix * sizeof(uint32_t), 256);
+  if (w) w->seg_ranges = reinterpret_cast<int2*>(base + off);
+  off += align_up((size_t)kSegMaxSegs * sizeof(int2), 256);
+  if (w) w->big_count = reinterpret_cast<int*>(base + off);
+  off += 256;
+  if (w) w->big_list = reinterpret_cast<int*>(base + off);
+  off += align_up(big_list_cap(c) * sizeof(int), 256);
   return off;
 }
 
-}  // namespace
+// Sort algorithm: full LSD over every key bit, or digit passes over the
+// tile bits followed by the per-segment pass.  The segmented form is the
+// default; POLYSPLAT_SORT=lsd selects the full LSD (both are exact).
+inline bool use_segmented() {
+  const char* e = getenv("POLYSPLAT_SORT");
+  return !(e && e[0] == 'l');
+}
 
-int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* space,
-                size_t space_bytes, const uint64_t* keys_in, const uint32_t* vals_in,
-                uint64_t* keys_out, uint32_t* vals_out, hipStream_t stream) {
-  PS_REQUIRE(max_pairs >= 0, "max_pairs must be >= 0");
-  PS_REQUIRE(d_n != nullptr || (n_host >= 0 && n_host <= max_pairs),
-             "num_rendered %d outside [0, %d]", n_host, max_pairs);
-  PS_REQUIRE(space != nullptr && space_bytes >= sort_layout(max_pairs, nullptr, nullptr),
-             "sort space too small (%zu < %zu)", space_bytes,
-             sort_layout(max_pairs, nullptr, nullptr));
-  PS_REQUIRE(end_bit > 0 && end_bit <= 64, "bad end bit %d", end_bit);
-  PS_REQUIRE(keys_in != keys_out && vals_in != vals_out, "sort outputs must not alias inputs");
-  if (d_n == nullptr && n_host == 0) return PS_OK;
-  SortWs w;
-  sort_layout(max_pairs, static_cast<char*>(space), &w);
-  const int passes = (end_bit + 7) / 8;
+int digit_passes(int n_host, const int* d_n, int max_pairs, int bit0, int bit1, const SortWs& w,
+                 const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
+                 uint32_t* vals_out, hipStream_t stream) {
+  const int passes = (bit1 - bit0 + 7) / 8;
   const uint64_t* kin = keys_in;
   const uint32_t* vin = vals_in;
   for (int p = 0; p < passes; ++p) {
@@ -303,7 +541,7 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
     const bool to_out = ((passes - 1 - p) % 2) == 0;
     uint64_t* kout = to_out ? keys_out : w.alt_keys;
     uint32_t* vout = to_out ? vals_out : w.alt_vals;
-    const int shift = 8 * p;
+    const int shift = bit0 + 8 * p;
     hipLaunchKernelGGL(ps_sort_upsweep, dim3(kSortBlocks), dim3(kSortThreads), 0, stream, kin,
                        d_n, n_host, max_pairs, shift, w.counts);
     PS_LAUNCH_CHECK();
@@ -316,6 +554,70 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
     kin = kout;
     vin = vout;
   }
+  return PS_OK;
+}
+
+}  // namespace
+
+int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* space,
+                size_t space_bytes, const uint64_t* keys_in, const uint32_t* vals_in,
+                uint64_t* keys_out, uint32_t* vals_out, hipStream_t stream, int* tile_ranges,
+                int tiles) {
+  PS_REQUIRE(max_pairs >= 0, "max_pairs must be >= 0");
+  PS_REQUIRE(d_n != nullptr || (n_host >= 0 && n_host <= max_pairs),
+             "num_rendered %d outside [0, %d]", n_host, max_pairs);
+  PS_REQUIRE(space != nullptr && space_bytes >= sort_layout(max_pairs, nullptr, nullptr),
+             "sort space too small (%zu < %zu)", space_bytes,
+             sort_layout(max_pairs, nullptr, nullptr));
+  PS_REQUIRE(end_bit > 0 && end_bit <= 64, "bad end bit %d", end_bit);
+  PS_REQUIRE(keys_in != keys_out && vals_in != vals_out, "sort outputs must not alias inputs");
+  PS_REQUIRE(tile_ranges == nullptr || tiles > 0, "tile ranges need a tile count");
+  SortWs w;
+  sort_layout(max_pairs, static_cast<char*>(space), &w);
+  const int seg_bits = end_bit - 32;
+  const bool segmented = use_segmented() && seg_bits > 0 &&
+                         (tile_ranges != nullptr || seg_bits <= 18);
+  if (!segmented) {
+    if (d_n == nullptr && n_host == 0) {
+      if (tile_ranges) return tile_ranges_scan_launch(0, nullptr, 0, tiles, keys_out,
+                                                      tile_ranges, stream);
+      return PS_OK;
+    }
+    int st = digit_passes(n_host, d_n, max_pairs, 0, end_bit, w, keys_in, vals_in, keys_out,
+                          vals_out, stream);
+    if (st != PS_OK || tile_ranges == nullptr) return st;
+    return tile_ranges_scan_launch(n_host, d_n, max_pairs, tiles, keys_out, tile_ranges, stream);
+  }
+  // Digit passes over the tile bits, then segment ranges, then the segment
+  // pass in place in the output.
+  int* ranges = tile_ranges;
+  int segs = tiles;
+  uint32_t mask = 0xffffffffu;
+  if (ranges == nullptr) {
+    ranges = reinterpret_cast<int*>(w.seg_ranges);
+    segs = 1 << seg_bits;
+    mask = (uint32_t)segs - 1u;
+  }
+  int st = PS_OK;
+  if (d_n != nullptr || n_host > 0)
+    st = digit_passes(n_host, d_n, max_pairs, 32, end_bit, w, keys_in, vals_in, keys_out,
+                      vals_out, stream);
+  if (st != PS_OK) return st;
+  st = tile_ranges_scan_launch(n_host, d_n, max_pairs, segs, keys_out, ranges, stream, mask,
+                               w.big_count);
+  if (st != PS_OK) return st;
+  if (d_n == nullptr && n_host == 0) return PS_OK;
+  const int big_cap = (int)big_list_cap((size_t)(max_pairs > 0 ? max_pairs : 1));
+  int grid = (segs + kSegWaves - 1) / kSegWaves;
+  if (grid > 16384) grid = 16384;
+  hipLaunchKernelGGL(ps_seg_small, dim3(grid), dim3(kSegWaves * kWave), 0, stream, keys_out,
+                     vals_out, reinterpret_cast<const int2*>(ranges), segs, w.big_list,
+                     w.big_count, big_cap);
+  PS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(ps_seg_big, dim3(512), dim3(kSortThreads), 0, stream, keys_out, vals_out,
+                     w.alt_keys, w.alt_vals, reinterpret_cast<const int2*>(ranges), w.big_list,
+                     w.big_count, big_cap);
+  PS_LAUNCH_CHECK();
   return PS_OK;
 }
 

@@ -3,10 +3,9 @@ memory-based admission and continuous batching over shared launches.
 
 The reference renders one frame per Python call and blocks on the pair
 count between preprocess and sort (pybind.cpp:1122-1125); it has no serving
-loop.  Here every batch of frames is one call of ``ops.forward_batch``
-(preprocess, sort, range build and persistent render shared by all frames
-of the batch, no host round trip inside), and the loop keeps several
-batches in flight on separate HIP streams:
+loop.  Here every frame is one call of ``ops.forward_frame`` whose stages
+pass the pair count on the device (no host round trip inside a frame), and
+the loop keeps several groups of frames in flight on separate HIP streams:
 
 * requests arrive on the host clock (Poisson process, orbit poses,
   resolutions drawn uniformly);
@@ -14,12 +13,12 @@ batches in flight on separate HIP streams:
   feature records and image need fit the key-memory budget next to the
   batches already on the GPU;
 * whenever a stream slot is free, all admitted-and-waiting requests up to
-  a frame cap and an estimated-GPU-time cap form the next batch (first in,
-  first out), so requests that arrive while others render join the next
-  launch instead of waiting for a fixed batch to fill;
-* a batch whose pairs exceed the estimate is detected from its device
-  counts (read back asynchronously) and its requests are re-queued at the
-  front with the measured count.
+  a frame cap and an estimated-GPU-time cap form the next group (first in,
+  first out), so requests that arrive while others render start as soon
+  as a slot frees instead of waiting for a fixed batch to fill;
+* a frame whose pairs exceed its estimate is detected from its device
+  count (read back asynchronously) and re-queued at the front with the
+  measured count.
 
 Latency is arrival -> image ready: the completion of a batch is the GPU
 timestamp of an event recorded after its render, mapped onto the host
@@ -207,17 +206,22 @@ class InFlight:
     bid: int
     reqs: list
     nbytes: int
-    capacity: int
+    capacities: list
     slot: int
-    done: object = None            # torch.cuda.Event
-    host_counts: object = None     # pinned int32 [nf + 1]: per-frame counts, then the total
+    done: object = None            # torch.cuda.Event after the group's last frame
+    host_counts: object = None     # pinned int32 [frames]: pairs each frame needed
     buffers: list = field(default_factory=list)
     outputs: list = field(default_factory=list)
 
 
 class GpuExecutor:
-    """Runs batches on the GPU with ``ops.forward_batch``; one HIP stream per
-    slot, all at the default priority."""
+    """Runs a group of admitted requests on one stream slot: each frame is
+    one ``ops.forward_frame`` call (preprocess, sort with tile ranges,
+    persistent render; the pair count stays on the device), launched back to
+    back on the slot's HIP stream, so frames of different slots overlap on
+    the device.  All streams keep the default priority.  A frame's pair
+    count is copied to pinned host memory without waiting; a frame whose
+    pairs exceeded its key buffers is reported for a retry."""
 
     def __init__(self, scenes_dev, device="cuda:0", slots: int = 2, feat_prec: str = "fp16",
                  radius: float = ORBIT_RADIUS):
@@ -236,6 +240,7 @@ class GpuExecutor:
         self.start = torch.cuda.Event(enable_timing=True)
         self.t0 = None
         self._bid = 0
+        self._empty = torch.empty(0, dtype=torch.float32, device=self.dev)
 
     def begin(self):
         torch = self.torch
@@ -253,63 +258,58 @@ class GpuExecutor:
                 return i
         return None
 
-    def launch(self, reqs, nbytes, capacity, slot) -> InFlight:
+    def _frame(self, r, cap):
+        """Allocates one frame's buffers on the current stream and enqueues it."""
+        torch = self.torch
+        dev = self.dev
+        sc = self.scenes[r.scene]
+        cap = max(int(cap), 1)
+        ku = torch.empty(cap, dtype=torch.int64, device=dev)
+        vu = torch.empty(cap, dtype=torch.int32, device=dev)
+        ks = torch.empty(cap, dtype=torch.int64, device=dev)
+        vs = torch.empty(cap, dtype=torch.int32, device=dev)
+        space = torch.empty(self.ops.get_sort_buffer_size(cap), dtype=torch.int8, device=dev)
+        curr = torch.empty(1, dtype=torch.int32, device=dev)
+        ranges = torch.empty((r.tiles, 2), dtype=torch.int32, device=dev)
+        ws = torch.empty(self.ops.forward_workspace_size(sc["P"]), dtype=torch.uint8, device=dev)
+        ws[:256].zero_()  # persistent-queue words start at zero
+        rec = torch.empty(24 * max(sc["P"], 1), dtype=torch.uint8, device=dev)
+        out = torch.empty((r.height, r.width, 3), dtype=torch.uint8, device=dev)
+        cam = orbit_camera(r.width, r.height, r.azimuth, self.radius)
+        e = self._empty
+        self.ops.forward_frame(sc["pos"], sc["shs"], sc["opac"], sc["cov"], r.width, r.height,
+                               cam.position, cam.rotation, cam.focal_x, cam.focal_y, cam.zFar,
+                               cam.zNear, 1, self.feat_prec, curr, space, ku, vu, ks, vs, e, e, e,
+                               rec, ranges, torch.tensor(r.bg, dtype=torch.float32), out, ws,
+                               False)
+        return curr, out, [ku, vu, ks, vs, space, curr, ranges, ws, rec]
+
+    def launch(self, reqs, nbytes, capacities, slot) -> InFlight:
         torch = self.torch
         s = self.slots[slot]
-        fl = InFlight(self._bid, reqs, nbytes, capacity, slot)
+        fl = InFlight(self._bid, reqs, nbytes, list(capacities), slot)
         self._bid += 1
+        host = torch.empty(len(reqs), dtype=torch.int32, pin_memory=True)
         with torch.cuda.stream(s):
-            dev = self.dev
-            cap = max(int(capacity), 1)
-            ku = torch.empty(cap, dtype=torch.int64, device=dev)
-            vu = torch.empty(cap, dtype=torch.int32, device=dev)
-            ks = torch.empty(cap, dtype=torch.int64, device=dev)
-            vs = torch.empty(cap, dtype=torch.int32, device=dev)
-            space = torch.empty(self.ops.get_sort_buffer_size(cap), dtype=torch.int8, device=dev)
-            curr = torch.empty(1, dtype=torch.int32, device=dev)
-            tiles = sum(r.tiles for r in reqs)
-            ranges = torch.empty((tiles, 2), dtype=torch.int32, device=dev)
-            P = [self.scenes[r.scene]["P"] for r in reqs]
-            ws_bytes = self.ops.forward_batch_workspace_size(P)
-            ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-            ws[:256].zero_()  # persistent-queue words start at zero
-            frames, outs, recs = [], [], []
-            empty = torch.empty(0, dtype=torch.float32, device=dev)
-            for r in reqs:
-                sc = self.scenes[r.scene]
-                cam = orbit_camera(r.width, r.height, r.azimuth, self.radius)
-                out = torch.empty((r.height, r.width, 3), dtype=torch.uint8, device=dev)
-                rec = torch.empty(24 * max(sc["P"], 1), dtype=torch.uint8, device=dev)
-                bg = torch.tensor(r.bg, dtype=torch.float32)
-                frames.append((sc["pos"], sc["shs"], sc["opac"], sc["cov"], r.width, r.height,
-                               cam.position, cam.rotation, cam.focal_x, cam.focal_y, cam.zFar,
-                               cam.zNear, bg, out, rec, empty, empty, empty))
-                outs.append(out)
-                recs.append(rec)
-            self.ops.forward_batch(frames, self.feat_prec, curr, space, ku, vu, ks, vs, ranges,
-                                   ws, False)
-            off = self.ops.forward_batch_counts_offset(P)
-            host = torch.empty(len(reqs) + 1, dtype=torch.int32, pin_memory=True)
-            host[:len(reqs)].copy_(ws[off:off + 4 * len(reqs)].view(torch.int32),
-                                   non_blocking=True)
-            host[len(reqs):].copy_(curr, non_blocking=True)
+            for i, (r, cap) in enumerate(zip(reqs, capacities)):
+                curr, out, bufs = self._frame(r, cap)
+                host[i:i + 1].copy_(curr, non_blocking=True)
+                fl.outputs.append(out)
+                fl.buffers.append(bufs)
             fl.done = torch.cuda.Event(enable_timing=True)
             fl.done.record(s)
         fl.host_counts = host
-        fl.buffers = [ku, vu, ks, vs, space, curr, ranges, ws, recs]
-        fl.outputs = outs
         self.busy[slot] = fl
         return fl
 
     def poll(self):
-        """Completed batches (their slots are freed)."""
+        """Completed groups (their slots are freed): (group, done_ms, counts)."""
         out = []
         for i, fl in enumerate(self.busy):
             if fl is not None and fl.done.query():
                 done_ms = self.start.elapsed_time(fl.done)
-                counts = fl.host_counts.tolist()
                 self.busy[i] = None
-                out.append((fl, done_ms, counts[:-1], counts[-1]))
+                out.append((fl, done_ms, fl.host_counts.tolist()))
         return out
 
     def idle(self) -> bool:
@@ -338,25 +338,26 @@ class ServingLoop:
             while pending and pending[0].arrival_s <= now:
                 self.sched.arrive(pending.popleft())
             progressed = False
-            for fl, done_ms, counts, total in self.ex.poll():
+            for fl, done_ms, counts in self.ex.poll():
                 progressed = True
                 self.sched.release(fl.nbytes)
-                if total > fl.capacity:
-                    # not rendered: learn the real counts and retry first
-                    self.overflows += 1
-                    for r, c in zip(fl.reqs, counts):
-                        self.sched.est.observe(r.key, pairs=c)
-                    self.sched.requeue_front(fl.reqs)
-                    continue
-                for r, c, img in zip(fl.reqs, counts, fl.outputs):
+                retry = []
+                for r, c, cap, img in zip(fl.reqs, counts, fl.capacities, fl.outputs):
+                    self.sched.est.observe(r.key, pairs=c)
+                    if c > cap:
+                        retry.append(r)  # not rendered: its pairs did not fit
+                        continue
                     r.pairs = c
                     r.done_ms = done_ms
                     r.batch = fl.bid
-                    self.sched.est.observe(r.key, pairs=c)
                     if self.keep_outputs:
                         self.outputs[r.rid] = img
                     done.append(r)
-                self.batches.append((fl.bid, [r.rid for r in fl.reqs], done_ms))
+                if retry:
+                    self.overflows += 1
+                    self.sched.requeue_front(retry)
+                self.batches.append((fl.bid, [r.rid for r in fl.reqs if r not in retry],
+                                     done_ms))
             while True:
                 slot = self.ex.free_slot()
                 if slot is None:
@@ -364,8 +365,8 @@ class ServingLoop:
                 batch, nbytes = self.sched.form_batch()
                 if not batch:
                     break
-                cap = sum(self.sched.est.capacity(r.key) for r in batch)
-                self.ex.launch(batch, nbytes, cap, slot)
+                caps = [self.sched.est.capacity(r.key) for r in batch]
+                self.ex.launch(batch, nbytes, caps, slot)
                 progressed = True
             if not progressed:
                 if not pending and not self.sched.waiting and self.ex.idle():
@@ -423,22 +424,22 @@ def calibrate(executor: GpuExecutor, est: Estimates, n_scenes: int, resolutions=
     for s in range(n_scenes):
         for (w, h) in resolutions:
             for a in azimuths:
+                r = Request(-1, s, w, h, a, 0.0)
                 cap = 1 << 20
                 for _ in range(4):
-                    r = Request(-1, s, w, h, a, 0.0)
-                    fl = executor.launch([r], 0, cap, 0)
+                    executor.launch([r], 0, [cap], 0)
                     torch.cuda.synchronize(executor.dev)
-                    (fl2, _, counts, total), = executor.poll()
-                    if total <= cap:
+                    (_, _, counts), = executor.poll()
+                    if counts[0] <= cap:
                         break
-                    cap = int(total * 1.05) + 4096
-                est.observe(r.key, pairs=total)
+                    cap = int(counts[0] * 1.05) + 4096
+                est.observe(r.key, pairs=counts[0])
                 t = []
                 for _ in range(repeats):
                     ev0 = torch.cuda.Event(enable_timing=True)
                     ev0.record(executor.slots[0])
-                    executor.launch([r], 0, cap, 0)
+                    executor.launch([r], 0, [cap], 0)
                     torch.cuda.synchronize(executor.dev)
-                    (fl2, _, _, _), = executor.poll()
-                    t.append(ev0.elapsed_time(fl2.done))
+                    (fl, _, _), = executor.poll()
+                    t.append(ev0.elapsed_time(fl.done))
                 est.observe(r.key, ms=float(np.median(t)))

@@ -244,3 +244,25 @@ def test_c2_full_size_bit_exact(gpu):
     say("c2 vs fp32", m32, "vs fp64", m64)
     assert m32["max_abs_diff"] <= 1, m32
     assert m64["max_abs_diff"] <= 1 and m64["psnr_db"] >= 50.0, m64
+
+
+def test_forward_workspace_reused_across_scene_sizes(gpu):
+    """One workspace for a large then a small scene: the persistent queue
+    words must not pick up the larger scene's preprocess data."""
+    big_s = scenes.make_config_scene("c1", seed=21, P=30_000)
+    small_s = scenes.make_config_scene("c1", seed=23, P=12_000)
+    big = Gpu(big_s, scenes.config_camera("c1"), sh_type="fp16", feat_prec="fp16",
+              capacity=400_000)
+    small = Gpu(small_s, scenes.make_camera(333, 250, 10.0, distance=9.0), sh_type="fp16",
+                feat_prec="fp16", capacity=400_000)
+    st, _, fresh = small.forward(BG, capi.V_LITE)
+    capi.check(st)
+    shared = torch.zeros(max(big.ws_bytes, small.ws_bytes), device=big.dev, dtype=torch.uint8)
+    for _ in range(2):
+        say("ws reuse big")
+        st, _, _ = big.forward(BG, capi.V_LITE, ws=shared)
+        capi.check(st)
+        say("ws reuse small")
+        st, _, img = small.forward(BG, capi.V_LITE, ws=shared)
+        capi.check(st)
+        np.testing.assert_array_equal(img, fresh)

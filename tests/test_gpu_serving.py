@@ -1,5 +1,5 @@
 """Serving loop on the GPU: every request rendered in the mixed stream
-(batched with others, overlapping on two streams) is byte-identical to the
+(grouped with others, overlapping on two streams) is byte-identical to the
 same request rendered alone through the single-frame entry point."""
 import math
 

@@ -106,3 +106,70 @@ def test_sort_large_device_count_properties(gpu):
     # permutation of the input
     assert bool(torch.equal(ku[vs.long()], ks))
     assert int(torch.bincount(vs.long(), minlength=n).max()) == 1
+
+
+def _keys_with_counts(counts, rng, tiles):
+    """Keys whose tile t appears counts[t] times, shuffled, with few depth
+    levels (ties everywhere)."""
+    t = np.repeat(np.arange(len(counts), dtype=np.uint64), counts)
+    rng.shuffle(t)
+    d = rng.integers(0, 5, len(t)).astype(np.uint64) + np.uint64(0x3e4ccccd)
+    return (t << np.uint64(32)) | d
+
+
+@pytest.mark.parametrize("mode", ["segmented", "lsd"])
+def test_sort_segment_sizes_around_the_lds_limit(gpu, mode, monkeypatch):
+    # segments of 0, 1, 2, 63..65, 895..897, 1023..1025 and 5000 pairs: wave path,
+    # its limit and the workgroup path for long segments
+    if mode == "lsd":
+        monkeypatch.setenv("POLYSPLAT_SORT", "lsd")
+    counts = [0, 1, 2, 63, 64, 65, 895, 896, 897, 1024, 1025, 5000, 0, 7]
+    rng = np.random.default_rng(99)
+    keys = _keys_with_counts(counts, rng, len(counts))
+    n = len(keys)
+    vals = rng.permutation(n).astype(np.uint32)
+    say("sort segments", mode, n)
+    k, v = _run_sort(keys, vals, 64, 64, cap=n + 3)  # 16 tiles
+    rk, rv = O.sort_pairs(keys, vals)
+    np.testing.assert_array_equal(k[:n], rk)
+    np.testing.assert_array_equal(v[:n], rv)
+
+
+def test_sort_ignores_key_bits_above_end_bit(gpu):
+    # The sort orders by bits [0, 32 + bit_length(tiles)) only, like the
+    # reference's radix sort; higher bits ride along.  Segments whose keys
+    # differ only above the sorted bits take the workgroup path.
+    rng = np.random.default_rng(5)
+    n = 20_000
+    tiles = 40  # end bit 38
+    t = rng.integers(0, 64, n).astype(np.uint64)
+    junk = rng.integers(0, 4, n).astype(np.uint64) << np.uint64(6)
+    d = rng.integers(0, 9, n).astype(np.uint64)
+    keys = ((t | junk) << np.uint64(32)) | d
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = _run_sort(keys, vals, 160, 64)
+    assert 32 + int(tiles).bit_length() == 38
+    masked = keys & np.uint64((1 << 38) - 1)
+    order = np.argsort(masked, kind="stable")
+    np.testing.assert_array_equal(v[:n], vals[order])
+    np.testing.assert_array_equal(k[:n], keys[order])
+
+
+def test_sort_lsd_and_segmented_agree_on_preprocess_keys(gpu, monkeypatch):
+    from polysplat_amd import scenes
+    from tests.gpu_helpers import Gpu
+
+    # 40% large splats close to the camera: 2.3M pairs, tiles up to ~1450
+    scene = scenes.make_scene(40_000, 8.0, [(0.6, -4.0, 0.5), (0.4, -1.0, 0.3)], seed=31)
+    spec = scenes.make_camera(900, 500, 8.0, distance=6.0)
+    g = Gpu(scene, spec, capacity=2_400_000)
+    M = g.preprocess()
+    say("sort agree", M)
+    g.sort()
+    seg_k, seg_v = g.ks[:M].clone(), g.vs[:M].clone()
+    monkeypatch.setenv("POLYSPLAT_SORT", "lsd")
+    g.ks.fill_(0)
+    g.sort()
+    assert torch.equal(g.ks[:M], seg_k) and torch.equal(g.vs[:M], seg_v)
+    kt = (seg_k >> 32).cpu().numpy()
+    assert np.bincount(kt).max() > 1024  # a long segment was exercised

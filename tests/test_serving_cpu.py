@@ -34,11 +34,11 @@ class SimExecutor:
                 return i
         return None
 
-    def launch(self, reqs, nbytes, capacity, slot):
+    def launch(self, reqs, nbytes, capacities, slot):
         start = max(self.gpu_free_ms, 1e3 * self.t)
         end = start + sum(self.cost_ms[r.key] for r in reqs)
         self.gpu_free_ms = end
-        fl = SV.InFlight(len(self.launched), reqs, nbytes, capacity, slot)
+        fl = SV.InFlight(len(self.launched), reqs, nbytes, capacities, slot)
         fl.done = end
         fl.outputs = [f"img{r.rid}" for r in reqs]
         self.busy[slot] = fl
@@ -51,7 +51,7 @@ class SimExecutor:
             if fl is not None and fl.done <= 1e3 * self.t:
                 counts = [self.pairs[r.key] for r in fl.reqs]
                 self.busy[i] = None
-                out.append((fl, fl.done, counts, sum(counts)))
+                out.append((fl, fl.done, counts))
         return out
 
     def idle(self):
@@ -148,7 +148,7 @@ def test_loop_completes_everything_in_order_with_latencies():
     # continuous batching: later batches pick up requests that arrived while
     # earlier ones ran, so some batches hold several frames
     assert max(len(b[1]) for b in loop.batches) > 1
-    # arrival order is kept across batches
+    # arrival order is kept across groups
     order = [rid for b in sorted(loop.batches, key=lambda b: b[0]) for rid in b[1]]
     assert order == sorted(order)
     s = SV.summarize(done, sched.rejected, wall_s=ex.t, batches=loop.batches)
