@@ -238,7 +238,16 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
 }
 
 namespace ps {
-static bool batch_enabled() { return false; }
+// The batched path is compiled in but refused at run time unless the build
+// defines PS_ENABLE_BATCH (the host emulation of the kernels does, to test
+// it on the CPU).
+static bool batch_enabled() {
+#ifdef PS_ENABLE_BATCH
+  return true;
+#else
+  return false;
+#endif
+}
 }  // namespace ps
 
 extern "C" size_t ps_forward_batch_workspace_size(int num_frames, const int* P) {

@@ -49,8 +49,13 @@ __device__ __forceinline__ SortPlan sort_plan(const int* d_n, int n_host, int ma
   return p;
 }
 
-__device__ __forceinline__ int digit_of(uint64_t k, int shift) {
-  return (int)((k >> shift) & 0xffu);
+// A digit is `width` (1..8) key bits from `shift`, packed as
+// shift | width << 8.  The last digit pass of a sort over [0, end_bit) takes
+// only the bits below end_bit, so higher key bits never affect the order.
+__host__ __device__ constexpr int digit_spec(int shift, int width) { return shift | (width << 8); }
+
+__device__ __forceinline__ int digit_of(uint64_t k, int spec) {
+  return (int)((k >> (spec & 63)) & ((1u << (spec >> 8)) - 1u));
 }
 
 // Tile item layout: wave w owns items [w*960, w*960 + 960) of the tile; item
@@ -65,7 +70,7 @@ __device__ __forceinline__ int tile_item(int wave, int r, int lane) {
 __global__ __launch_bounds__(kSortThreads) void ps_sort_upsweep(const uint64_t* __restrict__ keys,
                                                                 const int* __restrict__ d_n,
                                                                 int n_host, int max_pairs,
-                                                                int shift,
+                                                                int spec,
                                                                 uint32_t* __restrict__ counts) {
   __shared__ uint32_t hist[4][kRadix];
   const SortPlan pl = sort_plan(d_n, n_host, max_pairs);
@@ -85,7 +90,7 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_upsweep(const uint64_t* 
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {  // all loads issue before any use
       const int i = tile_item(wave, r, lane);
-      d[r] = (i < tile_n) ? digit_of(keys[tile0 + i], shift) : -1;
+      d[r] = (i < tile_n) ? digit_of(keys[tile0 + i], spec) : -1;
     }
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r)
@@ -149,7 +154,7 @@ __device__ __forceinline__ void load_tile(const uint64_t* __restrict__ keys,
 __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
     const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint64_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, const int* __restrict__ d_n,
-    int n_host, int max_pairs, int shift, const uint32_t* __restrict__ offsets,
+    int n_host, int max_pairs, int spec, const uint32_t* __restrict__ offsets,
     const uint32_t* __restrict__ totals) {
   __shared__ uint64_t skey[kSortTile];
   __shared__ uint32_t sval[kSortTile];
@@ -188,7 +193,7 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
     uint32_t pos[kSortItems];
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
-      const int d = (tile_item(wave, r, lane) < tile_n) ? digit_of(k[r], shift) : -1;
+      const int d = (tile_item(wave, r, lane) < tile_n) ? digit_of(k[r], spec) : -1;
       dg[r] = d;
       uint64_t peers = ballot(d >= 0);
 #pragma unroll
@@ -233,7 +238,7 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
     __syncthreads();
     for (int i = tid; i < tile_n; i += kSortThreads) {
       const uint64_t key = skey[i];
-      const int d = digit_of(key, shift);
+      const int d = digit_of(key, spec);
       const uint32_t dst = gbase[d] + ((uint32_t)i - wcnt[0][d]);
       if (dst < (uint32_t)pl.n) {
         keys_out[dst] = key;
@@ -401,7 +406,7 @@ __global__ __launch_bounds__(kSortThreads) void ps_seg_big(
     const int2 r = ranges[big_list[b]];
     const int n = r.y - r.x;
     for (int p = 0; p < 4; ++p) {
-      const int shift = 8 * p;
+      const int spec = digit_spec(8 * p, 8);
       const uint64_t* sk = ((p & 1) ? tkeys : keys) + r.x;
       const uint32_t* sv = ((p & 1) ? tvals : vals) + r.x;
       uint64_t* dk = ((p & 1) ? keys : tkeys) + r.x;
@@ -413,7 +418,7 @@ __global__ __launch_bounds__(kSortThreads) void ps_seg_big(
       wcnt[3][tid] = 0;
       __syncthreads();
       for (int i = tid; i < n; i += kSortThreads)
-        atomicAdd(&wcnt[wave][digit_of(sk[i], shift)], 1u);
+        atomicAdd(&wcnt[wave][digit_of(sk[i], spec)], 1u);
       __syncthreads();
       int total;
       const int ex = block256_excl_scan(
@@ -434,7 +439,7 @@ __global__ __launch_bounds__(kSortThreads) void ps_seg_big(
           const bool ok = i < n;
           k[q] = ok ? sk[i] : 0ull;
           v[q] = ok ? sv[i] : 0u;
-          dg[q] = ok ? digit_of(k[q], shift) : -1;
+          dg[q] = ok ? digit_of(k[q], spec) : -1;
         }
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
@@ -541,15 +546,16 @@ int digit_passes(int n_host, const int* d_n, int max_pairs, int bit0, int bit1, 
     const bool to_out = ((passes - 1 - p) % 2) == 0;
     uint64_t* kout = to_out ? keys_out : w.alt_keys;
     uint32_t* vout = to_out ? vals_out : w.alt_vals;
-    const int shift = bit0 + 8 * p;
+    const int lo = bit0 + 8 * p;
+    const int spec = digit_spec(lo, (bit1 - lo < 8) ? bit1 - lo : 8);
     hipLaunchKernelGGL(ps_sort_upsweep, dim3(kSortBlocks), dim3(kSortThreads), 0, stream, kin,
-                       d_n, n_host, max_pairs, shift, w.counts);
+                       d_n, n_host, max_pairs, spec, w.counts);
     PS_LAUNCH_CHECK();
     hipLaunchKernelGGL(ps_sort_scan, dim3(kRadix), dim3(kSortThreads), 0, stream, w.counts,
                        w.offsets, w.totals);
     PS_LAUNCH_CHECK();
     hipLaunchKernelGGL(ps_sort_downsweep, dim3(kSortBlocks), dim3(kSortThreads), 0, stream, kin,
-                       vin, kout, vout, d_n, n_host, max_pairs, shift, w.offsets, w.totals);
+                       vin, kout, vout, d_n, n_host, max_pairs, spec, w.offsets, w.totals);
     PS_LAUNCH_CHECK();
     kin = kout;
     vin = vout;

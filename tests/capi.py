@@ -56,43 +56,48 @@ _lib = None
 
 
 def lib():
+    """The GPU library with every exported function's signature applied."""
     global _lib
     if _lib is None:
         if not LIB.exists():
             raise RuntimeError(f"{LIB} is not built; run python -m polysplat_amd._build")
-        L = C.CDLL(str(LIB))
-        P, I, S = C.c_void_p, C.c_int, C.c_size_t
-        sig = {
-            "ps_status_string": ([I], C.c_char_p),
-            "ps_last_error": ([], C.c_char_p),
-            "ps_version": ([], C.c_char_p),
-            "ps_camera_matrices": ([P, P, P, P], I),
-            "ps_preprocess_workspace_size": ([I], S),
-            "ps_preprocess": ([I, P, P, I, P, P, P, P, P, P, P, I, P, P, I, P, P, S, P], I),
-            "ps_preprocess_count_ptr": ([P], P),
-            "ps_preprocess_status_ptr": ([P], P),
-            "ps_get_sort_buffer_size": ([I], S),
-            "ps_sort_gaussian": ([I, P, I, I, I, I, I, P, S, P, P, P, P, P], I),
-            "ps_fetch_num_rendered": ([P, P, P], I),
-            "ps_precompute_tile_ranges": ([I, P, I, I, I, I, P, P, P], I),
-            "ps_precompute_tile_ranges_scan": ([I, P, I, I, I, I, I, P, P, P], I),
-            "ps_gather_features": ([I, P, I, P, P, P, P, P, P, P, P], I),
-            "ps_gather_records": ([I, P, I, P, P, P, P], I),
-            "ps_render_16x16": ([I, P, I, I, P, P, P, P, P, P, P, P], I),
-            "ps_render_16x16_persistent": ([I, P, I, I, I, P, P, P, P, I, P, P, P, I, P, P, P, P],
-                                           I),
-            "ps_forward_workspace_size": ([I], S),
-            "ps_forward": ([I, P, P, I, P, P, P, I, I, P, P, P, I, P, P], I),
-            "ps_forward_batch_workspace_size": ([I, P], S),
-            "ps_forward_batch": ([I, P, I, I, P, I, P, P], I),
-            "ps_forward_batch_counts_ptr": ([P, I, P], P),
-        }
-        for name, (args, res) in sig.items():
-            f = getattr(L, name)
-            f.argtypes = args
-            f.restype = res
-        _lib = L
+        _lib = bind(C.CDLL(str(LIB)))
     return _lib
+
+
+def bind(L):
+    """Applies the C ABI signatures to a loaded library (GPU or host emulation)."""
+    P, I, S = C.c_void_p, C.c_int, C.c_size_t
+    sig = {
+        "ps_status_string": ([I], C.c_char_p),
+        "ps_last_error": ([], C.c_char_p),
+        "ps_version": ([], C.c_char_p),
+        "ps_camera_matrices": ([P, P, P, P], I),
+        "ps_preprocess_workspace_size": ([I], S),
+        "ps_preprocess": ([I, P, P, I, P, P, P, P, P, P, P, I, P, P, I, P, P, S, P], I),
+        "ps_preprocess_count_ptr": ([P], P),
+        "ps_preprocess_status_ptr": ([P], P),
+        "ps_get_sort_buffer_size": ([I], S),
+        "ps_sort_gaussian": ([I, P, I, I, I, I, I, P, S, P, P, P, P, P], I),
+        "ps_fetch_num_rendered": ([P, P, P], I),
+        "ps_precompute_tile_ranges": ([I, P, I, I, I, I, P, P, P], I),
+        "ps_precompute_tile_ranges_scan": ([I, P, I, I, I, I, I, P, P, P], I),
+        "ps_gather_features": ([I, P, I, P, P, P, P, P, P, P, P], I),
+        "ps_gather_records": ([I, P, I, P, P, P, P], I),
+        "ps_render_16x16": ([I, P, I, I, P, P, P, P, P, P, P, P], I),
+        "ps_render_16x16_persistent": ([I, P, I, I, I, P, P, P, P, I, P, P, P, I, P, P, P, P],
+                                       I),
+        "ps_forward_workspace_size": ([I], S),
+        "ps_forward": ([I, P, P, I, P, P, P, I, I, P, P, P, I, P, P], I),
+        "ps_forward_batch_workspace_size": ([I, P], S),
+        "ps_forward_batch": ([I, P, I, I, P, I, P, P], I),
+        "ps_forward_batch_counts_ptr": ([P, I, P], P),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    return L
 
 
 def check(status: int) -> None:

@@ -1,0 +1,241 @@
+// Host emulation of the HIP subset the rasterizer kernels use, for running
+// the unmodified kernel sources on the CPU (test infrastructure only).
+//
+// Every work-item of a workgroup is a fiber on the calling host thread;
+// workgroups run one after another.  Fibers switch only at barriers, so
+// cross-lane operations (ballot, readlane, ds_bpermute, shfl_xor) exchange
+// values through a per-wave slot array between two barrier phases of that
+// wave; __builtin_amdgcn_wave_barrier is a barrier of the wave and
+// __syncthreads one of the workgroup, so the ordering the kernels rely on (a
+// wave's LDS operations in program order) holds wherever the sources mark
+// it.  A work-item that returns drops out of both barriers and reads as an
+// inactive lane.
+//
+// __shared__ becomes function-scope static storage: only one workgroup is
+// alive at a time.  Device pointers are host pointers, so AddressSanitizer
+// sees every out-of-bounds access of a kernel.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <atomic>
+#include <functional>
+#include <tuple>
+#include <type_traits>
+#include <utility>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+
+struct dim3 {
+  unsigned x, y, z;
+  constexpr dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct emu_uint3 {
+  unsigned x, y, z;
+};
+
+struct alignas(8) float2 { float x, y; };
+struct float3 { float x, y, z; };
+struct alignas(16) float4 { float x, y, z, w; };
+struct alignas(8) int2 { int x, y; };
+struct alignas(8) uint2 { unsigned x, y; };
+struct alignas(16) uint4 { unsigned x, y, z, w; };
+inline float2 make_float2(float a, float b) { return {a, b}; }
+inline float3 make_float3(float a, float b, float c) { return {a, b, c}; }
+inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
+inline int2 make_int2(int a, int b) { return {a, b}; }
+
+namespace emu {
+
+constexpr int kMaxThreads = 1024;
+
+struct Ctx {
+  emu_uint3 tid, bid;
+  dim3 bdim, gdim;
+  int lane, wave;
+};
+inline thread_local Ctx ctx;
+
+// A barrier among the fibers of a workgroup (or of one wave).
+struct Bar {
+  int expected = 0;
+  int arrived = 0;
+  uint64_t gen = 0;
+};
+void bar_wait(Bar* b);  // blocks the calling fiber until every member arrived
+
+struct BlockSync {
+  Bar block;
+  Bar wave[kMaxThreads / 64];
+  uint64_t slot[kMaxThreads / 64][64];
+  bool alive[kMaxThreads / 64][64];
+};
+BlockSync* sync_state();
+
+// Runs fn on every work-item of the grid, workgroup by workgroup.
+void run_grid(dim3 grid, dim3 block, const std::function<void()>& fn);
+
+inline uint64_t exchange(uint64_t v, int src_lane) {
+  BlockSync& S = *sync_state();
+  const int w = ctx.wave;
+  S.slot[w][ctx.lane] = v;
+  bar_wait(&S.wave[w]);
+  const uint64_t r = (src_lane >= 0 && src_lane < 64 && S.alive[w][src_lane]) ? S.slot[w][src_lane]
+                                                                               : 0;
+  bar_wait(&S.wave[w]);
+  return r;
+}
+
+}  // namespace emu
+
+#define threadIdx (::emu::ctx.tid)
+#define blockIdx (::emu::ctx.bid)
+#define blockDim (::emu::ctx.bdim)
+#define gridDim (::emu::ctx.gdim)
+
+inline void __syncthreads() { emu::bar_wait(&emu::sync_state()->block); }
+inline void __threadfence_block() { std::atomic_thread_fence(std::memory_order_seq_cst); }
+inline void __builtin_amdgcn_wave_barrier() {
+  emu::bar_wait(&emu::sync_state()->wave[emu::ctx.wave]);
+}
+
+inline unsigned long long __ballot(int p) {
+  emu::BlockSync& S = *emu::sync_state();
+  const int w = emu::ctx.wave;
+  S.slot[w][emu::ctx.lane] = p ? 1u : 0u;
+  emu::bar_wait(&S.wave[w]);
+  unsigned long long m = 0;
+  for (int i = 0; i < 64; ++i)
+    if (S.alive[w][i] && S.slot[w][i]) m |= 1ull << i;
+  emu::bar_wait(&S.wave[w]);
+  return m;
+}
+inline int __builtin_amdgcn_readlane(int v, int l) {
+  return (int)(uint32_t)emu::exchange((uint32_t)v, l);
+}
+inline int __builtin_amdgcn_ds_bpermute(int addr, int v) {
+  return (int)(uint32_t)emu::exchange((uint32_t)v, (addr >> 2) & 63);
+}
+inline int __shfl_xor(int v, int m) {
+  return (int)(uint32_t)emu::exchange((uint32_t)v, emu::ctx.lane ^ m);
+}
+inline unsigned __builtin_amdgcn_mbcnt_lo(unsigned mask, unsigned acc) {
+  const int l = emu::ctx.lane < 32 ? emu::ctx.lane : 32;
+  const unsigned below = l >= 32 ? 0xffffffffu : ((1u << l) - 1u);
+  return acc + (unsigned)__builtin_popcount(mask & below);
+}
+inline unsigned __builtin_amdgcn_mbcnt_hi(unsigned mask, unsigned acc) {
+  const int l = emu::ctx.lane - 32;
+  const unsigned below = l <= 0 ? 0u : ((1u << l) - 1u);
+  return acc + (unsigned)__builtin_popcount(mask & below);
+}
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
+inline unsigned min(unsigned a, unsigned b) { return a < b ? a : b; }
+inline unsigned max(unsigned a, unsigned b) { return a > b ? a : b; }
+inline int __popcll(unsigned long long m) { return __builtin_popcountll(m); }
+inline float __builtin_amdgcn_exp2f(float x) { return exp2f(x); }
+
+inline float __uint_as_float(unsigned u) {
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+inline unsigned __float_as_uint(float f) {
+  unsigned u;
+  memcpy(&u, &f, 4);
+  return u;
+}
+inline float __int_as_float(int u) {
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+inline int __float_as_int(float f) {
+  int u;
+  memcpy(&u, &f, 4);
+  return u;
+}
+
+template <typename T>
+inline T atomicAdd(T* p, T v) {
+  return std::atomic_ref<T>(*p).fetch_add(v);
+}
+template <typename T>
+inline T atomicExch(T* p, T v) {
+  return std::atomic_ref<T>(*p).exchange(v);
+}
+
+// ---------------------------------------------------------------- runtime
+typedef struct ihipStream_t* hipStream_t;
+typedef struct ihipEvent_t* hipEvent_t;
+typedef enum { hipSuccess = 0, hipErrorInvalidValue = 1 } hipError_t;
+typedef enum {
+  hipMemcpyHostToHost = 0,
+  hipMemcpyHostToDevice = 1,
+  hipMemcpyDeviceToHost = 2,
+  hipMemcpyDeviceToDevice = 3,
+  hipMemcpyDefault = 4
+} hipMemcpyKind;
+enum { hipHostMallocDefault = 0, hipEventDisableTiming = 2 };
+typedef enum { hipDeviceAttributeMultiprocessorCount = 0 } hipDeviceAttribute_t;
+
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
+  memcpy(d, s, n);
+  return hipSuccess;
+}
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
+  memcpy(d, s, n);
+  return hipSuccess;
+}
+inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) {
+  *p = malloc(n);
+  return *p ? hipSuccess : hipErrorInvalidValue;
+}
+inline hipError_t hipHostFree(void* p) {
+  free(p);
+  return hipSuccess;
+}
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) {
+  *e = reinterpret_cast<hipEvent_t>(new char[1]);
+  return hipSuccess;
+}
+inline hipError_t hipEventDestroy(hipEvent_t e) {
+  delete[] reinterpret_cast<char*>(e);
+  return hipSuccess;
+}
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline const char* hipGetErrorString(hipError_t) { return "emulated"; }
+inline hipError_t hipGetDevice(int* d) {
+  *d = 0;
+  return hipSuccess;
+}
+inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
+  *v = 4;  // few compute units: persistent grids stay small
+  return hipSuccess;
+}
+template <typename K>
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, K, int, size_t) {
+  *n = 2;
+  return hipSuccess;
+}
+
+template <typename... P, typename... A>
+inline void emu_launch(void (*kernel)(P...), dim3 grid, dim3 block, A&&... args) {
+  std::tuple<std::decay_t<P>...> t(static_cast<std::decay_t<P>>(args)...);
+  emu::run_grid(grid, block, [&]() { std::apply(kernel, t); });
+}
+#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) \
+  emu_launch(kernel, dim3(grid), dim3(block), ##__VA_ARGS__)
