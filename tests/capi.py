@@ -101,8 +101,12 @@ def bind(L):
 
 
 def check(status: int) -> None:
+    check_with(lib() if status != PS_OK else None, status)
+
+
+def check_with(L, status: int) -> None:
+    """Raises with the library's own message when status is not PS_OK."""
     if status != PS_OK:
-        L = lib()
         raise RuntimeError(f"{L.ps_status_string(status).decode()}: {L.ps_last_error().decode()}")
 
 

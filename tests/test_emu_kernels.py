@@ -1,0 +1,125 @@
+"""The HIP kernels themselves, run on the CPU through the host emulation
+(same sources, work-items as fibers), checked against the oracle.
+
+These cover the kernel logic without a GPU: sort semantics (ties, long
+segments, key bits above end_bit), one full frame bit-exact against the
+oracle, batched frames against single frames, and that no kernel reads
+memory it did not write (random-filled buffers give the same results as
+zeroed ones)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from polysplat_amd import scenes
+from tests import capi
+from tests.emu import frames as HF
+from tests.gpu_helpers import oracle_frame
+
+
+@pytest.fixture(scope="module")
+def emu():
+    subprocess.run([sys.executable, str(HF.HERE / "build.py")], check=True,
+                   stdout=subprocess.DEVNULL)
+    return HF.load(asan=False)
+
+
+def _sort(L, keys, vals, W, H, cap=None):
+    n = len(keys)
+    cap = max(n, 1) if cap is None else cap
+    ku = np.zeros(cap, np.uint64)
+    vu = np.zeros(cap, np.uint32)
+    ku[:n], vu[:n] = keys, vals
+    ks = np.full(cap, 2**64 - 1, np.uint64)
+    vs = np.full(cap, 2**32 - 1, np.uint32)
+    nb = L.ps_get_sort_buffer_size(cap)
+    space = np.zeros(nb, np.uint8)
+    capi.check_with(L, L.ps_sort_gaussian(n, None, cap, W, H, 16, 16, HF.ptr(space), nb,
+                                          HF.ptr(ku), HF.ptr(vu), HF.ptr(ks), HF.ptr(vs), None))
+    np.testing.assert_array_equal(ku[:n], keys)  # inputs preserved
+    return ks, vs
+
+
+def _tie_keys(counts, rng):
+    t = np.repeat(np.arange(len(counts), dtype=np.uint64), counts)
+    rng.shuffle(t)
+    d = rng.integers(0, 5, len(t)).astype(np.uint64) + np.uint64(0x3e4ccccd)
+    return (t << np.uint64(32)) | d
+
+
+@pytest.mark.parametrize("mode", ["segmented", "lsd"])
+def test_sort_segments_around_the_lds_limit(emu, mode, monkeypatch):
+    if mode == "lsd":
+        monkeypatch.setenv("POLYSPLAT_SORT", "lsd")
+    counts = [0, 1, 2, 63, 64, 65, 895, 896, 897, 1024, 1025, 3755, 0, 7]
+    rng = np.random.default_rng(99)
+    keys = _tie_keys(counts, rng)
+    vals = rng.permutation(len(keys)).astype(np.uint32)
+    k, v = _sort(emu, keys, vals, 64, 64, cap=len(keys) + 3)
+    rk, rv = O.sort_pairs(keys, vals)
+    n = len(keys)
+    np.testing.assert_array_equal(k[:n], rk)
+    np.testing.assert_array_equal(v[:n], rv)
+    assert (k[n:] == np.uint64(2**64 - 1)).all()
+
+
+def test_sort_orders_only_bits_below_end_bit(emu):
+    rng = np.random.default_rng(5)
+    n = 6000
+    t = rng.integers(0, 64, n).astype(np.uint64)
+    junk = rng.integers(0, 4, n).astype(np.uint64) << np.uint64(6)
+    d = rng.integers(0, 9, n).astype(np.uint64)
+    keys = ((t | junk) << np.uint64(32)) | d
+    vals = np.arange(n, dtype=np.uint32)
+    for mode in ("segmented", "lsd"):
+        os.environ["POLYSPLAT_SORT"] = mode
+        try:
+            k, v = _sort(emu, keys, vals, 160, 64)  # 40 tiles: end bit 38
+        finally:
+            del os.environ["POLYSPLAT_SORT"]
+        order = np.argsort(keys & np.uint64((1 << 38) - 1), kind="stable")
+        np.testing.assert_array_equal(v[:n], vals[order], err_msg=mode)
+        np.testing.assert_array_equal(k[:n], keys[order], err_msg=mode)
+
+
+def _scene():
+    return scenes.make_scene(3000, 10.0, [(0.8, -4.0, 0.5), (0.2, -1.8, 0.4)], seed=3)
+
+
+@pytest.mark.parametrize("garbage", [False, True])
+def test_frame_bit_exact_against_the_oracle(emu, garbage):
+    sc = _scene()
+    spec = scenes.make_camera(200, 120, 10.0, distance=12.0)
+    ref = oracle_frame(sc, spec, sh_type="fp16", feat_prec="fp16", bg=(0.1, 0.2, 0.3),
+                       check_every=8, fp64=False)
+    M = ref["pre"]["M"]
+    f = HF.HostFrame(emu, sc, spec, capacity=M + 100, garbage=garbage, seed=7)
+    st, n, img = f.forward((0.1, 0.2, 0.3))
+    capi.check_with(emu, st)
+    assert n == M
+    np.testing.assert_array_equal(f.ks[:M], ref["keys_sorted"])
+    np.testing.assert_array_equal(f.vs[:M], ref["vals_sorted"])
+    assert np.abs(img.astype(int) - ref["image"].astype(int)).max() <= 1
+
+
+def test_batched_frames_equal_single_frames(emu):
+    a = _scene()
+    b = scenes.make_scene(1500, 6.0, [(0.7, -3.5, 0.5), (0.3, -1.5, 0.3)], seed=8)
+    specs = [scenes.make_camera(200, 120, 10.0, distance=12.0),
+             scenes.make_camera(97, 65, 6.0, distance=7.0)]
+    bgs = [(0.0, 0.0, 0.0), (1.0, 0.5, 0.25)]
+    frames = [HF.HostFrame(emu, s, sp, capacity=200_000) for s, sp in zip((a, b), specs)]
+    singles = [f.forward(bg) for f, bg in zip(frames, bgs)]
+    for st, _, _ in singles:
+        capi.check_with(emu, st)
+    for order in ([0, 1], [1, 0]):
+        fr = [frames[i] for i in order]
+        batch = HF.HostBatch(emu, fr, 400_000, garbage=True)
+        st, counts, imgs = batch.run([bgs[i] for i in order])
+        capi.check_with(emu, st)
+        assert counts == [singles[i][1] for i in order]
+        for img, i in zip(imgs, order):
+            np.testing.assert_array_equal(img, singles[i][2])
