@@ -289,14 +289,25 @@ def main():
         gbs = alg[n] / (stage_ms[i] * 1e-3) / 1e9
         stages[n] = {"ms": round(float(stage_ms[i]), 4), "alg_bytes": int(alg[n]),
                      "GB_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
-    dom = max(names, key=lambda n: stages[n]["ms"])
-    roof = {"bound": "hbm", "kernel": dom, "achieved": stages[dom]["GB_s"],
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": stages[dom]["hbm_frac"],
-            "traffic": None}
+    # Roofline of the dominant kernel: the persistent render, one launch per
+    # frame (ev[3] -> ev[4] brackets exactly that launch on the stream it runs
+    # on).  Its algorithmic bytes: per needed pair a 4-byte id and a 24-byte
+    # record, per tile an 8-byte range, per pixel 3 output bytes.
+    rend_ms = float(stage_ms[3])
+    rend_bytes = alg["render"]
+    roof = {"bound": "hbm", "kernel": "ps_render_persistent<fp16 records, tile ranges>",
+            "achieved": round(rend_bytes / (rend_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(rend_bytes / (rend_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": None, "launch_ms": round(rend_ms, 4), "alg_bytes_per_launch": int(rend_bytes)}
+    prof = ROOT / "profiles" / "render_c3_hbm_traffic.json"
+    if prof.exists():
+        t = json.loads(prof.read_text())
+        roof["traffic"] = int(t["hbm_bytes_per_launch"])
+        roof["traffic_source"] = "profiles/render_c3_hbm_traffic.json"
     if Mn:
-        # render is VALU-bound: ~11 VALU lane-ops + one exp2 per pixel-Gaussian
-        px_evals = Mn * 256
-        roof["render_pixel_gaussians_per_s"] = px_evals / (stage_ms[3] * 1e-3)
+        # The render is bound by VALU issue (about a dozen VALU instructions
+        # and one exp2 per pixel and Gaussian), not by HBM.
+        roof["pixel_gaussians_per_s"] = Mn * 256 / (rend_ms * 1e-3)
 
     fp16 = results["fp16_persistent"]["ms_per_frame"]
     fp32 = results.get("fp32_plain", {}).get("ms_per_frame", float("nan"))
