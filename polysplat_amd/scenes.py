@@ -64,7 +64,9 @@ def cov3d_from_scale_rot(scale: torch.Tensor, rot: torch.Tensor) -> torch.Tensor
         torch.stack([2 * (x * z - r * y), 2 * (y * z + r * x), 1 - 2 * (x * x + y * y)], 1),
     ], 1)
     M = R * s[:, None, :]
-    sig = M @ M.transpose(1, 2)
+    # Sigma[a][b] = sum_r M[a][r] M[b][r], summed elementwise: no batched
+    # matrix product with a transposed operand on the device.
+    sig = (M[:, :, None, :] * M[:, None, :, :]).sum(-1)
     out = torch.stack([sig[:, 0, 0], sig[:, 0, 1], sig[:, 0, 2],
                        sig[:, 1, 1], sig[:, 1, 2], sig[:, 2, 2]], 1)
     return out.float()
