@@ -288,81 +288,52 @@ __device__ __forceinline__ uint64_t match_digit(int d, bool ok) {
   return peers;
 }
 
-// One stable counting pass of a wave's n LDS items (low words sd, values sv)
-// by digit `shift`, into dd/dv.  Items are taken in index order, 64 at a
-// time; a wave's LDS operations execute in order, so the per-digit counters
-// need no barrier between rounds.
-__device__ __forceinline__ void wave_counting_pass(const uint32_t* sd, const uint32_t* sv,
-                                                   uint32_t* dd, uint32_t* dv, uint32_t* cnt,
-                                                   int n, int shift, int lane, uint64_t below) {
-  for (int i = lane; i < kRadix; i += kWave) cnt[i] = 0;
-  __builtin_amdgcn_wave_barrier();
-  const int rounds = (n + kWave - 1) / kWave;
-  for (int r = 0; r < rounds; ++r) {
-    const int j = r * kWave + lane;
-    const bool ok = j < n;
-    const int d = ok ? (int)((sd[j] >> shift) & 0xffu) : 0;
-    const uint64_t peers = match_digit(d, ok);
-    if (ok && (peers & below) == 0) cnt[d] += (uint32_t)popc64(peers);
-    __builtin_amdgcn_wave_barrier();
-  }
-  const uint32_t c0 = cnt[4 * lane], c1 = cnt[4 * lane + 1], c2 = cnt[4 * lane + 2],
-                 c3 = cnt[4 * lane + 3];
-  const int sum = (int)(c0 + c1 + c2 + c3);
-  const uint32_t ex = (uint32_t)(wave_incl_scan(sum, lane) - sum);
-  __builtin_amdgcn_wave_barrier();
-  cnt[4 * lane] = ex;
-  cnt[4 * lane + 1] = ex + c0;
-  cnt[4 * lane + 2] = ex + c0 + c1;
-  cnt[4 * lane + 3] = ex + c0 + c1 + c2;
-  __builtin_amdgcn_wave_barrier();
-  for (int r = 0; r < rounds; ++r) {
-    const int j = r * kWave + lane;
-    const bool ok = j < n;
-    const uint32_t lo = ok ? sd[j] : 0u;
-    const uint32_t v = ok ? sv[j] : 0u;
-    const int d = (int)((lo >> shift) & 0xffu);
-    const uint64_t peers = match_digit(d, ok);
-    const uint32_t rank = (uint32_t)popc64(peers & below);
-    const uint32_t base = ok ? cnt[d] : 0u;
-    __builtin_amdgcn_wave_barrier();
-    if (ok && rank == 0) cnt[d] = base + (uint32_t)popc64(peers);
-    __builtin_amdgcn_wave_barrier();
-    if (ok) {
-      dd[base + rank] = lo;
-      dv[base + rank] = v;
-    }
-  }
-  __builtin_amdgcn_wave_barrier();
-}
-
 // Each wave takes segments seg = wave id, wave id + total waves, ...  A
 // segment longer than kSegCap, or whose keys' high words differ (bits above
-// the sorted ones), goes to the list for ps_seg_big.  Sorted in place.
+// the sorted ones), goes to the list for ps_seg_big.  The wave keeps its
+// segment in registers (item j in round j / 64 of lane j % 64) and runs one
+// stable counting pass per 8-bit digit of the low word: rank every item
+// among the segment's items with the same digit (ballot match per round,
+// per-digit counters in LDS), scatter the (low word, value) pairs through
+// LDS and read them back in the new order.  Digits that are the same for
+// every item of the segment are skipped (a stable pass over a constant digit
+// changes nothing); for depths the top byte usually is.  Sorted in place.
 __global__ __launch_bounds__(kSegWaves * kWave) void ps_seg_small(
     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, const int2* __restrict__ ranges,
     int segs, int* __restrict__ big_list, int* __restrict__ big_count, int big_cap) {
-  __shared__ uint32_t a_lo[kSegWaves][kSegCap];
-  __shared__ uint32_t a_v[kSegWaves][kSegCap];
-  __shared__ uint32_t b_lo[kSegWaves][kSegCap];
-  __shared__ uint32_t b_v[kSegWaves][kSegCap];
+  constexpr int R = kSegCap / kWave;  // rounds of 64 items
+  __shared__ uint32_t s_lo[kSegWaves][kSegCap];
+  __shared__ uint32_t s_v[kSegWaves][kSegCap];
   __shared__ uint32_t cnt[kSegWaves][kRadix];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t below = lanes_below(lane);
+  uint32_t* const slo = s_lo[wave];
+  uint32_t* const sv = s_v[wave];
+  uint32_t* const c = cnt[wave];
   for (int seg = blockIdx.x * kSegWaves + wave; seg < segs; seg += gridDim.x * kSegWaves) {
-    const int2 r = ranges[seg];
-    const int n = r.y - r.x;
+    const int2 rg = ranges[seg];
+    const int n = rg.y - rg.x;
     if (n <= 1) continue;
     bool big = n > kSegCap;
     uint32_t hi0 = 0;
+    uint32_t lo[R], v[R];
+    uint32_t diff = 0;
     if (!big) {
-      hi0 = (uint32_t)(keys[r.x] >> 32);
+      const uint64_t k0 = keys[rg.x];
+      hi0 = (uint32_t)(k0 >> 32);
       bool same = true;
-      for (int j = lane; j < n; j += kWave) {
-        const uint64_t k = keys[r.x + j];
-        same = same && (uint32_t)(k >> 32) == hi0;
-        a_lo[wave][j] = (uint32_t)k;
-        a_v[wave][j] = vals[r.x + j];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int j = r * kWave + lane;
+        lo[r] = 0;
+        v[r] = 0;
+        if (j < n) {
+          const uint64_t k = keys[rg.x + j];
+          same = same && (uint32_t)(k >> 32) == hi0;
+          lo[r] = (uint32_t)k;
+          v[r] = vals[rg.x + j];
+          diff |= lo[r] ^ (uint32_t)k0;
+        }
       }
       big = ballot(!same) != 0;
     }
@@ -373,15 +344,70 @@ __global__ __launch_bounds__(kSegWaves * kWave) void ps_seg_small(
       }
       continue;
     }
-    __builtin_amdgcn_wave_barrier();
-    wave_counting_pass(a_lo[wave], a_v[wave], b_lo[wave], b_v[wave], cnt[wave], n, 0, lane, below);
-    wave_counting_pass(b_lo[wave], b_v[wave], a_lo[wave], a_v[wave], cnt[wave], n, 8, lane, below);
-    wave_counting_pass(a_lo[wave], a_v[wave], b_lo[wave], b_v[wave], cnt[wave], n, 16, lane, below);
-    wave_counting_pass(b_lo[wave], b_v[wave], a_lo[wave], a_v[wave], cnt[wave], n, 24, lane, below);
+    const int rounds = (n + kWave - 1) / kWave;
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) diff |= (uint32_t)__shfl_xor((int)diff, s);
+    for (int p = 0; p < 4; ++p) {
+      const int shift = 8 * p;
+      if (((diff >> shift) & 0xffu) == 0) continue;  // uniform: constant digit
+      for (int i = lane; i < kRadix; i += kWave) c[i] = 0;
+      __builtin_amdgcn_wave_barrier();
+      uint64_t pm[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r < rounds) {
+          const bool ok = r * kWave + lane < n;
+          const int d = (int)((lo[r] >> shift) & 0xffu);
+          pm[r] = match_digit(d, ok);
+          if (ok && (pm[r] & below) == 0) c[d] += (uint32_t)popc64(pm[r]);
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+      const uint32_t c0 = c[4 * lane], c1 = c[4 * lane + 1], c2 = c[4 * lane + 2],
+                     c3 = c[4 * lane + 3];
+      const int sum = (int)(c0 + c1 + c2 + c3);
+      const uint32_t ex = (uint32_t)(wave_incl_scan(sum, lane) - sum);
+      __builtin_amdgcn_wave_barrier();
+      c[4 * lane] = ex;
+      c[4 * lane + 1] = ex + c0;
+      c[4 * lane + 2] = ex + c0 + c1;
+      c[4 * lane + 3] = ex + c0 + c1 + c2;
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r < rounds) {
+          const bool ok = r * kWave + lane < n;
+          const int d = (int)((lo[r] >> shift) & 0xffu);
+          const uint32_t rank = (uint32_t)popc64(pm[r] & below);
+          const uint32_t base = ok ? c[d] : 0u;
+          __builtin_amdgcn_wave_barrier();
+          if (ok && rank == 0) c[d] = base + (uint32_t)popc64(pm[r]);
+          __builtin_amdgcn_wave_barrier();
+          if (ok) {
+            slo[base + rank] = lo[r];
+            sv[base + rank] = v[r];
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int j = r * kWave + lane;
+        if (r < rounds && j < n) {
+          lo[r] = slo[j];
+          v[r] = sv[j];
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
     const uint64_t hi = (uint64_t)hi0 << 32;
-    for (int j = lane; j < n; j += kWave) {
-      keys[r.x + j] = hi | a_lo[wave][j];
-      vals[r.x + j] = a_v[wave][j];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int j = r * kWave + lane;
+      if (j < n) {
+        keys[rg.x + j] = hi | lo[r];
+        vals[rg.x + j] = v[r];
+      }
     }
   }
 }
