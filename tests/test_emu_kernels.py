@@ -123,3 +123,25 @@ def test_batched_frames_equal_single_frames(emu):
         assert counts == [singles[i][1] for i in order]
         for img, i in zip(imgs, order):
             np.testing.assert_array_equal(img, singles[i][2])
+
+
+def test_quadrant_renderer_matches_the_tile_renderer(emu, monkeypatch):
+    # The precomputed-range path renders with the quadrant renderer; the
+    # one-wave-per-tile renderer is its reference: same pixels to within one
+    # 8-bit level (skipped Gaussians add < 2^-32; quadrants stop on their own).
+    sc = scenes.make_scene(4000, 8.0, [(0.85, -4.0, 0.5), (0.12, -2.2, 0.4), (0.03, -0.7, 0.3)],
+                           seed=12)
+    spec = scenes.make_camera(150, 110, 8.0, distance=11.0)
+    f = HF.HostFrame(emu, sc, spec, capacity=400_000)
+    st, n, quad = f.forward((0.2, 0.3, 0.4))
+    capi.check_with(emu, st)
+    monkeypatch.setenv("POLYSPLAT_RENDER", "tile")
+    st, n2, tile = f.forward((0.2, 0.3, 0.4))
+    capi.check_with(emu, st)
+    assert n == n2
+    d = np.abs(quad.astype(int) - tile.astype(int))
+    assert d.max() <= 1
+    assert (d > 0).mean() < 0.01
+    ref = oracle_frame(sc, spec, sh_type="fp16", feat_prec="fp16", bg=(0.2, 0.3, 0.4),
+                       check_every=8, fp64=False)
+    assert np.abs(quad.astype(int) - ref["image"].astype(int)).max() <= 1
