@@ -26,7 +26,6 @@
 #include "ps_host.h"
 #include "ps_wave.h"
 
-#include <stdlib.h>
 
 namespace ps {
 namespace {
@@ -103,6 +102,14 @@ __device__ __forceinline__ Feat load_feat(int idx, const float2* __restrict__ xy
   return f;
 }
 
+// Stands in for the slots past the end of a tile's list.
+__device__ __forceinline__ Feat neutral_feat() {
+  Feat f;
+  f.x = f.y = f.a = f.b = f.c = f.r = f.g = f.bl = 0.0f;
+  f.w = -__builtin_inff();
+  return f;
+}
+
 __device__ __forceinline__ uint8_t encode(float x) {
   return (uint8_t)fminf(fmaxf(0.0f, x * 255.0f), 255.0f);
 }
@@ -118,6 +125,7 @@ __device__ __forceinline__ void render_tile(
     const float2* __restrict__ xy, const float4* __restrict__ rgbd, const float4* __restrict__ con,
     const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
     float4 (*sA)[kBatch], float4 (*sB)[kBatch], float (*sC)[kBatch], int lane) {
+  static_assert(kBatch % CHECK == 0, "a batch holds whole check groups");
   const int tx = tile % gx, ty = tile / gx;
   const int px0 = tx * kTile + 2 * (lane & 7);
   const int py0 = ty * kTile + 2 * (lane >> 3);
@@ -127,8 +135,10 @@ __device__ __forceinline__ void render_tile(
   float Cr[4] = {0.f, 0.f, 0.f, 0.f}, Cg[4] = {0.f, 0.f, 0.f, 0.f}, Cb[4] = {0.f, 0.f, 0.f, 0.f};
   const int n = end - start;
   if (n > 0) {
-    // Stage batch 0, prefetch batch 1.
-    Feat pre;
+    // Stage batch 0, prefetch batch 1.  Slots past the tile's list hold a
+    // neutral Gaussian (power -inf: alpha 0, colour and T unchanged), so the
+    // shading loop runs whole groups of CHECK with no per-Gaussian bound.
+    Feat pre = neutral_feat();
     if (lane < n) {
       const int idx = GATHERED ? start + lane : (int)order[start + lane];
       pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
@@ -137,6 +147,7 @@ __device__ __forceinline__ void render_tile(
     sA[0][lane] = make_float4(pre.x, pre.y, pre.w, pre.a);
     sB[0][lane] = make_float4(pre.b, pre.c, pre.r, pre.g);
     sC[0][lane] = pre.bl;
+    pre = neutral_feat();
     if (kBatch + lane < n) {
       const int idx = GATHERED ? start + kBatch + lane : (int)order[start + kBatch + lane];
       pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
@@ -144,39 +155,43 @@ __device__ __forceinline__ void render_tile(
     __syncthreads();
     for (int b0 = 0;; b0 += kBatch) {
       const int cnt = (n - b0 < kBatch) ? (n - b0) : kBatch;
+      const int groups = (cnt + CHECK - 1) / CHECK;
       bool done = false;
-      for (int k = 0; k < cnt; ++k) {
-        const float4 A = sA[cur][k];
-        const float4 B = sB[cur][k];
-        const float bl = sC[cur][k];
-        // A = (x, y, w, a), B = (b, c, r, g)
-        const float dx0 = A.x - pxf0, dx1 = A.x - pxf1;
-        const float dy0 = A.y - pyf0, dy1 = A.y - pyf1;
-        const float xs0 = A.w * dx0 * dx0, xs1 = A.w * dx1 * dx1;
-        const float xc0 = B.x * dx0, xc1 = B.x * dx1;
-        const float ys0 = A.z + B.y * dy0 * dy0, ys1 = A.z + B.y * dy1 * dy1;
-        float pw[4];
-        pw[0] = fmaf(xc0, dy0, xs0 + ys0);
-        pw[1] = fmaf(xc1, dy0, xs1 + ys0);
-        pw[2] = fmaf(xc0, dy1, xs0 + ys1);
-        pw[3] = fmaf(xc1, dy1, xs1 + ys1);
+      for (int g = 0; g < groups && !done; ++g) {
+        // Two Gaussians per loop trip: LDS addresses as immediate offsets and
+        // no per-Gaussian loop overhead, within 64 VGPRs (8 waves per SIMD).
+#pragma unroll 2
+        for (int j = 0; j < CHECK; ++j) {
+          const int k = g * CHECK + j;
+          const float4 A = sA[cur][k];
+          const float4 B = sB[cur][k];
+          const float bl = sC[cur][k];
+          // A = (x, y, w, a), B = (b, c, r, g)
+          const float dx0 = A.x - pxf0, dx1 = A.x - pxf1;
+          const float dy0 = A.y - pyf0, dy1 = A.y - pyf1;
+          const float xs0 = A.w * dx0 * dx0, xs1 = A.w * dx1 * dx1;
+          const float xc0 = B.x * dx0, xc1 = B.x * dx1;
+          const float ys0 = A.z + B.y * dy0 * dy0, ys1 = A.z + B.y * dy1 * dy1;
+          float pw[4];
+          pw[0] = fmaf(xc0, dy0, xs0 + ys0);
+          pw[1] = fmaf(xc1, dy0, xs1 + ys0);
+          pw[2] = fmaf(xc0, dy1, xs0 + ys1);
+          pw[3] = fmaf(xc1, dy1, xs1 + ys1);
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
-          if (DT && pw[p] < -7.994353375f) continue;
-          const float alpha = __builtin_amdgcn_exp2f(pw[p]);
-          const float wgt = alpha * T[p];
-          Cr[p] = fmaf(B.z, wgt, Cr[p]);
-          Cg[p] = fmaf(B.w, wgt, Cg[p]);
-          Cb[p] = fmaf(bl, wgt, Cb[p]);
-          T[p] = T[p] - wgt;
-        }
-        if ((k % CHECK) == CHECK - 1) {
-          const bool sat = T[0] < kTMin && T[1] < kTMin && T[2] < kTMin && T[3] < kTMin;
-          if (ballot(!sat) == 0) {
-            done = true;
-            break;
+          for (int p = 0; p < 4; ++p) {
+            if (DT && pw[p] < -7.994353375f) continue;
+            const float alpha = __builtin_amdgcn_exp2f(pw[p]);
+            const float wgt = alpha * T[p];
+            Cr[p] = fmaf(B.z, wgt, Cr[p]);
+            Cg[p] = fmaf(B.w, wgt, Cg[p]);
+            Cb[p] = fmaf(bl, wgt, Cb[p]);
+            T[p] = T[p] - wgt;
           }
         }
+        // Saturation test after every CHECK Gaussians of the tile (the
+        // padded group at the end of the list changes nothing).
+        const bool sat = T[0] < kTMin && T[1] < kTMin && T[2] < kTMin && T[3] < kTMin;
+        done = ballot(!sat) == 0;
       }
       if (done || b0 + kBatch >= n) break;
       // Publish the prefetched batch, start loading the one after it.
@@ -185,6 +200,7 @@ __device__ __forceinline__ void render_tile(
       sA[cur][lane] = make_float4(pre.x, pre.y, pre.w, pre.a);
       sB[cur][lane] = make_float4(pre.b, pre.c, pre.r, pre.g);
       sC[cur][lane] = pre.bl;
+      pre = neutral_feat();
       const int nb = b0 + 2 * kBatch + lane;
       if (nb < n) {
         const int idx = GATHERED ? start + nb : (int)order[start + nb];
@@ -214,7 +230,7 @@ __device__ __forceinline__ void render_tile(
 
 // One workgroup per tile.
 template <int PREC, int CHECK>
-__global__ __launch_bounds__(kWave) void ps_render_plain(
+__global__ __launch_bounds__(kWave, 8) void ps_render_plain(
     const uint64_t* __restrict__ keys, const int* __restrict__ d_n, int n_host, int W, int H,
     int gx, const uint32_t* __restrict__ order, const float2* __restrict__ xy,
     const float4* __restrict__ rgbd, const float4* __restrict__ con,
@@ -233,7 +249,7 @@ __global__ __launch_bounds__(kWave) void ps_render_plain(
 
 // Persistent: tiles from the queue.  RANGES: read tile_ranges, else search.
 template <int PREC, bool RANGES, bool GATHERED, int CHECK, bool DT>
-__global__ __launch_bounds__(kWave) void ps_render_persistent(
+__global__ __launch_bounds__(kWave, 8) void ps_render_persistent(
     const uint64_t* __restrict__ keys, const int* __restrict__ d_n, int n_host,
     const int2* __restrict__ ranges, int W, int H, int gx, int tiles,
     const uint32_t* __restrict__ order, const float2* __restrict__ xy,
@@ -272,174 +288,13 @@ __global__ __launch_bounds__(kWave) void ps_render_persistent(
   }
 }
 
-// ------------------------------------------------------------ quadrant renderer
-// Persistent renderer for the precomputed-range path: one 256-thread
-// workgroup per tile, one wave per 8x8 quadrant, one pixel per lane.  When a
-// batch of 64 Gaussians is staged in LDS, each Gaussian gets a 4-bit mask of
-// the quadrants where its alpha can reach 2^-32: with the power written as
-// w + q(d), q(d) = a dx^2 + b dx dy + c dy^2 negative definite, q(d) <= lmax |d|^2
-// (lmax the eigenvalue of q nearest zero, taken with a relative rounding
-// allowance), so w + lmax * dist^2(centre, quadrant) bounds the power over
-// the quadrant.  A wave shades only the Gaussians live in its quadrant; a
-// skipped Gaussian would have added at most 2^-32 of colour and left T
-// unchanged in FP32.  Each quadrant stops on its own once all its pixels
-// have T < 1e-4 (tested every 8 shaded Gaussians); the pixel arithmetic is
-// the same as render_tile's.  Staging rotates over the four waves, one
-// batch ahead, so loads overlap shading.
-constexpr float kQuadSkipPower = -32.0f;
-
-__device__ __forceinline__ float dist_to_span(float v, float lo, float hi) {
-  return v < lo ? lo - v : (v > hi ? v - hi : 0.0f);
-}
-
-// Quadrant bit k (k = qy * 2 + qx) set when the Gaussian can reach
-// 2^kQuadSkipPower somewhere in quadrant k of the tile at (tx0, ty0).
-__device__ __forceinline__ uint32_t quad_mask(const Feat& f, float tx0, float ty0) {
-  const float h = 0.5f * (f.a - f.c);
-  const float root = __builtin_sqrtf(h * h + 0.25f * f.b * f.b);
-  float lmax = 0.5f * (f.a + f.c) + root;
-  lmax += 1e-6f * (fabsf(f.a) + fabsf(f.b) + fabsf(f.c));  // rounding allowance
-  lmax = fminf(lmax, 0.0f);
-  uint32_t m = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const float x0 = tx0 + 8.0f * (float)(k & 1), y0 = ty0 + 8.0f * (float)(k >> 1);
-    const float ddx = dist_to_span(f.x, x0, x0 + 7.0f);
-    const float ddy = dist_to_span(f.y, y0, y0 + 7.0f);
-    const float bound = f.w + lmax * (ddx * ddx + ddy * ddy);
-    if (!(bound < kQuadSkipPower)) m |= 1u << k;  // NaN keeps the quadrant
-  }
-  return m;
-}
-
-template <int PREC>
-__global__ __launch_bounds__(4 * kWave) void ps_render_quad(
-    const int2* __restrict__ ranges, int W, int H, int gx, int tiles,
-    const uint32_t* __restrict__ order, const float2* __restrict__ xy,
-    const float4* __restrict__ rgbd, const float4* __restrict__ con,
-    const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
-    unsigned* __restrict__ queue) {
-  __shared__ float4 sA[2][kBatch];  // x, y, w, a
-  __shared__ float4 sB[2][kBatch];  // b, c, r, g
-  __shared__ float sC[2][kBatch];   // blue
-  __shared__ uint32_t sM[2][kBatch];
-  __shared__ int s_done[4];
-  __shared__ int s_tile;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint64_t below = lanes_below(lane);
-  (void)below;
-  int tile = blockIdx.x;
-  while (tile < tiles) {
-    const int2 rg = ranges[tile];
-    const int start = rg.x, n = rg.y - rg.x;
-    const int tx = tile % gx, ty = tile / gx;
-    const float tx0 = (float)(tx * kTile), ty0 = (float)(ty * kTile);
-    const int px = tx * kTile + 8 * (wave & 1) + (lane & 7);
-    const int py = ty * kTile + 8 * (wave >> 1) + (lane >> 3);
-    const float pxf = (float)px, pyf = (float)py;
-    float T = 1.0f, Cr = 0.0f, Cg = 0.0f, Cb = 0.0f;
-    if (n > 0) {
-      // Stage batch 0 (wave 0).
-      if (wave == 0) {
-        Feat f;
-        f.x = f.y = f.w = f.a = f.b = f.c = f.r = f.g = f.bl = 0.0f;
-        uint32_t m = 0;
-        if (lane < n) {
-          f = load_feat<PREC>((int)order[start + lane], xy, rgbd, con, rec);
-          m = quad_mask(f, tx0, ty0);
-        }
-        sA[0][lane] = make_float4(f.x, f.y, f.w, f.a);
-        sB[0][lane] = make_float4(f.b, f.c, f.r, f.g);
-        sC[0][lane] = f.bl;
-        sM[0][lane] = m;
-      }
-      __syncthreads();
-      bool done = false;
-      int shaded = 0;
-      for (int b = 0;; ++b) {
-        const int cur = b & 1;
-        const int nb0 = (b + 1) * kBatch;  // first Gaussian of the next batch
-        const bool stager = wave == ((b + 1) & 3) && nb0 < n;
-        Feat nf;
-        nf.x = nf.y = nf.w = nf.a = nf.b = nf.c = nf.r = nf.g = nf.bl = 0.0f;
-        if (stager && nb0 + lane < n)
-          nf = load_feat<PREC>((int)order[start + nb0 + lane], xy, rgbd, con, rec);
-        if (!done) {
-          const int cnt = (n - b * kBatch < kBatch) ? (n - b * kBatch) : kBatch;
-          const uint32_t mine = (lane < cnt) ? sM[cur][lane] : 0u;
-          uint64_t live = ballot(((mine >> wave) & 1u) != 0u);
-          while (live) {
-            const int g = __builtin_ctzll(live);
-            live &= live - 1;
-            const float4 A = sA[cur][g];
-            const float4 B = sB[cur][g];
-            const float bl = sC[cur][g];
-            const float dx = A.x - pxf, dy = A.y - pyf;
-            const float xs = A.w * dx * dx;
-            const float xc = B.x * dx;
-            const float ys = A.z + B.y * dy * dy;
-            const float pw = fmaf(xc, dy, xs + ys);
-            const float alpha = __builtin_amdgcn_exp2f(pw);
-            const float wgt = alpha * T;
-            Cr = fmaf(B.z, wgt, Cr);
-            Cg = fmaf(B.w, wgt, Cg);
-            Cb = fmaf(bl, wgt, Cb);
-            T = T - wgt;
-            if ((++shaded & 7) == 0 && ballot(!(T < kTMin)) == 0) {
-              done = true;
-              break;
-            }
-          }
-        }
-        if (stager) {
-          const int slot = cur ^ 1;
-          const uint32_t m = (nb0 + lane < n) ? quad_mask(nf, tx0, ty0) : 0u;
-          sA[slot][lane] = make_float4(nf.x, nf.y, nf.w, nf.a);
-          sB[slot][lane] = make_float4(nf.b, nf.c, nf.r, nf.g);
-          sC[slot][lane] = nf.bl;
-          sM[slot][lane] = m;
-        }
-        if (lane == 0) s_done[wave] = done ? 1 : 0;
-        __syncthreads();
-        const bool all_done = s_done[0] && s_done[1] && s_done[2] && s_done[3];
-        if (all_done || nb0 >= n) break;
-      }
-    }
-    if (px < W && py < H) {
-      uint8_t* o = out + ((size_t)py * W + px) * 3;
-      if (n > 0) {
-        const float t = T < kTMin ? 0.0f : T;
-        o[0] = encode(Cr + t * bg.x);
-        o[1] = encode(Cg + t * bg.y);
-        o[2] = encode(Cb + t * bg.z);
-      } else {
-        o[0] = encode(bg.x);
-        o[1] = encode(bg.y);
-        o[2] = encode(bg.z);
-      }
-    }
-    if (tid == 0) s_tile = (int)atomicAdd(&queue[0], 1u) + (int)gridDim.x;
-    __syncthreads();
-    tile = s_tile;
-    __syncthreads();  // s_tile and the staging buffers are rewritten next
-  }
-  // The last workgroup out resets the queue for the next launch.
-  if (tid == 0) {
-    const unsigned finished = atomicAdd(&queue[1], 1u);
-    if (finished == gridDim.x - 1) {
-      atomicExch(&queue[0], 0u);
-      atomicExch(&queue[1], 0u);
-    }
-  }
-}
-
 // Persistent renderer over the tiles of several frames (one queue for all):
 // global tile t belongs to the last frame whose tile0 <= t.  Frame tile
 // offsets are staged in LDS once per workgroup.
 constexpr int kMaxBatch = PS_MAX_BATCH_FRAMES;
 
 template <int PREC, int CHECK>
-__global__ __launch_bounds__(kWave) void ps_render_persistent_batch(
+__global__ __launch_bounds__(kWave, 8) void ps_render_persistent_batch(
     const FrameDev* __restrict__ frames, int nf, const int2* __restrict__ ranges, int tiles,
     const uint32_t* __restrict__ order, unsigned* __restrict__ queue) {
   __shared__ float4 sA[2][kBatch];
@@ -475,14 +330,6 @@ __global__ __launch_bounds__(kWave) void ps_render_persistent_batch(
   }
 }
 
-// Renderer for the precomputed-range path: the quadrant renderer unless
-// POLYSPLAT_RENDER=tile selects the one-wave-per-tile renderer (both stay
-// within one 8-bit level of the oracle).
-inline bool quad_renderer() {
-  const char* e = getenv("POLYSPLAT_RENDER");
-  return !(e && e[0] == 't');
-}
-
 struct GridCache {
   int device = -1;
   int cus = 0;
@@ -503,10 +350,10 @@ int device_cus(hipStream_t) {
 }
 
 template <typename K>
-int persistent_grid(K kernel, int tiles, int block = kWave) {
+int persistent_grid(K kernel, int tiles) {
   int cus = device_cus(nullptr);
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWave, 0) != hipSuccess ||
       per_cu <= 0)
     per_cu = 8;
   long long g = (long long)(cus > 0 ? cus : 256) * per_cu;
@@ -521,16 +368,6 @@ int launch_persistent(const uint64_t* keys, const int* d_n, int n_host, const in
                       hipStream_t stream) {
   const int gx = (W + kTile - 1) / kTile, gy = (H + kTile - 1) / kTile;
   const int tiles = gx * gy;
-  if (RANGES && !GATHERED && !DT && quad_renderer()) {
-    auto kq = ps_render_quad<PREC>;
-    const int grid = persistent_grid(kq, tiles, 4 * kWave);
-    hipLaunchKernelGGL(kq, dim3(grid), dim3(4 * kWave), 0, stream,
-                       reinterpret_cast<const int2*>(ranges), W, H, gx, tiles, order,
-                       (const float2*)xy, (const float4*)rgbd, (const float4*)con,
-                       (const FeatRec16*)rec, bg, out, queue);
-    PS_LAUNCH_CHECK();
-    return PS_OK;
-  }
   auto k = ps_render_persistent<PREC, RANGES, GATHERED, CHECK, DT>;
   const int grid = persistent_grid(k, tiles);
   hipLaunchKernelGGL(k, dim3(grid), dim3(kWave), 0, stream, keys, d_n, n_host,

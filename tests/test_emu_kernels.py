@@ -125,23 +125,30 @@ def test_batched_frames_equal_single_frames(emu):
             np.testing.assert_array_equal(img, singles[i][2])
 
 
-def test_quadrant_renderer_matches_the_tile_renderer(emu, monkeypatch):
-    # The precomputed-range path renders with the quadrant renderer; the
-    # one-wave-per-tile renderer is its reference: same pixels to within one
-    # 8-bit level (skipped Gaussians add < 2^-32; quadrants stop on their own).
-    sc = scenes.make_scene(4000, 8.0, [(0.85, -4.0, 0.5), (0.12, -2.2, 0.4), (0.03, -0.7, 0.3)],
-                           seed=12)
-    spec = scenes.make_camera(150, 110, 8.0, distance=11.0)
-    f = HF.HostFrame(emu, sc, spec, capacity=400_000)
-    st, n, quad = f.forward((0.2, 0.3, 0.4))
-    capi.check_with(emu, st)
-    monkeypatch.setenv("POLYSPLAT_RENDER", "tile")
-    st, n2, tile = f.forward((0.2, 0.3, 0.4))
-    capi.check_with(emu, st)
-    assert n == n2
-    d = np.abs(quad.astype(int) - tile.astype(int))
-    assert d.max() <= 1
-    assert (d > 0).mean() < 0.01
-    ref = oracle_frame(sc, spec, sh_type="fp16", feat_prec="fp16", bg=(0.2, 0.3, 0.4),
+def test_renderer_pads_partial_groups_neutrally(emu):
+    # Tile lists of every length modulo the check cadence (and past one and
+    # two 64-Gaussian batches): the renderer shades whole groups, the slots
+    # past a list's end must change nothing.  Dense, large splats give long
+    # lists; the persistent (cadence 8) and plain (cadence 2) paths both run.
+    sc = scenes.make_scene(2500, 6.0, [(0.6, -3.0, 0.5), (0.4, -1.2, 0.4)], seed=31)
+    spec = scenes.make_camera(96, 80, 6.0, distance=8.0)
+    ref = oracle_frame(sc, spec, sh_type="fp32", feat_prec="fp32", bg=(0.3, 0.1, 0.2),
                        check_every=8, fp64=False)
-    assert np.abs(quad.astype(int) - ref["image"].astype(int)).max() <= 1
+    lens = np.diff(ref["ranges"], axis=1).ravel()
+    assert len(set(lens % 8)) >= 6 and lens.max() > 128, lens
+    f = HF.HostFrame(emu, sc, spec, sh_type="fp32", feat_prec="fp32",
+                     capacity=ref["pre"]["M"] + 64, garbage=True, seed=4)
+    st, n, img = f.forward((0.3, 0.1, 0.2))
+    capi.check_with(emu, st)
+    assert n == ref["pre"]["M"]
+    assert np.abs(img.astype(int) - ref["image"].astype(int)).max() <= 1
+    # plain renderer over the same sorted pairs (cadence 2)
+    out = np.zeros_like(img)
+    keys = f.ks[:n].copy()
+    capi.check_with(emu, emu.ps_render_16x16(n, None, spec.width, spec.height, HF.ptr(f.xy),
+                                             HF.ptr(f.rgbd), HF.ptr(f.conic), HF.ptr(keys),
+                                             HF.ptr(f.vs), capi.floats3((0.3, 0.1, 0.2)),
+                                             HF.ptr(out), None))
+    ref2 = oracle_frame(sc, spec, sh_type="fp32", feat_prec="fp32", bg=(0.3, 0.1, 0.2),
+                        check_every=2, fp64=False)
+    assert np.abs(out.astype(int) - ref2["image"].astype(int)).max() <= 1
