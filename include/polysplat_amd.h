@@ -81,9 +81,13 @@ int ps_camera_matrices(const ps_camera* cam, float view[16], float proj[16], flo
  * Keys u64[capacity] (tile << 32 | depth bits) and values u32[capacity] are
  * appended at *curr_offset in a canonical order (Gaussian id, then tile
  * row-major); *curr_offset is advanced by the number of pairs.  If the pairs
- * do not fit, nothing is emitted, *curr_offset still receives the required
- * total and bit 0 of the status word in the workspace is set (read it with
- * ps_preprocess_status).  workspace: ps_preprocess_workspace_size(P) bytes. */
+ * do not fit (or *curr_offset is negative), no pair is written at or past
+ * `capacity` (pairs below it may be), *curr_offset still receives the
+ * required total and bit 0 of the status word is set (the device pointer
+ * from ps_preprocess_status_ptr).  Bit 1 of the status word reports that the
+ * single-launch emission gave up waiting for a predecessor workgroup, which
+ * a correct kernel never does.  workspace: ps_preprocess_workspace_size(P)
+ * bytes; no part of it needs clearing. */
 size_t ps_preprocess_workspace_size(int P);
 int ps_preprocess(int P, const float* positions, const void* shs, int sh_prec,
                   const float* opacities, const float* cov3d, const ps_camera* cam,

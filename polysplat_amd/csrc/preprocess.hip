@@ -34,6 +34,7 @@
 #include "ps_host.h"
 #include "ps_wave.h"
 
+#include <stdlib.h>
 #include <string.h>
 
 #include <type_traits>
@@ -43,6 +44,16 @@ namespace {
 
 constexpr int kPreThreads = 256;
 constexpr int kMassive = 64;
+
+// A lane with no Gaussian (past P, culled, or emitting nothing).
+__device__ __forceinline__ Proj no_proj() {
+  Proj p;
+  p.xy_x = p.xy_y = p.con_a = p.con_b = p.con_c = 0.0f;
+  p.power = p.log2_opacity = p.depth = 0.0f;
+  p.rmin_x = p.rmin_y = p.rmax_x = p.rmax_y = 0;
+  p.valid = false;
+  return p;
+}
 
 // Per-lane view of a projected Gaussian used by the sweeps.
 struct Cand {
@@ -115,13 +126,38 @@ __device__ __forceinline__ Cand permute_cand(const Cand& v, int src) {
   return c;
 }
 
+// The sweeps report each step to a sink, called by the whole wave once per
+// step: `ok` is this lane's tile-test result, `m` the step's ballot of them,
+// `tile` this lane's frame-local tile id, `owner` the lane that owns this
+// lane's Gaussian and `k` the index of this hit among that Gaussian's hits
+// (valid where ok).
+struct NullSink {
+  __device__ __forceinline__ void step(bool, uint64_t, int, int, int) {}
+};
+
+// Writes keys straight to global memory at off(owner) + k, never past the
+// owner's count.
+struct GlobalSink {
+  int off, cnt, gid, tile0;
+  float depth;
+  uint64_t* keys;
+  uint32_t* vals;
+  __device__ __forceinline__ void step(bool ok, uint64_t, int tile, int owner, int k) {
+    const int o_off = permute(off, owner), o_cnt = permute(cnt, owner);
+    const int o_gid = permute(gid, owner);
+    const float o_depth = permute(depth, owner);
+    if (ok && k < o_cnt) {
+      keys[o_off + k] = make_key(tile0 + tile, o_depth);
+      vals[o_off + k] = (uint32_t)o_gid;
+    }
+  }
+};
+
 // Massive Gaussians: the wave walks each one's rect 64 tiles at a time in
-// row-major order.  Returns this lane's own tile count (count mode); in emit
-// mode writes keys at off + running index.
-template <bool EMIT>
-__device__ int sweep_massive(const Cand& me, bool is_massive, int lane, int gid, int off, int cnt,
-                             int bx, int by, int grid_x, int tile0, uint64_t* keys,
-                             uint32_t* vals) {
+// row-major order.  Returns this lane's own tile count.
+template <class Sink>
+__device__ int sweep_massive(const Cand& me, bool is_massive, int lane, int bx, int by,
+                             int grid_x, Sink& sink) {
   uint64_t todo = ballot(is_massive);
   int mine = 0;
   const uint64_t below = lanes_below(lane);
@@ -132,9 +168,6 @@ __device__ int sweep_massive(const Cand& me, bool is_massive, int lane, int gid,
     const Proj pq = proj_from(q);
     const int w = q.rx1 - q.rx0;
     const int total = w * (q.ry1 - q.ry0);
-    const int qoff = EMIT ? bcast(off, i) : 0;
-    const int qcnt = EMIT ? bcast(cnt, i) : 0;
-    const int qgid = bcast(gid, i);
     int run = 0;
     for (int t0 = 0; t0 < total; t0 += kWave) {
       const int t = t0 + lane;
@@ -146,14 +179,7 @@ __device__ int sweep_massive(const Cand& me, bool is_massive, int lane, int gid,
         ok = tile_hits(q.rx0 + tx, q.ry0 + ty, pq, bx, by);
       }
       const uint64_t m = ballot(ok);
-      const int k = run + popc64(m & below);
-      // The count pass found qcnt tiles with the same arithmetic; the bound
-      // keeps every write inside this Gaussian's slots regardless.
-      if (EMIT && ok && k < qcnt) {
-        const int slot = qoff + k;
-        keys[slot] = make_key(tile0 + (q.ry0 + ty) * grid_x + (q.rx0 + tx), q.depth);
-        vals[slot] = (uint32_t)qgid;
-      }
+      sink.step(ok, m, (q.ry0 + ty) * grid_x + (q.rx0 + tx), i, run + popc64(m & below));
       run += popc64(m);
     }
     if (lane == i) mine = run;
@@ -163,11 +189,10 @@ __device__ int sweep_massive(const Cand& me, bool is_massive, int lane, int gid,
 
 // Fragmented Gaussians: all lanes' small rects become one stream of
 // (Gaussian, tile) tasks, 64 per step; the owner of task t is the first lane
-// whose inclusive prefix of areas exceeds t.
-template <bool EMIT>
-__device__ int sweep_fragmented(const Cand& me, bool is_frag, int lane, int gid, int off, int cnt,
-                                int bx, int by, int grid_x, int tile0, uint64_t* keys,
-                                uint32_t* vals) {
+// whose inclusive prefix of areas exceeds t.  Returns this lane's own count.
+template <class Sink>
+__device__ int sweep_fragmented(const Cand& me, bool is_frag, int lane, int bx, int by,
+                                int grid_x, Sink& sink) {
   const int area = is_frag ? (me.rx1 - me.rx0) * (me.ry1 - me.ry0) : 0;
   const int incl = wave_incl_scan(area, lane);
   const int excl = incl - area;
@@ -187,10 +212,7 @@ __device__ int sweep_fragmented(const Cand& me, bool is_frag, int lane, int gid,
     const int o = tv ? lo : lane;
     const Cand q = permute_cand(me, o);
     const int o_excl = permute(excl, o);
-    const int o_gid = permute(gid, o);
-    const int o_off = EMIT ? permute(off, o) : 0;
-    const int o_done = EMIT ? permute(mine, o) : 0;
-    const int o_cnt = EMIT ? permute(cnt, o) : 0;
+    const int o_done = permute(mine, o);
     bool ok = false;
     int tx = 0, ty = 0;
     if (tv) {
@@ -202,12 +224,8 @@ __device__ int sweep_fragmented(const Cand& me, bool is_frag, int lane, int gid,
     }
     const uint64_t m = ballot(ok);
     const int seg_lo = imax(o_excl - t0, 0);
-    const int k = o_done + popc64(m & below & ~lanes_below(seg_lo));
-    if (EMIT && ok && k < o_cnt) {
-      const int slot = o_off + k;
-      keys[slot] = make_key(tile0 + (q.ry0 + ty) * grid_x + (q.rx0 + tx), q.depth);
-      vals[slot] = (uint32_t)o_gid;
-    }
+    sink.step(ok, m, (q.ry0 + ty) * grid_x + (q.rx0 + tx), o,
+              o_done + popc64(m & below & ~lanes_below(seg_lo)));
     // Each owner adds the passing tasks of its own segment in this step.
     if (is_frag) {
       const int lo_l = imax(excl - t0, 0), hi_l = imin(incl - t0, kWave);
@@ -215,6 +233,19 @@ __device__ int sweep_fragmented(const Cand& me, bool is_frag, int lane, int gid,
     }
   }
   return mine;
+}
+
+// All of a wave's (Gaussian, tile) hits, in the three routes: single-tile
+// rects (tested by their own lane), massive and fragmented.  Returns this
+// lane's count.  `single_ok` is this lane's result for a single-tile rect.
+template <class Sink>
+__device__ __forceinline__ int sweep_all(const Cand& me, int area, bool single_ok, int lane,
+                                         int bx, int by, int grid_x, Sink& sink) {
+  sink.step(single_ok, ballot(single_ok), me.ry0 * grid_x + me.rx0, lane, 0);
+  int count = single_ok ? 1 : 0;
+  count += sweep_massive(me, area >= kMassive, lane, bx, by, grid_x, sink);
+  count += sweep_fragmented(me, area >= 2 && area < kMassive, lane, bx, by, grid_x, sink);
+  return count;
 }
 
 __device__ __forceinline__ void load_cov(const float* cov3, int g, float c[6]) {
@@ -257,6 +288,51 @@ __device__ __forceinline__ uint16_t to_half_bits(float f, int prec) {
   return prec == kFeatBF16 ? f32_to_bf16(f) : __half_as_ushort(__float2half_rn(f));
 }
 
+// Projects Gaussian g of frame F (no Gaussian when g >= P).
+__device__ __forceinline__ Proj project_gaussian(const FrameDev& F, int g, float& px, float& py,
+                                                 float& pz) {
+  px = py = pz = 0.0f;
+  if (g >= F.P) return no_proj();
+  px = F.pos[3 * (size_t)g + 0];
+  py = F.pos[3 * (size_t)g + 1];
+  pz = F.pos[3 * (size_t)g + 2];
+  float c[6];
+  load_cov(F.cov3, g, c);
+  return project(px, py, pz, F.opac[g], c, F.cp);
+}
+
+// Render features of an emitting Gaussian: SH colour, scaled conic, and the
+// FP32 arrays and/or the 16-bit record, whichever the frame asks for.
+template <int SHP>
+__device__ __forceinline__ void write_features(const FrameDev& F, int g, const Proj& pj, float px,
+                                               float py, float pz) {
+  const float4 con = make_float4((-0.5f * kLog2e) * pj.con_a, -kLog2e * pj.con_b,
+                                 (-0.5f * kLog2e) * pj.con_c, pj.log2_opacity);
+  float sh[48];
+  load_sh<SHP>(F.shs, g, sh);
+  const F3 rgb = sh_color(px, py, pz, F.cp.campos, sh);
+  if (F.xy) F.xy[g] = make_float2(pj.xy_x, pj.xy_y);
+  if (F.conic) F.conic[g] = con;
+  if (F.rgbd) F.rgbd[g] = make_float4(rgb.x, rgb.y, rgb.z, pj.depth);
+  if (F.rec) {
+    FeatRec16 r;
+    r.x = pj.xy_x;
+    r.y = pj.xy_y;
+    r.w = con.w;
+    r.ca = to_half_bits(con.x * kConicScale, F.rec_prec);
+    r.cb = to_half_bits(con.y * kConicScale, F.rec_prec);
+    r.cc = to_half_bits(con.z * kConicScale, F.rec_prec);
+    r.r = to_half_bits(rgb.x, F.rec_prec);
+    r.g = to_half_bits(rgb.y, F.rec_prec);
+    r.b = to_half_bits(rgb.z, F.rec_prec);
+    F.rec[g] = r;
+  }
+}
+
+__device__ __forceinline__ int rect_area(const Proj& pj) {
+  return pj.valid ? (pj.rmax_x - pj.rmin_x) * (pj.rmax_y - pj.rmin_y) : 0;
+}
+
 // Count pass for the workgroup `vb` of a launch (vb - F.block0 is the
 // frame-local workgroup).  counts/block_sums are indexed by launch position.
 template <int SHP>
@@ -266,49 +342,15 @@ __device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* _
   const int tid = threadIdx.x, lane = tid & 63;
   const int g = (vb - F.block0) * kPreThreads + tid;
   const CamParams& cp = F.cp;
-  Proj pj;
-  pj.valid = false;
-  float px = 0.f, py = 0.f, pz = 0.f;
-  if (g < F.P) {
-    px = F.pos[3 * (size_t)g + 0];
-    py = F.pos[3 * (size_t)g + 1];
-    pz = F.pos[3 * (size_t)g + 2];
-    float c[6];
-    load_cov(F.cov3, g, c);
-    pj = project(px, py, pz, F.opac[g], c, cp);
-  }
-  const int area = pj.valid ? (pj.rmax_x - pj.rmin_x) * (pj.rmax_y - pj.rmin_y) : 0;
-  int count = 0;
-  if (area == 1) count = tile_hits(pj.rmin_x, pj.rmin_y, pj, cp.block_x, cp.block_y) ? 1 : 0;
-  const Cand me = cand_from(pj);
-  count += sweep_massive<false>(me, area >= kMassive, lane, g, 0, 0, cp.block_x, cp.block_y,
-                                cp.grid_x, 0, nullptr, nullptr);
-  count += sweep_fragmented<false>(me, area >= 2 && area < kMassive, lane, g, 0, 0, cp.block_x,
-                                   cp.block_y, cp.grid_x, 0, nullptr, nullptr);
+  float px, py, pz;
+  const Proj pj = project_gaussian(F, g, px, py, pz);
+  const int area = rect_area(pj);
+  const bool single_ok = area == 1 && tile_hits(pj.rmin_x, pj.rmin_y, pj, cp.block_x, cp.block_y);
+  NullSink none;
+  const int count =
+      sweep_all(cand_from(pj), area, single_ok, lane, cp.block_x, cp.block_y, cp.grid_x, none);
   counts[(size_t)vb * kPreThreads + tid] = count;
-  if (count > 0) {
-    const float4 con = make_float4((-0.5f * kLog2e) * pj.con_a, -kLog2e * pj.con_b,
-                                   (-0.5f * kLog2e) * pj.con_c, pj.log2_opacity);
-    float sh[48];
-    load_sh<SHP>(F.shs, g, sh);
-    const F3 rgb = sh_color(px, py, pz, cp.campos, sh);
-    if (F.xy) F.xy[g] = make_float2(pj.xy_x, pj.xy_y);
-    if (F.conic) F.conic[g] = con;
-    if (F.rgbd) F.rgbd[g] = make_float4(rgb.x, rgb.y, rgb.z, pj.depth);
-    if (F.rec) {
-      FeatRec16 r;
-      r.x = pj.xy_x;
-      r.y = pj.xy_y;
-      r.w = con.w;
-      r.ca = to_half_bits(con.x * kConicScale, F.rec_prec);
-      r.cb = to_half_bits(con.y * kConicScale, F.rec_prec);
-      r.cc = to_half_bits(con.z * kConicScale, F.rec_prec);
-      r.r = to_half_bits(rgb.x, F.rec_prec);
-      r.g = to_half_bits(rgb.y, F.rec_prec);
-      r.b = to_half_bits(rgb.z, F.rec_prec);
-      F.rec[g] = r;
-    }
-  }
+  if (count > 0) write_features<SHP>(F, g, pj, px, py, pz);
   int total;
   block256_excl_scan(count, scratch, total);
   if (tid == 0) block_sums[vb] = total;
@@ -334,7 +376,11 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_count_batch(const FrameDev
 
 // state[0] = offset the frame's keys start at, state[1] = pair count for the
 // later stages (0 after overflow), state[2] = status bits (bit 0: overflow).
-constexpr int kStateBase = 0, kStateCount = 1, kStateFlags = 2, kStateWords = 64;
+// state[3] = workgroup ticket of the fused kernel (cleared before each launch).
+// Status bit 1: a workgroup of the fused kernel gave up waiting for its
+// predecessors' prefix (the frame's keys are then not valid).
+constexpr int kStateBase = 0, kStateCount = 1, kStateFlags = 2, kStateTicket = 3,
+              kStateWords = 64;
 
 __global__ __launch_bounds__(1024) void ps_pre_scan(int nblocks, const int* __restrict__ block_sums,
                                                     int* __restrict__ block_offsets,
@@ -386,7 +432,8 @@ __global__ __launch_bounds__(1024) void ps_pre_scan(int nblocks, const int* __re
   if (tid == 0) {
     const int base = reset_offset ? 0 : *curr_offset;
     const long long end = (long long)base + (long long)total;
-    const bool overflow = end > (long long)capacity || end > 0x7fffffffLL;
+    // A negative starting offset cannot be written at: reported as overflow.
+    const bool overflow = base < 0 || end > (long long)capacity || end > 0x7fffffffLL;
     *curr_offset = (int)(end > 0x7fffffffLL ? 0x7fffffff : end);
     state[kStateBase] = base;
     state[kStateCount] = overflow ? 0 : (int)end;
@@ -422,24 +469,12 @@ __device__ __forceinline__ void pre_emit_body(const FrameDev& F, int vb,
   if (total == 0) return;  // uniform across the block
   const CamParams& cp = F.cp;
   const int off = state[kStateBase] + block_offsets[vb] + excl;
-  Proj pj;
-  pj.valid = false;
-  if (count > 0) {
-    float c[6];
-    load_cov(F.cov3, g, c);
-    pj = project(F.pos[3 * (size_t)g], F.pos[3 * (size_t)g + 1], F.pos[3 * (size_t)g + 2],
-                 F.opac[g], c, cp);
-  }
-  const int area = (count > 0 && pj.valid) ? (pj.rmax_x - pj.rmin_x) * (pj.rmax_y - pj.rmin_y) : 0;
-  if (area == 1) {
-    keys[off] = make_key(F.tile0 + pj.rmin_y * cp.grid_x + pj.rmin_x, pj.depth);
-    vals[off] = (uint32_t)g;
-  }
+  float px, py, pz;
+  const Proj pj = count > 0 ? project_gaussian(F, g, px, py, pz) : no_proj();
+  const int area = count > 0 ? rect_area(pj) : 0;
   const Cand me = cand_from(pj);
-  sweep_massive<true>(me, area >= kMassive, lane, g, off, count, cp.block_x, cp.block_y,
-                      cp.grid_x, F.tile0, keys, vals);
-  sweep_fragmented<true>(me, area >= 2 && area < kMassive, lane, g, off, count, cp.block_x,
-                         cp.block_y, cp.grid_x, F.tile0, keys, vals);
+  GlobalSink sink{off, count, g, F.tile0, pj.depth, keys, vals};
+  sweep_all(me, area, area == 1, lane, cp.block_x, cp.block_y, cp.grid_x, sink);
 }
 
 __global__ __launch_bounds__(kPreThreads) void ps_pre_emit(FrameDev F, const int* __restrict__ counts,
@@ -458,11 +493,190 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_emit_batch(
   pre_emit_body(frames[f], blockIdx.x, counts, block_offsets, state, keys, vals);
 }
 
+// ------------------------------------------------------------ fused emission
+// One launch for count, scan and emit.  Each workgroup takes the next ticket
+// (so every workgroup it waits for is already running), projects its 256
+// Gaussians, writes their features and runs the tile tests once, keeping the
+// hits of each wave in LDS as (tile, owner lane | k << 6).  Its pair total
+// goes out as an aggregate; a decoupled look-back over the predecessors'
+// status words (aggregate or inclusive prefix, flag in the top two bits)
+// gives the workgroup's first slot, and the kept hits are written there in
+// canonical slots: excl(owner) + k.  A wave whose hits do not fit in LDS
+// runs its sweeps again, writing straight to global memory.  Pairs are
+// written only by workgroups whose whole range fits below `capacity`; the
+// last workgroup publishes the total, the count for the later stages and
+// the overflow bit.  Status words and the ticket are cleared by
+// ps_pre_fused_reset before each launch.
+constexpr int kHitCap = 512;  // LDS hit slots per wave
+constexpr uint64_t kLbAggregate = 1ull << 62, kLbInclusive = 2ull << 62;
+constexpr uint64_t kLbValue = (1ull << 62) - 1ull;
+// Polls of one look-back window before a workgroup gives up (status bit 1).
+// Every predecessor publishes within its own run time, so this bound is
+// reached only if the kernel is broken; it keeps such a kernel finite.
+constexpr int kLbSpinLimit = 1 << 20;
+
+struct LdsSink {
+  uint2* hits;
+  int n;  // hits seen by the wave (wave-uniform); more than kHitCap: spilled
+  uint64_t below;
+  __device__ __forceinline__ void step(bool ok, uint64_t m, int tile, int owner, int k) {
+    const int idx = n + popc64(m & below);
+    if (ok && idx < kHitCap)
+      hits[idx] = make_uint2((unsigned)tile, (unsigned)owner | ((unsigned)k << 6));
+    n += popc64(m);
+    // k must fit in 26 bits of the packed entry; otherwise the wave re-sweeps.
+    if (ballot(ok && k >= (1 << 26))) n = kHitCap + 1 + n;
+  }
+};
+
+__device__ __forceinline__ long long wave_sum_i64(long long v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const int lo = __shfl_xor((int)(uint32_t)(uint64_t)v, d);
+    const int hi = __shfl_xor((int)(uint32_t)((uint64_t)v >> 32), d);
+    v += (long long)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+  }
+  return v;
+}
+
+// Sum of the predecessors' totals of workgroup vb (>= 1), from their status
+// words, 64 predecessors per window.  Whole wave.
+__device__ long long look_back(const unsigned long long* status, int vb, int lane,
+                               bool& gave_up) {
+  long long prefix = 0;
+  int pos = vb - 1;
+  int polls = 0;
+  while (true) {
+    const int idx = pos - lane;
+    const unsigned long long w =
+        idx >= 0 ? __hip_atomic_load(status + idx, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+                 : kLbInclusive;
+    const unsigned flag = (unsigned)(w >> 62);
+    const uint64_t inc = ballot(flag == 2u);
+    const uint64_t none = ballot(flag == 0u);
+    const int first = inc ? __builtin_ctzll(inc) : 64;
+    const uint64_t need = inc ? ((2ull << first) - 1ull) : ~0ull;  // lanes 0..first
+    if (none & need) {
+      if (++polls >= kLbSpinLimit) {
+        gave_up = true;
+        return prefix;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    prefix += wave_sum_i64(lane <= first ? (long long)(w & kLbValue) : 0ll);
+    if (first < 64) return prefix;
+    pos -= kWave;
+    polls = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void ps_pre_fused_reset(unsigned long long* __restrict__ status,
+                                                          int nb, int* __restrict__ state) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nb) status[i] = 0ull;
+  if (i == 0) {
+    state[kStateTicket] = 0;
+    state[kStateFlags] = 0;
+  }
+}
+
+template <int SHP>
+__global__ __launch_bounds__(kPreThreads) void ps_pre_fused(
+    FrameDev F, int nb, unsigned long long* __restrict__ status, int* __restrict__ state,
+    int* __restrict__ curr_offset, int capacity, int reset_offset, uint64_t* __restrict__ keys,
+    uint32_t* __restrict__ vals) {
+  __shared__ uint2 s_hits[kPreThreads / kWave][kHitCap];
+  __shared__ int scratch[4];
+  __shared__ int s_vb;
+  __shared__ long long s_start;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) s_vb = atomicAdd(&state[kStateTicket], 1);
+  __syncthreads();
+  const int vb = s_vb;
+  const int g = vb * kPreThreads + tid;
+  const CamParams& cp = F.cp;
+
+  // Count: project, tile tests (hits kept in LDS), features.
+  float px, py, pz;
+  const Proj pj = project_gaussian(F, g, px, py, pz);
+  const int area = rect_area(pj);
+  const bool single_ok = area == 1 && tile_hits(pj.rmin_x, pj.rmin_y, pj, cp.block_x, cp.block_y);
+  const Cand me = cand_from(pj);
+  LdsSink kept{s_hits[wave], 0, lanes_below(lane)};
+  const int count = sweep_all(me, area, single_ok, lane, cp.block_x, cp.block_y, cp.grid_x, kept);
+  if (count > 0) write_features<SHP>(F, g, pj, px, py, pz);
+  int total;
+  const int excl = block256_excl_scan(count, scratch, total);
+
+  // Scan: this workgroup's first slot.
+  if (wave == 0) {
+    bool gave_up = false;
+    long long start;
+    if (vb == 0) {
+      start = reset_offset ? 0 : (long long)*curr_offset;
+      // A negative starting offset cannot be written at: every workgroup
+      // then lies past capacity and the frame reports an overflow.
+      if (start < 0) start = (long long)capacity + 1;
+      if (lane == 0) {
+        state[kStateBase] = (int)start;
+        __hip_atomic_store(status, kLbInclusive | (uint64_t)(start + total), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+    } else {
+      if (lane == 0)
+        __hip_atomic_store(status + vb, kLbAggregate | (uint64_t)total, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      start = look_back(status, vb, lane, gave_up);
+      if (lane == 0) {
+        __hip_atomic_store(status + vb, kLbInclusive | ((uint64_t)(start + total) & kLbValue),
+                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (gave_up) atomicOr(&state[kStateFlags], 2);
+      }
+    }
+    if (lane == 0) {
+      s_start = start;
+      if (vb == nb - 1) {
+        const long long end = start + total;
+        const bool overflow = end > (long long)capacity || end > 0x7fffffffLL;
+        *curr_offset = (int)(end > 0x7fffffffLL ? 0x7fffffffLL : end);
+        state[kStateCount] = overflow ? 0 : (int)end;
+        if (overflow) atomicOr(&state[kStateFlags], 1);
+      }
+    }
+  }
+  __syncthreads();
+  const long long start = s_start;
+  if (total == 0 || start + total > (long long)capacity) return;  // uniform across the block
+
+  // Emit: the kept hits, or the sweeps again for a wave that spilled.
+  const int off = (int)start + excl;
+  const float depth = pj.depth;
+  if (kept.n <= kHitCap) {
+    for (int e0 = 0; e0 < kept.n; e0 += kWave) {
+      const int e = e0 + lane;
+      const uint2 h = e < kept.n ? s_hits[wave][e] : make_uint2(0u, (unsigned)lane);
+      const int owner = (int)(h.y & 63u), k = (int)(h.y >> 6);
+      const int o_off = permute(off, owner);
+      const float o_depth = permute(depth, owner);
+      if (e < kept.n) {
+        keys[o_off + k] = make_key((int)h.x, o_depth);
+        vals[o_off + k] = (uint32_t)(vb * kPreThreads + wave * kWave + owner);
+      }
+    }
+  } else {
+    GlobalSink sink{off, count, g, 0, depth, keys, vals};
+    sweep_all(me, count > 0 ? area : 0, count > 0 && area == 1, lane, cp.block_x, cp.block_y,
+              cp.grid_x, sink);
+  }
+}
+
 struct PreWs {
   int* state;
   int* counts;
   int* block_sums;
   int* block_offsets;
+  unsigned long long* status;  // look-back words of the fused kernel
 };
 
 // Workspace for `nb` preprocess workgroups (a single frame has
@@ -478,10 +692,21 @@ inline size_t pre_layout_blocks(int nb, char* base, PreWs* w) {
   off += align_up(b * sizeof(int), 256);
   if (w) w->block_offsets = reinterpret_cast<int*>(base + off);
   off += align_up(b * sizeof(int), 256);
+  if (w) w->status = reinterpret_cast<unsigned long long*>(base + off);
+  off += align_up(b * sizeof(unsigned long long), 256);
   return off;
 }
 
 inline int blocks_for(int P) { return (P + kPreThreads - 1) / kPreThreads; }
+
+// Key emission in one launch (count, scan and emit fused) unless
+// POLYSPLAT_EMIT=3pass selects the three-launch form.  Both give the same
+// keys; they differ only in what an overflow leaves behind (the three-launch
+// form writes no pair at all).
+inline bool fused_emission() {
+  const char* e = getenv("POLYSPLAT_EMIT");
+  return !(e && e[0] == '3');
+}
 
 inline size_t pre_layout(int P, char* base, PreWs* w) {
   return pre_layout_blocks(blocks_for(P), base, w);
@@ -562,6 +787,24 @@ int preprocess_launch(int P, const float* positions, const void* shs, int sh_pre
   const FrameDev F = make_frame_dev(P, positions, shs, opacities, cov3d, cam, points_xy, rgb_depth,
                                     conic_opacity, feat_records, feat_prec, nullptr, nullptr);
   const int nb = blocks_for(P);
+  if (nb == 0) {
+    // No Gaussians: the scan kernel alone publishes the (unchanged) offset.
+    hipLaunchKernelGGL(ps_pre_scan, dim3(1), dim3(1024), 0, stream, 0, w.block_sums,
+                       w.block_offsets, curr_offset, capacity, reset_offset ? 1 : 0, w.state,
+                       (const FrameDev*)nullptr, 0, (int*)nullptr);
+    PS_LAUNCH_CHECK();
+    return PS_OK;
+  }
+  if (fused_emission()) {
+    hipLaunchKernelGGL(ps_pre_fused_reset, dim3((nb + 255) / 256), dim3(256), 0, stream, w.status,
+                       nb, w.state);
+    PS_LAUNCH_CHECK();
+    return launch_count(sh_prec, nb, stream, [&](auto shp) {
+      hipLaunchKernelGGL(ps_pre_fused<decltype(shp)::value>, dim3(nb), dim3(kPreThreads), 0,
+                         stream, F, nb, w.status, w.state, curr_offset, capacity,
+                         reset_offset ? 1 : 0, keys, values);
+    });
+  }
   st = launch_count(sh_prec, nb, stream, [&](auto shp) {
     hipLaunchKernelGGL(ps_pre_count<decltype(shp)::value>, dim3(nb), dim3(kPreThreads), 0, stream,
                        F, w.counts, w.block_sums);
@@ -571,11 +814,9 @@ int preprocess_launch(int P, const float* positions, const void* shs, int sh_pre
                      w.block_offsets, curr_offset, capacity, reset_offset ? 1 : 0, w.state,
                      (const FrameDev*)nullptr, 0, (int*)nullptr);
   PS_LAUNCH_CHECK();
-  if (nb > 0) {
-    hipLaunchKernelGGL(ps_pre_emit, dim3(nb), dim3(kPreThreads), 0, stream, F, w.counts,
-                       w.block_offsets, w.state, keys, values);
-    PS_LAUNCH_CHECK();
-  }
+  hipLaunchKernelGGL(ps_pre_emit, dim3(nb), dim3(kPreThreads), 0, stream, F, w.counts,
+                     w.block_offsets, w.state, keys, values);
+  PS_LAUNCH_CHECK();
   return PS_OK;
 }
 

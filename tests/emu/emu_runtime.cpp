@@ -1,10 +1,17 @@
 // Work-item scheduler of the host kernel emulation (test infrastructure
-// only): each workgroup's work-items run as fibers (ucontext) on the calling
-// thread, switching at barriers; workgroups run one after another.
+// only): each workgroup's work-items run as fibers (ucontext) on one host
+// thread, switching at barriers.  With EMU_THREADS=n (n > 1) a launch runs on
+// n host threads that take workgroups in increasing order from a shared
+// counter, as the hardware dispatcher does, so workgroups overlap in time and
+// cross-workgroup protocols (tickets, look-back status words) meet real
+// concurrency; `__shared__` storage and the scheduler state are per thread.
 #include <hip/hip_runtime.h>
 
 #include <ucontext.h>
 
+#include <atomic>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #if defined(__SANITIZE_ADDRESS__)
@@ -19,11 +26,11 @@ namespace {
 
 constexpr size_t kStack = 256 * 1024;
 
-BlockSync g_sync;
+thread_local BlockSync g_sync;
 
 struct Fiber {
   ucontext_t uc;
-  std::vector<char> stack;
+  std::unique_ptr<char[]> stack;
   Ctx ctx;
   Bar* wait_bar = nullptr;
   uint64_t wait_gen = 0;
@@ -39,7 +46,7 @@ struct Sched {
   const void* main_stack_bottom = nullptr;
   size_t main_stack_size = 0;
 };
-Sched S;
+thread_local Sched S;
 
 void finish_switch_into_fiber() {
 #if EMU_ASAN
@@ -87,12 +94,65 @@ void switch_to(int i) {
   Fiber& f = S.fibers[i];
   ctx = f.ctx;
 #if EMU_ASAN
-  __sanitizer_start_switch_fiber(&S.main_fake_stack, f.stack.data(), f.stack.size());
+  __sanitizer_start_switch_fiber(&S.main_fake_stack, f.stack.get(), kStack);
   swapcontext(&S.main, &f.uc);
   __sanitizer_finish_switch_fiber(S.main_fake_stack, nullptr, nullptr);
 #else
   swapcontext(&S.main, &f.uc);
 #endif
+}
+
+// Runs workgroup (bx, by) to completion on the calling thread.
+void run_block(dim3 grid, dim3 block, unsigned bx, unsigned by) {
+  const unsigned n = block.x * block.y * block.z;
+  const unsigned nwaves = n / 64;
+  if (S.fibers.size() < n) S.fibers.resize(n);
+  for (unsigned i = 0; i < n; ++i)
+    if (!S.fibers[i].stack) S.fibers[i].stack.reset(new char[kStack]);
+  g_sync.block = Bar{(int)n, 0, 0};
+  for (unsigned w = 0; w < nwaves; ++w) {
+    g_sync.wave[w] = Bar{64, 0, 0};
+    for (int l = 0; l < 64; ++l) g_sync.alive[w][l] = true;
+  }
+  for (unsigned i = 0; i < n; ++i) {
+    Fiber& f = S.fibers[i];
+    f.ctx.tid = {i % block.x, (i / block.x) % block.y, 0};
+    f.ctx.bid = {bx, by, 0};
+    f.ctx.bdim = block;
+    f.ctx.gdim = grid;
+    f.ctx.lane = (int)(i & 63);
+    f.ctx.wave = (int)(i >> 6);
+    f.wait_bar = nullptr;
+    f.done = false;
+    getcontext(&f.uc);
+    f.uc.uc_stack.ss_sp = f.stack.get();
+    f.uc.uc_stack.ss_size = kStack;
+    f.uc.uc_link = nullptr;
+    makecontext(&f.uc, fiber_entry, 0);
+  }
+  unsigned live = n;
+  while (live > 0) {
+    bool progress = false;
+    for (unsigned i = 0; i < n; ++i) {
+      Fiber& f = S.fibers[i];
+      if (f.done) continue;
+      if (f.wait_bar && f.wait_bar->gen == f.wait_gen) continue;
+      f.wait_bar = nullptr;
+      switch_to((int)i);
+      progress = true;
+      if (f.done) --live;
+    }
+    if (!progress) {
+      fprintf(stderr, "emu: deadlock in block (%u, %u): no work-item can proceed\n", bx, by);
+      abort();
+    }
+  }
+}
+
+int launch_threads() {
+  const char* e = getenv("EMU_THREADS");
+  const int t = e ? atoi(e) : 1;
+  return t < 1 ? 1 : (t > 64 ? 64 : t);
 }
 
 }  // namespace
@@ -112,61 +172,37 @@ void bar_wait(Bar* b) {
   yield_to_main(false);
 }
 
+void pause() { std::this_thread::yield(); }
+
 void run_grid(dim3 grid, dim3 block, const std::function<void()>& fn) {
   const unsigned n = block.x * block.y * block.z;
   if (n == 0 || n > (unsigned)kMaxThreads || (n % 64) != 0) {
     fprintf(stderr, "emu: unsupported block size %u\n", n);
     abort();
   }
-  const unsigned nwaves = n / 64;
-  if (S.fibers.size() < n) S.fibers.resize(n);
-  for (unsigned i = 0; i < n; ++i)
-    if (S.fibers[i].stack.empty()) S.fibers[i].stack.resize(kStack);
-  const std::function<void()>* saved_fn = S.fn;
-  S.fn = &fn;
-  for (unsigned by = 0; by < grid.y; ++by) {
-    for (unsigned bx = 0; bx < grid.x; ++bx) {
-      g_sync.block = Bar{(int)n, 0, 0};
-      for (unsigned w = 0; w < nwaves; ++w) {
-        g_sync.wave[w] = Bar{64, 0, 0};
-        for (int l = 0; l < 64; ++l) g_sync.alive[w][l] = true;
-      }
-      for (unsigned i = 0; i < n; ++i) {
-        Fiber& f = S.fibers[i];
-        f.ctx.tid = {i % block.x, (i / block.x) % block.y, 0};
-        f.ctx.bid = {bx, by, 0};
-        f.ctx.bdim = block;
-        f.ctx.gdim = grid;
-        f.ctx.lane = (int)(i & 63);
-        f.ctx.wave = (int)(i >> 6);
-        f.wait_bar = nullptr;
-        f.done = false;
-        getcontext(&f.uc);
-        f.uc.uc_stack.ss_sp = f.stack.data();
-        f.uc.uc_stack.ss_size = f.stack.size();
-        f.uc.uc_link = nullptr;
-        makecontext(&f.uc, fiber_entry, 0);
-      }
-      unsigned live = n;
-      while (live > 0) {
-        bool progress = false;
-        for (unsigned i = 0; i < n; ++i) {
-          Fiber& f = S.fibers[i];
-          if (f.done) continue;
-          if (f.wait_bar && f.wait_bar->gen == f.wait_gen) continue;
-          f.wait_bar = nullptr;
-          switch_to((int)i);
-          progress = true;
-          if (f.done) --live;
-        }
-        if (!progress) {
-          fprintf(stderr, "emu: deadlock in block (%u, %u): no work-item can proceed\n", bx, by);
-          abort();
-        }
-      }
-    }
+  const unsigned long long nblocks = (unsigned long long)grid.x * grid.y;
+  const int threads = launch_threads();
+  if (threads == 1 || nblocks <= 1) {
+    const std::function<void()>* saved_fn = S.fn;
+    S.fn = &fn;
+    for (unsigned by = 0; by < grid.y; ++by)
+      for (unsigned bx = 0; bx < grid.x; ++bx) run_block(grid, block, bx, by);
+    S.fn = saved_fn;
+    return;
   }
-  S.fn = saved_fn;
+  std::atomic<unsigned long long> next{0};
+  auto worker = [&]() {
+    S.fn = &fn;
+    for (;;) {
+      const unsigned long long b = next.fetch_add(1);
+      if (b >= nblocks) break;
+      run_block(grid, block, (unsigned)(b % grid.x), (unsigned)(b / grid.x));
+    }
+  };
+  std::vector<std::thread> pool;
+  const int t = (unsigned long long)threads < nblocks ? threads : (int)nblocks;
+  for (int i = 0; i < t; ++i) pool.emplace_back(worker);
+  for (auto& th : pool) th.join();
 }
 
 }  // namespace emu

@@ -11,9 +11,10 @@
 // it.  A work-item that returns drops out of both barriers and reads as an
 // inactive lane.
 //
-// __shared__ becomes function-scope static storage: only one workgroup is
-// alive at a time.  Device pointers are host pointers, so AddressSanitizer
-// sees every out-of-bounds access of a kernel.
+// __shared__ becomes function-scope static thread-local storage: a host
+// thread runs one workgroup at a time (several threads with EMU_THREADS).
+// Device pointers are host pointers, so AddressSanitizer sees every
+// out-of-bounds access of a kernel.
 #pragma once
 
 #include <math.h>
@@ -33,7 +34,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
-#define __shared__ static
+#define __shared__ static thread_local
 
 struct dim3 {
   unsigned x, y, z;
@@ -53,6 +54,7 @@ inline float2 make_float2(float a, float b) { return {a, b}; }
 inline float3 make_float3(float a, float b, float c) { return {a, b, c}; }
 inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
 inline int2 make_int2(int a, int b) { return {a, b}; }
+inline uint2 make_uint2(unsigned a, unsigned b) { return {a, b}; }
 
 namespace emu {
 
@@ -72,6 +74,7 @@ struct Bar {
   uint64_t gen = 0;
 };
 void bar_wait(Bar* b);  // blocks the calling fiber until every member arrived
+void pause();           // lets other host threads run (a polling wave)
 
 struct BlockSync {
   Bar block;
@@ -174,6 +177,23 @@ template <typename T>
 inline T atomicExch(T* p, T v) {
   return std::atomic_ref<T>(*p).exchange(v);
 }
+template <typename T>
+inline T atomicOr(T* p, T v) {
+  return std::atomic_ref<T>(*p).fetch_or(v);
+}
+
+// Scoped atomics, sequentially consistent on the host (stronger than the
+// orders the kernels ask for).
+#define __HIP_MEMORY_SCOPE_AGENT 3
+template <typename T>
+inline T __hip_atomic_load(const T* p, int, int) {
+  return std::atomic_ref<T>(*const_cast<T*>(p)).load();
+}
+template <typename T, typename V>
+inline void __hip_atomic_store(T* p, V v, int, int) {
+  std::atomic_ref<T>(*p).store((T)v);
+}
+inline void __builtin_amdgcn_s_sleep(int) { emu::pause(); }
 
 // ---------------------------------------------------------------- runtime
 typedef struct ihipStream_t* hipStream_t;

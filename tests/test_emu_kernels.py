@@ -152,3 +152,95 @@ def test_renderer_pads_partial_groups_neutrally(emu):
     ref2 = oracle_frame(sc, spec, sh_type="fp32", feat_prec="fp32", bg=(0.3, 0.1, 0.2),
                         check_every=2, fp64=False)
     assert np.abs(out.astype(int) - ref2["image"].astype(int)).max() <= 1
+
+
+# ------------------------------------------------------------------ key emission
+def _emission_scene():
+    # Mostly small splats plus a few that cover hundreds of tiles: 50 of the
+    # 312 waves keep more hits than their LDS list holds (they sweep again),
+    # and Gaussians with 64..512 hits keep indices above 63 in LDS.
+    sc = scenes.make_scene(20000, 6.0, [(0.9, -3.5, 0.5), (0.097, -2.0, 0.4), (0.003, 0.2, 0.3)],
+                           seed=41)
+    return sc, scenes.make_camera(640, 480, 6.0, distance=7.0)
+
+
+def _preprocess(L, sc, spec, capacity, extra=0, base=0, fill=-7):
+    """ps_preprocess into key arrays `extra` entries longer than capacity."""
+    P = sc.num_vertex
+    pos = np.ascontiguousarray(sc.position.numpy())
+    shs = np.ascontiguousarray(sc.shs.numpy())
+    op = np.ascontiguousarray(sc.opacity.numpy())
+    cov = np.ascontiguousarray(sc.cov3d.numpy())
+    xy = np.zeros((P, 2), np.float32)
+    rgbd = np.zeros((P, 4), np.float32)
+    con = np.zeros((P, 4), np.float32)
+    ku = np.full(capacity + extra, fill, np.int64).view(np.uint64)
+    vu = np.full(capacity + extra, fill, np.int32).view(np.uint32)
+    curr = np.array([base], np.int32)
+    nb = L.ps_preprocess_workspace_size(P)
+    ws = np.random.default_rng(2).integers(0, 256, nb, dtype=np.uint8)  # not cleared
+    cam = capi.camera(spec)
+    st = L.ps_preprocess(P, HF.ptr(pos), HF.ptr(shs), 0, HF.ptr(op), HF.ptr(cov),
+                         capi.C.byref(cam), HF.ptr(xy), HF.ptr(rgbd), HF.ptr(con), None, 1,
+                         HF.ptr(ku), HF.ptr(vu), capacity, HF.ptr(curr), HF.ptr(ws), nb, None)
+    status = int(ws[8:12].view(np.int32)[0])  # state word 2
+    count = int(ws[4:8].view(np.int32)[0])  # state word 1
+    return st, int(curr[0]), status, count, ku, vu, (xy, rgbd, con)
+
+
+@pytest.mark.parametrize("threads", ["1", "8"])
+def test_fused_emission_matches_oracle_and_three_pass(emu, threads, monkeypatch):
+    monkeypatch.setenv("EMU_THREADS", threads)
+    sc, spec = _emission_scene()
+    cam = O.camera_from_spec(spec)
+    ref = O.preprocess(sc.position.numpy(), sc.shs.numpy(), sc.opacity.numpy(),
+                       sc.cov3d.numpy(), cam)
+    M = ref["M"]
+    st, curr, status, count, ku, vu, feats = _preprocess(emu, sc, spec, M + 10, extra=5)
+    capi.check_with(emu, st)
+    assert (curr, status, count) == (M, 0, M)
+    np.testing.assert_array_equal(ku[:M], ref["keys"])
+    np.testing.assert_array_equal(vu[:M], ref["vals"])
+    assert (ku[M:].view(np.int64) == -7).all()
+    vis = ref["counts"] > 0
+    np.testing.assert_array_equal(feats[0][vis], ref["xy"][vis])
+    np.testing.assert_array_equal(feats[2][vis], ref["conic"][vis])
+    np.testing.assert_array_equal(feats[1][vis], ref["rgbd"][vis])
+    # three-launch form, and appending after existing pairs
+    monkeypatch.setenv("POLYSPLAT_EMIT", "3pass")
+    st, curr3, _, _, ku3, vu3, _ = _preprocess(emu, sc, spec, M + 10)
+    capi.check_with(emu, st)
+    np.testing.assert_array_equal(ku3[:M], ku[:M])
+    np.testing.assert_array_equal(vu3[:M], vu[:M])
+    monkeypatch.delenv("POLYSPLAT_EMIT")
+    st, curr2, status, count, ku2, vu2, _ = _preprocess(emu, sc, spec, M + 100, base=77)
+    capi.check_with(emu, st)
+    assert (curr2, status, count) == (M + 77, 0, M + 77)
+    np.testing.assert_array_equal(ku2[77:77 + M], ref["keys"])
+    assert (ku2[:77].view(np.int64) == -7).all()
+
+
+@pytest.mark.parametrize("threads", ["1", "8"])
+def test_fused_emission_overflow_writes_nothing_past_capacity(emu, threads, monkeypatch):
+    monkeypatch.setenv("EMU_THREADS", threads)
+    sc, spec = _emission_scene()
+    cam = O.camera_from_spec(spec)
+    ref = O.preprocess(sc.position.numpy(), sc.shs.numpy(), sc.opacity.numpy(),
+                       sc.cov3d.numpy(), cam)
+    M = ref["M"]
+    for cap in (M - 1, M // 2, 0):
+        st, curr, status, count, ku, vu, _ = _preprocess(emu, sc, spec, cap, extra=M)
+        capi.check_with(emu, st)
+        assert curr == M and status & 1 and count == 0, (cap, curr, status, count)
+        assert (ku[cap:].view(np.int64) == -7).all() and (vu[cap:].view(np.int32) == -7).all()
+        # what was written below capacity are whole workgroups' canonical pairs
+        done = ku[:cap].view(np.int64) != -7
+        np.testing.assert_array_equal(ku[:cap][done], ref["keys"][:cap][done])
+    # the three-launch form writes nothing at all
+    monkeypatch.setenv("POLYSPLAT_EMIT", "3pass")
+    st, curr, status, count, ku, vu, _ = _preprocess(emu, sc, spec, M - 1, extra=M)
+    assert curr == M and status & 1 and (ku.view(np.int64) == -7).all()
+    # a negative starting offset is an overflow, with nothing written
+    monkeypatch.delenv("POLYSPLAT_EMIT")
+    st, curr, status, count, ku, vu, _ = _preprocess(emu, sc, spec, M + 10, base=-5)
+    assert status & 1 and count == 0 and (ku.view(np.int64) == -7).all()

@@ -159,22 +159,30 @@ def test_forward_pipeline_matches_stages(case):
     np.testing.assert_array_equal(img1, img2)
 
 
-def test_overflow_is_reported_and_nothing_is_written(gpu):
+def test_overflow_is_reported_and_nothing_is_written_past_capacity(gpu):
     scene = scenes.make_config_scene("c1", seed=2, P=20_000)
     spec = scenes.config_camera("c1")
     ref = oracle_frame(scene, spec, fp64=False)
     M = ref["pre"]["M"]
-    g = Gpu(scene, spec, capacity=M - 1)
-    g.ku.fill_(-7)
+    g = Gpu(scene, spec, capacity=M + 4096)
     say("overflow", M)
-    assert g.preprocess() == M  # the count still reports what is needed
-    assert g.status_word() & 1
-    assert (g.ku == -7).all().item()
-    st, n, _ = g.forward(BG, capi.V_LITE)
-    assert st == capi.PS_ERR_OVERFLOW and n == M
+    for cap in (M - 1, M // 3):
+        g.cap = cap  # the key arrays stay M + 4096 long: the tail is a guard
+        g.ku.fill_(-7)
+        g.vu.fill_(-7)
+        assert g.preprocess() == M  # the count still reports what is needed
+        assert g.status_word() & 1
+        assert (g.ku[cap:] == -7).all().item() and (g.vu[cap:] == -7).all().item()
+        # below capacity: only whole workgroups' canonical pairs
+        ku = g.ku[:cap].cpu().numpy()
+        done = ku != -7
+        np.testing.assert_array_equal(ku[done].view(np.uint64), ref["pre"]["keys"][:cap][done])
+        st, n, _ = g.forward(BG, capi.V_LITE)
+        assert st == capi.PS_ERR_OVERFLOW and n == M
     # exactly enough room works
     g2 = Gpu(scene, spec, capacity=M)
     assert g2.preprocess() == M and g2.status_word() == 0
+    np.testing.assert_array_equal(g2.ku.cpu().numpy().view(np.uint64), ref["pre"]["keys"])
 
 
 def test_preprocess_appends_at_curr_offset(gpu):
