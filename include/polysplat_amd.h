@@ -84,9 +84,7 @@ int ps_camera_matrices(const ps_camera* cam, float view[16], float proj[16], flo
  * do not fit (or *curr_offset is negative), no pair is written at or past
  * `capacity` (pairs below it may be), *curr_offset still receives the
  * required total and bit 0 of the status word is set (the device pointer
- * from ps_preprocess_status_ptr).  Bit 1 of the status word reports that the
- * single-launch emission gave up waiting for a predecessor workgroup, which
- * a correct kernel never does.  workspace: ps_preprocess_workspace_size(P)
+ * from ps_preprocess_status_ptr).  workspace: ps_preprocess_workspace_size(P)
  * bytes; no part of it needs clearing. */
 size_t ps_preprocess_workspace_size(int P);
 int ps_preprocess(int P, const float* positions, const void* shs, int sh_prec,

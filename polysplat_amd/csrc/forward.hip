@@ -229,10 +229,6 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
     st = read_back(ptrs, 2, vals, stream);
     if (st != PS_OK) return st;
     if (num_rendered_out) *num_rendered_out = vals[0];
-    if (vals[1] & 2) {
-      set_last_error("key emission did not complete (a workgroup gave up waiting)");
-      return PS_ERR_HIP;
-    }
     if (vals[1] & 1) {
       set_last_error("%d pairs do not fit in %d", vals[0], fb->capacity);
       return PS_ERR_OVERFLOW;

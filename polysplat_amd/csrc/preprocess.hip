@@ -3,17 +3,20 @@
 //
 // The reference (preprocess.cu:313-641) emits keys with one global atomic
 // counter, so the unsorted order -- and the order of equal keys after the
-// stable sort -- changes from run to run.  Here emission is split in three
-// launches so that the key array comes out in a canonical order (Gaussian id,
-// then tile row-major) and the exact pair count is known on the device before
-// anything is written:
-//   1. ps_pre_count : project every Gaussian, count the tiles its ellipse
-//                     really touches, write its render features, and reduce
-//                     the counts per 256-Gaussian block;
-//   2. ps_pre_scan  : scan the block sums, publish the pair count, detect
-//                     overflow;
-//   3. ps_pre_emit  : re-project the Gaussians that emit (bit-identical, so
-//                     the same tiles pass), and write keys at their offsets.
+// stable sort -- changes from run to run.  Here the key array comes out in a
+// canonical order (Gaussian id, then tile row-major), with the exact pair
+// count known on the device before any key is written:
+//   1. ps_pre_count    : project every Gaussian, count the tiles its ellipse
+//                        really touches, write its render features and an
+//                        emit record (rect, depth bits, hit mask), and add the
+//                        256-Gaussian block's count into its chunk sum;
+//   2. ps_pre_emit_rec : each workgroup's offset from the chunk sums and the
+//                        block sums of its chunk; keys written from the hit
+//                        masks (a Gaussian whose rect is too large for a mask
+//                        is projected and swept again, bit-identically).
+// POLYSPLAT_EMIT=3pass selects the earlier form: count, one scan kernel over
+// the block sums, and an emit pass that projects and sweeps every emitting
+// Gaussian again.
 //
 // Tile enumeration inside a wave routes each Gaussian by its candidate-tile
 // count (the rect area):
@@ -44,6 +47,28 @@ namespace {
 
 constexpr int kPreThreads = 256;
 constexpr int kMassive = 64;
+
+// What the count pass leaves for the emit pass about a Gaussian that emits:
+// rect origin and width (x0w = x0 | width << 24; width 0 marks a rect of more
+// than kMaskTiles tiles, which the emit pass projects and sweeps again), the
+// depth bits of its keys, and the hit mask over the rect (bit t: tile t of
+// the rect in row-major order).
+struct alignas(16) EmitRec {
+  uint32_t x0w;
+  uint32_t y0;
+  uint32_t depth;
+  uint32_t mask;
+};
+// state[0] = offset the frame's keys start at, state[1] = pair count for the
+// later stages (0 after overflow), state[2] = status bits (bit 0: overflow).
+constexpr int kStateBase = 0, kStateCount = 1, kStateFlags = 2, kStateWords = 64;
+
+constexpr int kMaskTiles = 32;
+// The count pass adds each workgroup's pair total into the sum of its chunk
+// of kChunkBlocks workgroups, so an emit workgroup finds its offset from at
+// most chunks + kChunkBlocks sums.
+constexpr int kChunkBlocks = 256;
+inline int chunks_for(int nb) { return (nb + kChunkBlocks - 1) / kChunkBlocks; }
 
 // A lane with no Gaussian (past P, culled, or emitting nothing).
 __device__ __forceinline__ Proj no_proj() {
@@ -192,7 +217,7 @@ __device__ int sweep_massive(const Cand& me, bool is_massive, int lane, int bx, 
 // whose inclusive prefix of areas exceeds t.  Returns this lane's own count.
 template <class Sink>
 __device__ int sweep_fragmented(const Cand& me, bool is_frag, int lane, int bx, int by,
-                                int grid_x, Sink& sink) {
+                                int grid_x, Sink& sink, uint32_t& mask) {
   const int area = is_frag ? (me.rx1 - me.rx0) * (me.ry1 - me.ry0) : 0;
   const int incl = wave_incl_scan(area, lane);
   const int excl = incl - area;
@@ -226,10 +251,16 @@ __device__ int sweep_fragmented(const Cand& me, bool is_frag, int lane, int bx, 
     const int seg_lo = imax(o_excl - t0, 0);
     sink.step(ok, m, (q.ry0 + ty) * grid_x + (q.rx0 + tx), o,
               o_done + popc64(m & below & ~lanes_below(seg_lo)));
-    // Each owner adds the passing tasks of its own segment in this step.
+    // Each owner adds the passing tasks of its own segment in this step, and
+    // their bits to its mask (rect-local index of lane j: t0 + j - excl).
     if (is_frag) {
       const int lo_l = imax(excl - t0, 0), hi_l = imin(incl - t0, kWave);
-      if (lo_l < hi_l) mine += popc64(m & lane_range(lo_l, hi_l));
+      if (lo_l < hi_l) {
+        const uint64_t seg = m & lane_range(lo_l, hi_l);
+        mine += popc64(seg);
+        const int first_local = t0 + lo_l - excl;
+        if (first_local < kMaskTiles) mask |= (uint32_t)((seg >> lo_l) << first_local);
+      }
     }
   }
   return mine;
@@ -237,14 +268,17 @@ __device__ int sweep_fragmented(const Cand& me, bool is_frag, int lane, int bx, 
 
 // All of a wave's (Gaussian, tile) hits, in the three routes: single-tile
 // rects (tested by their own lane), massive and fragmented.  Returns this
-// lane's count.  `single_ok` is this lane's result for a single-tile rect.
+// lane's count and, for rects of at most kMaskTiles tiles, its hit mask.
+// `single_ok` is this lane's result for a single-tile rect.
 template <class Sink>
 __device__ __forceinline__ int sweep_all(const Cand& me, int area, bool single_ok, int lane,
-                                         int bx, int by, int grid_x, Sink& sink) {
+                                         int bx, int by, int grid_x, Sink& sink,
+                                         uint32_t& mask) {
   sink.step(single_ok, ballot(single_ok), me.ry0 * grid_x + me.rx0, lane, 0);
   int count = single_ok ? 1 : 0;
+  mask = single_ok ? 1u : 0u;
   count += sweep_massive(me, area >= kMassive, lane, bx, by, grid_x, sink);
-  count += sweep_fragmented(me, area >= 2 && area < kMassive, lane, bx, by, grid_x, sink);
+  count += sweep_fragmented(me, area >= 2 && area < kMassive, lane, bx, by, grid_x, sink, mask);
   return count;
 }
 
@@ -335,9 +369,17 @@ __device__ __forceinline__ int rect_area(const Proj& pj) {
 
 // Count pass for the workgroup `vb` of a launch (vb - F.block0 is the
 // frame-local workgroup).  counts/block_sums are indexed by launch position.
+// With `recs`, also leaves each emitting Gaussian's EmitRec, adds the
+// workgroup's total into its chunk sum and (workgroup 0) records the
+// frame's starting offset in state[kStateBase].
 template <int SHP>
 __device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* __restrict__ counts,
-                                               int* __restrict__ block_sums) {
+                                               int* __restrict__ block_sums,
+                                               EmitRec* __restrict__ recs,
+                                               unsigned long long* __restrict__ chunk_sums,
+                                               int* __restrict__ state,
+                                               const int* __restrict__ curr_offset,
+                                               int reset_offset) {
   __shared__ int scratch[4];
   const int tid = threadIdx.x, lane = tid & 63;
   const int g = (vb - F.block0) * kPreThreads + tid;
@@ -347,19 +389,38 @@ __device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* _
   const int area = rect_area(pj);
   const bool single_ok = area == 1 && tile_hits(pj.rmin_x, pj.rmin_y, pj, cp.block_x, cp.block_y);
   NullSink none;
-  const int count =
-      sweep_all(cand_from(pj), area, single_ok, lane, cp.block_x, cp.block_y, cp.grid_x, none);
+  uint32_t mask;
+  const int count = sweep_all(cand_from(pj), area, single_ok, lane, cp.block_x, cp.block_y,
+                              cp.grid_x, none, mask);
   counts[(size_t)vb * kPreThreads + tid] = count;
-  if (count > 0) write_features<SHP>(F, g, pj, px, py, pz);
+  if (count > 0) {
+    write_features<SHP>(F, g, pj, px, py, pz);
+    if (recs) {
+      const uint32_t w = area <= kMaskTiles ? (uint32_t)(pj.rmax_x - pj.rmin_x) : 0u;
+      EmitRec r;
+      r.x0w = (uint32_t)pj.rmin_x | (w << 24);
+      r.y0 = (uint32_t)pj.rmin_y;
+      r.depth = f2u(pj.depth);
+      r.mask = mask;
+      recs[(size_t)vb * kPreThreads + tid] = r;
+    }
+  }
   int total;
   block256_excl_scan(count, scratch, total);
-  if (tid == 0) block_sums[vb] = total;
+  if (tid == 0) {
+    block_sums[vb] = total;
+    if (chunk_sums) atomicAdd(&chunk_sums[vb / kChunkBlocks], (unsigned long long)total);
+    if (recs && vb == 0) state[kStateBase] = reset_offset ? 0 : *curr_offset;
+  }
 }
 
 template <int SHP>
-__global__ __launch_bounds__(kPreThreads) void ps_pre_count(FrameDev F, int* __restrict__ counts,
-                                                            int* __restrict__ block_sums) {
-  pre_count_body<SHP>(F, blockIdx.x, counts, block_sums);
+__global__ __launch_bounds__(kPreThreads) void ps_pre_count(
+    FrameDev F, int* __restrict__ counts, int* __restrict__ block_sums,
+    EmitRec* __restrict__ recs, unsigned long long* __restrict__ chunk_sums,
+    int* __restrict__ state, const int* __restrict__ curr_offset, int reset_offset) {
+  pre_count_body<SHP>(F, blockIdx.x, counts, block_sums, recs, chunk_sums, state, curr_offset,
+                      reset_offset);
 }
 
 __device__ __forceinline__ int frame_of_block(const FrameDev* frames, int nf, int vb) {
@@ -371,16 +432,10 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_count_batch(const FrameDev
                                                                   int nf, int* __restrict__ counts,
                                                                   int* __restrict__ block_sums) {
   const int f = frame_of_block(frames, nf, blockIdx.x);
-  pre_count_body<SHP>(frames[f], blockIdx.x, counts, block_sums);
+  pre_count_body<SHP>(frames[f], blockIdx.x, counts, block_sums, nullptr, nullptr, nullptr,
+                      nullptr, 0);
 }
 
-// state[0] = offset the frame's keys start at, state[1] = pair count for the
-// later stages (0 after overflow), state[2] = status bits (bit 0: overflow).
-// state[3] = workgroup ticket of the fused kernel (cleared before each launch).
-// Status bit 1: a workgroup of the fused kernel gave up waiting for its
-// predecessors' prefix (the frame's keys are then not valid).
-constexpr int kStateBase = 0, kStateCount = 1, kStateFlags = 2, kStateTicket = 3,
-              kStateWords = 64;
 
 __global__ __launch_bounds__(1024) void ps_pre_scan(int nblocks, const int* __restrict__ block_sums,
                                                     int* __restrict__ block_offsets,
@@ -474,7 +529,8 @@ __device__ __forceinline__ void pre_emit_body(const FrameDev& F, int vb,
   const int area = count > 0 ? rect_area(pj) : 0;
   const Cand me = cand_from(pj);
   GlobalSink sink{off, count, g, F.tile0, pj.depth, keys, vals};
-  sweep_all(me, area, area == 1, lane, cp.block_x, cp.block_y, cp.grid_x, sink);
+  uint32_t mask;
+  sweep_all(me, area, area == 1, lane, cp.block_x, cp.block_y, cp.grid_x, sink, mask);
 }
 
 __global__ __launch_bounds__(kPreThreads) void ps_pre_emit(FrameDev F, const int* __restrict__ counts,
@@ -493,41 +549,12 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_emit_batch(
   pre_emit_body(frames[f], blockIdx.x, counts, block_offsets, state, keys, vals);
 }
 
-// ------------------------------------------------------------ fused emission
-// One launch for count, scan and emit.  Each workgroup takes the next ticket
-// (so every workgroup it waits for is already running), projects its 256
-// Gaussians, writes their features and runs the tile tests once, keeping the
-// hits of each wave in LDS as (tile, owner lane | k << 6).  Its pair total
-// goes out as an aggregate; a decoupled look-back over the predecessors'
-// status words (aggregate or inclusive prefix, flag in the top two bits)
-// gives the workgroup's first slot, and the kept hits are written there in
-// canonical slots: excl(owner) + k.  A wave whose hits do not fit in LDS
-// runs its sweeps again, writing straight to global memory.  Pairs are
-// written only by workgroups whose whole range fits below `capacity`; the
-// last workgroup publishes the total, the count for the later stages and
-// the overflow bit.  Status words and the ticket are cleared by
-// ps_pre_fused_reset before each launch.
-constexpr int kHitCap = 512;  // LDS hit slots per wave
-constexpr uint64_t kLbAggregate = 1ull << 62, kLbInclusive = 2ull << 62;
-constexpr uint64_t kLbValue = (1ull << 62) - 1ull;
-// Polls of one look-back window before a workgroup gives up (status bit 1).
-// Every predecessor publishes within its own run time, so this bound is
-// reached only if the kernel is broken; it keeps such a kernel finite.
-constexpr int kLbSpinLimit = 1 << 20;
-
-struct LdsSink {
-  uint2* hits;
-  int n;  // hits seen by the wave (wave-uniform); more than kHitCap: spilled
-  uint64_t below;
-  __device__ __forceinline__ void step(bool ok, uint64_t m, int tile, int owner, int k) {
-    const int idx = n + popc64(m & below);
-    if (ok && idx < kHitCap)
-      hits[idx] = make_uint2((unsigned)tile, (unsigned)owner | ((unsigned)k << 6));
-    n += popc64(m);
-    // k must fit in 26 bits of the packed entry; otherwise the wave re-sweeps.
-    if (ballot(ok && k >= (1 << 26))) n = kHitCap + 1 + n;
-  }
-};
+// ------------------------------------------------- emission from the records
+__global__ __launch_bounds__(256) void ps_pre_chunk_reset(unsigned long long* __restrict__ sums,
+                                                          int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) sums[i] = 0ull;
+}
 
 __device__ __forceinline__ long long wave_sum_i64(long long v) {
 #pragma unroll
@@ -539,136 +566,71 @@ __device__ __forceinline__ long long wave_sum_i64(long long v) {
   return v;
 }
 
-// Sum of the predecessors' totals of workgroup vb (>= 1), from their status
-// words, 64 predecessors per window.  Whole wave.
-__device__ long long look_back(const unsigned long long* status, int vb, int lane,
-                               bool& gave_up) {
-  long long prefix = 0;
-  int pos = vb - 1;
-  int polls = 0;
-  while (true) {
-    const int idx = pos - lane;
-    const unsigned long long w =
-        idx >= 0 ? __hip_atomic_load(status + idx, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
-                 : kLbInclusive;
-    const unsigned flag = (unsigned)(w >> 62);
-    const uint64_t inc = ballot(flag == 2u);
-    const uint64_t none = ballot(flag == 0u);
-    const int first = inc ? __builtin_ctzll(inc) : 64;
-    const uint64_t need = inc ? ((2ull << first) - 1ull) : ~0ull;  // lanes 0..first
-    if (none & need) {
-      if (++polls >= kLbSpinLimit) {
-        gave_up = true;
-        return prefix;
-      }
-      __builtin_amdgcn_s_sleep(1);
-      continue;
-    }
-    prefix += wave_sum_i64(lane <= first ? (long long)(w & kLbValue) : 0ll);
-    if (first < 64) return prefix;
-    pos -= kWave;
-    polls = 0;
-  }
-}
-
-__global__ __launch_bounds__(256) void ps_pre_fused_reset(unsigned long long* __restrict__ status,
-                                                          int nb, int* __restrict__ state) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < nb) status[i] = 0ull;
-  if (i == 0) {
-    state[kStateTicket] = 0;
-    state[kStateFlags] = 0;
-  }
-}
-
-template <int SHP>
-__global__ __launch_bounds__(kPreThreads) void ps_pre_fused(
-    FrameDev F, int nb, unsigned long long* __restrict__ status, int* __restrict__ state,
-    int* __restrict__ curr_offset, int capacity, int reset_offset, uint64_t* __restrict__ keys,
-    uint32_t* __restrict__ vals) {
-  __shared__ uint2 s_hits[kPreThreads / kWave][kHitCap];
+// Emit pass over the count pass's records.  Workgroup vb starts at the
+// frame's offset plus the chunk sums before its chunk plus the block sums of
+// its chunk before it; keys go to their canonical slots (excl + k).  A
+// Gaussian with a hit mask writes its keys from the mask; one whose rect was
+// too large for a mask is projected and swept again, as the three-launch
+// emit does.  Pairs are written only by workgroups whose whole range fits
+// below `capacity`.  The last workgroup publishes the total, the count for
+// the later stages and the overflow bit.
+__global__ __launch_bounds__(kPreThreads) void ps_pre_emit_rec(
+    FrameDev F, int nb, const int* __restrict__ counts, const int* __restrict__ block_sums,
+    const unsigned long long* __restrict__ chunk_sums, const EmitRec* __restrict__ recs,
+    int* __restrict__ state, int* __restrict__ curr_offset, int capacity,
+    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   __shared__ int scratch[4];
-  __shared__ int s_vb;
-  __shared__ long long s_start;
+  __shared__ long long s_part[kPreThreads / kWave];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (tid == 0) s_vb = atomicAdd(&state[kStateTicket], 1);
-  __syncthreads();
-  const int vb = s_vb;
+  const int vb = blockIdx.x, chunk = vb / kChunkBlocks;
+  long long part = 0;
+  for (int i = tid; i < chunk; i += kPreThreads) part += (long long)chunk_sums[i];
+  const int b = chunk * kChunkBlocks + tid;
+  if (b < vb) part += block_sums[b];
+  part = wave_sum_i64(part);
+  if (lane == 0) s_part[wave] = part;
   const int g = vb * kPreThreads + tid;
-  const CamParams& cp = F.cp;
-
-  // Count: project, tile tests (hits kept in LDS), features.
-  float px, py, pz;
-  const Proj pj = project_gaussian(F, g, px, py, pz);
-  const int area = rect_area(pj);
-  const bool single_ok = area == 1 && tile_hits(pj.rmin_x, pj.rmin_y, pj, cp.block_x, cp.block_y);
-  const Cand me = cand_from(pj);
-  LdsSink kept{s_hits[wave], 0, lanes_below(lane)};
-  const int count = sweep_all(me, area, single_ok, lane, cp.block_x, cp.block_y, cp.grid_x, kept);
-  if (count > 0) write_features<SHP>(F, g, pj, px, py, pz);
+  const int count = g < F.P ? counts[g] : 0;
   int total;
-  const int excl = block256_excl_scan(count, scratch, total);
-
-  // Scan: this workgroup's first slot.
-  if (wave == 0) {
-    bool gave_up = false;
-    long long start;
-    if (vb == 0) {
-      start = reset_offset ? 0 : (long long)*curr_offset;
-      // A negative starting offset cannot be written at: every workgroup
-      // then lies past capacity and the frame reports an overflow.
-      if (start < 0) start = (long long)capacity + 1;
-      if (lane == 0) {
-        state[kStateBase] = (int)start;
-        __hip_atomic_store(status, kLbInclusive | (uint64_t)(start + total), __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      }
-    } else {
-      if (lane == 0)
-        __hip_atomic_store(status + vb, kLbAggregate | (uint64_t)total, __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      start = look_back(status, vb, lane, gave_up);
-      if (lane == 0) {
-        __hip_atomic_store(status + vb, kLbInclusive | ((uint64_t)(start + total) & kLbValue),
-                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        if (gave_up) atomicOr(&state[kStateFlags], 2);
-      }
-    }
-    if (lane == 0) {
-      s_start = start;
-      if (vb == nb - 1) {
-        const long long end = start + total;
-        const bool overflow = end > (long long)capacity || end > 0x7fffffffLL;
-        *curr_offset = (int)(end > 0x7fffffffLL ? 0x7fffffffLL : end);
-        state[kStateCount] = overflow ? 0 : (int)end;
-        if (overflow) atomicOr(&state[kStateFlags], 1);
-      }
-    }
+  const int excl = block256_excl_scan(count, scratch, total);  // its barriers publish s_part
+  const long long base = state[kStateBase];
+  const long long start = base + s_part[0] + s_part[1] + s_part[2] + s_part[3];
+  // A negative starting offset cannot be written at: reported as overflow.
+  const bool writable = base >= 0 && start + total <= (long long)capacity;
+  if (vb == nb - 1 && tid == 0) {
+    const long long end = start + total;
+    const bool overflow = base < 0 || end > (long long)capacity || end > 0x7fffffffLL;
+    *curr_offset = (int)(end > 0x7fffffffLL ? 0x7fffffffLL : end);
+    state[kStateCount] = overflow ? 0 : (int)end;
+    state[kStateFlags] = overflow ? 1 : 0;
   }
-  __syncthreads();
-  const long long start = s_start;
-  if (total == 0 || start + total > (long long)capacity) return;  // uniform across the block
-
-  // Emit: the kept hits, or the sweeps again for a wave that spilled.
+  if (total == 0 || !writable) return;  // uniform across the block
   const int off = (int)start + excl;
-  const float depth = pj.depth;
-  if (kept.n <= kHitCap) {
-    for (int e0 = 0; e0 < kept.n; e0 += kWave) {
-      const int e = e0 + lane;
-      const uint2 h = e < kept.n ? s_hits[wave][e] : make_uint2(0u, (unsigned)lane);
-      const int owner = (int)(h.y & 63u), k = (int)(h.y >> 6);
-      const int o_off = permute(off, owner);
-      const float o_depth = permute(depth, owner);
-      if (e < kept.n) {
-        keys[o_off + k] = make_key((int)h.x, o_depth);
-        vals[o_off + k] = (uint32_t)(vb * kPreThreads + wave * kWave + owner);
-      }
+  const CamParams& cp = F.cp;
+  EmitRec r;
+  r.x0w = r.y0 = r.depth = r.mask = 0u;
+  if (count > 0) r = recs[g];
+  const int w = (int)(r.x0w >> 24);
+  if (count > 0 && w > 0) {
+    const int x0 = (int)(r.x0w & 0xffffffu), y0 = (int)r.y0;
+    uint32_t m = r.mask;
+    for (int k = 0; m != 0u && k < count; ++k) {
+      const int t = __builtin_ctz(m);
+      m &= m - 1u;
+      const int ty = t / w, tx = t - ty * w;
+      const uint32_t tile = (uint32_t)(F.tile0 + (y0 + ty) * cp.grid_x + (x0 + tx));
+      keys[off + k] = ((uint64_t)tile << 32) | (uint64_t)r.depth;
+      vals[off + k] = (uint32_t)g;
     }
-  } else {
-    GlobalSink sink{off, count, g, 0, depth, keys, vals};
-    sweep_all(me, count > 0 ? area : 0, count > 0 && area == 1, lane, cp.block_x, cp.block_y,
-              cp.grid_x, sink);
   }
+  const bool big = count > 0 && w == 0;
+  if (ballot(big) == 0) return;  // uniform across the wave
+  float px, py, pz;
+  const Proj pj = big ? project_gaussian(F, g, px, py, pz) : no_proj();
+  const int area = big ? rect_area(pj) : 0;
+  GlobalSink sink{off, count, g, F.tile0, pj.depth, keys, vals};
+  uint32_t mask;
+  sweep_all(cand_from(pj), area, false, lane, cp.block_x, cp.block_y, cp.grid_x, sink, mask);
 }
 
 struct PreWs {
@@ -676,7 +638,8 @@ struct PreWs {
   int* counts;
   int* block_sums;
   int* block_offsets;
-  unsigned long long* status;  // look-back words of the fused kernel
+  EmitRec* recs;                    // per Gaussian, written by the count pass
+  unsigned long long* chunk_sums;   // pairs per chunk of kChunkBlocks workgroups
 };
 
 // Workspace for `nb` preprocess workgroups (a single frame has
@@ -692,18 +655,20 @@ inline size_t pre_layout_blocks(int nb, char* base, PreWs* w) {
   off += align_up(b * sizeof(int), 256);
   if (w) w->block_offsets = reinterpret_cast<int*>(base + off);
   off += align_up(b * sizeof(int), 256);
-  if (w) w->status = reinterpret_cast<unsigned long long*>(base + off);
-  off += align_up(b * sizeof(unsigned long long), 256);
+  if (w) w->recs = reinterpret_cast<EmitRec*>(base + off);
+  off += align_up(b * kPreThreads * sizeof(EmitRec), 256);
+  if (w) w->chunk_sums = reinterpret_cast<unsigned long long*>(base + off);
+  off += align_up((size_t)chunks_for((int)b) * sizeof(unsigned long long), 256);
   return off;
 }
 
 inline int blocks_for(int P) { return (P + kPreThreads - 1) / kPreThreads; }
 
-// Key emission in one launch (count, scan and emit fused) unless
-// POLYSPLAT_EMIT=3pass selects the three-launch form.  Both give the same
-// keys; they differ only in what an overflow leaves behind (the three-launch
-// form writes no pair at all).
-inline bool fused_emission() {
+// Key emission from the count pass's records (two launches and a chunk
+// reset) unless POLYSPLAT_EMIT=3pass selects count, scan and a re-projecting
+// emit.  Both give the same keys; they differ only in what an overflow leaves
+// behind (the three-launch form writes no pair at all).
+inline bool record_emission() {
   const char* e = getenv("POLYSPLAT_EMIT");
   return !(e && e[0] == '3');
 }
@@ -758,6 +723,7 @@ static int check_frame_args(int P, const void* positions, const void* shs, const
   PS_REQUIRE(cam->block_x > 0 && cam->block_y > 0, "tile size must be positive");
   const long long tiles = (long long)tiles_x(cam) * tiles_y(cam);
   PS_REQUIRE(tiles < (1LL << 31), "too many tiles");
+  PS_REQUIRE(tiles_x(cam) < (1 << 24), "too many tile columns (emit records hold 24 bits)");
   PS_REQUIRE(P == 0 || (positions && shs && opacities && cov3d), "scene pointer is NULL");
   return PS_OK;
 }
@@ -795,19 +761,27 @@ int preprocess_launch(int P, const float* positions, const void* shs, int sh_pre
     PS_LAUNCH_CHECK();
     return PS_OK;
   }
-  if (fused_emission()) {
-    hipLaunchKernelGGL(ps_pre_fused_reset, dim3((nb + 255) / 256), dim3(256), 0, stream, w.status,
-                       nb, w.state);
+  if (record_emission()) {
+    const int nc = chunks_for(nb);
+    hipLaunchKernelGGL(ps_pre_chunk_reset, dim3((nc + 255) / 256), dim3(256), 0, stream,
+                       w.chunk_sums, nc);
     PS_LAUNCH_CHECK();
-    return launch_count(sh_prec, nb, stream, [&](auto shp) {
-      hipLaunchKernelGGL(ps_pre_fused<decltype(shp)::value>, dim3(nb), dim3(kPreThreads), 0,
-                         stream, F, nb, w.status, w.state, curr_offset, capacity,
-                         reset_offset ? 1 : 0, keys, values);
+    st = launch_count(sh_prec, nb, stream, [&](auto shp) {
+      hipLaunchKernelGGL(ps_pre_count<decltype(shp)::value>, dim3(nb), dim3(kPreThreads), 0,
+                         stream, F, w.counts, w.block_sums, w.recs, w.chunk_sums, w.state,
+                         curr_offset, reset_offset ? 1 : 0);
     });
+    if (st != PS_OK) return st;
+    hipLaunchKernelGGL(ps_pre_emit_rec, dim3(nb), dim3(kPreThreads), 0, stream, F, nb, w.counts,
+                       w.block_sums, (const unsigned long long*)w.chunk_sums,
+                       (const EmitRec*)w.recs, w.state, curr_offset, capacity, keys, values);
+    PS_LAUNCH_CHECK();
+    return PS_OK;
   }
   st = launch_count(sh_prec, nb, stream, [&](auto shp) {
     hipLaunchKernelGGL(ps_pre_count<decltype(shp)::value>, dim3(nb), dim3(kPreThreads), 0, stream,
-                       F, w.counts, w.block_sums);
+                       F, w.counts, w.block_sums, (EmitRec*)nullptr,
+                       (unsigned long long*)nullptr, w.state, curr_offset, 0);
   });
   if (st != PS_OK) return st;
   hipLaunchKernelGGL(ps_pre_scan, dim3(1), dim3(1024), 0, stream, nb, w.block_sums,

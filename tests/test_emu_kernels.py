@@ -156,9 +156,10 @@ def test_renderer_pads_partial_groups_neutrally(emu):
 
 # ------------------------------------------------------------------ key emission
 def _emission_scene():
-    # Mostly small splats plus a few that cover hundreds of tiles: 50 of the
-    # 312 waves keep more hits than their LDS list holds (they sweep again),
-    # and Gaussians with 64..512 hits keep indices above 63 in LDS.
+    # Mostly small splats plus a few that cover hundreds of tiles: most
+    # Gaussians leave a hit mask for the emit pass, the large ones (rects of
+    # more than 32 tiles, fragmented and massive) are projected and swept
+    # again there.
     sc = scenes.make_scene(20000, 6.0, [(0.9, -3.5, 0.5), (0.097, -2.0, 0.4), (0.003, 0.2, 0.3)],
                            seed=41)
     return sc, scenes.make_camera(640, 480, 6.0, distance=7.0)
@@ -189,7 +190,7 @@ def _preprocess(L, sc, spec, capacity, extra=0, base=0, fill=-7):
 
 
 @pytest.mark.parametrize("threads", ["1", "8"])
-def test_fused_emission_matches_oracle_and_three_pass(emu, threads, monkeypatch):
+def test_record_emission_matches_oracle_and_three_pass(emu, threads, monkeypatch):
     monkeypatch.setenv("EMU_THREADS", threads)
     sc, spec = _emission_scene()
     cam = O.camera_from_spec(spec)
@@ -221,7 +222,7 @@ def test_fused_emission_matches_oracle_and_three_pass(emu, threads, monkeypatch)
 
 
 @pytest.mark.parametrize("threads", ["1", "8"])
-def test_fused_emission_overflow_writes_nothing_past_capacity(emu, threads, monkeypatch):
+def test_record_emission_overflow_writes_nothing_past_capacity(emu, threads, monkeypatch):
     monkeypatch.setenv("EMU_THREADS", threads)
     sc, spec = _emission_scene()
     cam = O.camera_from_spec(spec)

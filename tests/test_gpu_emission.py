@@ -1,11 +1,11 @@
-"""Key emission on the GPU, through the C ABI: the single-launch form (count,
-look-back scan and emit fused) and the three-launch form against the oracle.
+"""Key emission on the GPU, through the C ABI: the record form (count pass
+leaves hit masks, emit pass writes from them) and the three-launch form
+(count, scan, re-projecting emit) against the oracle.
 
 The scene mixes small splats with a few that cover hundreds of tiles, so
-some waves keep more hits than their LDS list holds and sweep again, and
-some Gaussians keep hit indices above 63 in LDS.  Bars: keys, values and
-features bit-exact; the pair count and status word as the oracle implies;
-repeated launches on one workspace give the same keys.
+both the hit-mask route and the re-sweep of large rects run.  Bars: keys,
+values and features bit-exact; the pair count and status word as the oracle
+implies; repeated launches on one workspace give the same keys.
 """
 import numpy as np
 import pytest
@@ -35,7 +35,7 @@ def case(gpu):
     return sc, spec, ref
 
 
-@pytest.mark.parametrize("form", ["fused", "3pass"])
+@pytest.mark.parametrize("form", ["records", "3pass"])
 def test_emission_bit_exact(case, form, monkeypatch):
     if form == "3pass":
         monkeypatch.setenv("POLYSPLAT_EMIT", "3pass")
@@ -59,8 +59,7 @@ def test_emission_bit_exact(case, form, monkeypatch):
 
 
 def test_emission_large_scene_against_three_pass(gpu, monkeypatch):
-    # 1.5M Gaussians at 1920x1080: thousands of workgroups in flight, so the
-    # look-back meets aggregates as well as inclusive prefixes.
+    # 1.5M Gaussians at 1920x1080 (5860 workgroups, 23 chunk sums).
     sc = scenes.make_scene(1_500_000, 30.0,
                            [(0.9, -3.8, 0.5), (0.097, -2.2, 0.4), (0.003, -0.5, 0.3)], seed=5)
     spec = scenes.make_camera(1920, 1080, 30.0)
@@ -68,15 +67,15 @@ def test_emission_large_scene_against_three_pass(gpu, monkeypatch):
     say("large emission", g.P)
     M = g.preprocess()
     assert g.status_word() == 0
-    fused_k = g.ku[:M].clone()
-    fused_v = g.vu[:M].clone()
-    fused_rec = g.records.clone()
+    rec_k = g.ku[:M].clone()
+    rec_v = g.vu[:M].clone()
+    rec_feat = g.records.clone()
     M2 = g.preprocess()
-    assert M2 == M and torch.equal(g.ku[:M], fused_k) and torch.equal(g.vu[:M], fused_v)
+    assert M2 == M and torch.equal(g.ku[:M], rec_k) and torch.equal(g.vu[:M], rec_v)
     monkeypatch.setenv("POLYSPLAT_EMIT", "3pass")
     g.ku.zero_()
     g.records.zero_()
     assert g.preprocess() == M and g.status_word() == 0
-    assert torch.equal(g.ku[:M], fused_k) and torch.equal(g.vu[:M], fused_v)
-    assert torch.equal(g.records, fused_rec)
+    assert torch.equal(g.ku[:M], rec_k) and torch.equal(g.vu[:M], rec_v)
+    assert torch.equal(g.records, rec_feat)
     say("large emission pairs", M)
