@@ -394,7 +394,6 @@ __device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* _
                               cp.grid_x, none, mask);
   counts[(size_t)vb * kPreThreads + tid] = count;
   if (count > 0) {
-    write_features<SHP>(F, g, pj, px, py, pz);
     if (recs) {
       const uint32_t w = area <= kMaskTiles ? (uint32_t)(pj.rmax_x - pj.rmin_x) : 0u;
       EmitRec r;
@@ -404,6 +403,7 @@ __device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* _
       r.mask = mask;
       recs[(size_t)vb * kPreThreads + tid] = r;
     }
+    write_features<SHP>(F, g, pj, px, py, pz);
   }
   int total;
   block256_excl_scan(count, scratch, total);
@@ -591,6 +591,9 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_emit_rec(
   if (lane == 0) s_part[wave] = part;
   const int g = vb * kPreThreads + tid;
   const int count = g < F.P ? counts[g] : 0;
+  // The record is loaded before the scan (a lane that emits nothing ignores
+  // whatever its slot holds), so its latency overlaps the scan's.
+  const EmitRec r = g < F.P ? recs[g] : EmitRec{0u, 0u, 0u, 0u};
   int total;
   const int excl = block256_excl_scan(count, scratch, total);  // its barriers publish s_part
   const long long base = state[kStateBase];
@@ -607,10 +610,7 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_emit_rec(
   if (total == 0 || !writable) return;  // uniform across the block
   const int off = (int)start + excl;
   const CamParams& cp = F.cp;
-  EmitRec r;
-  r.x0w = r.y0 = r.depth = r.mask = 0u;
-  if (count > 0) r = recs[g];
-  const int w = (int)(r.x0w >> 24);
+  const int w = count > 0 ? (int)(r.x0w >> 24) : 0;
   if (count > 0 && w > 0) {
     const int x0 = (int)(r.x0w & 0xffffffu), y0 = (int)r.y0;
     uint32_t m = r.mask;
