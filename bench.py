@@ -70,7 +70,8 @@ def copy_probe():
 def serve(args):
     """Serving stream: 4 resident scenes (1M, 3M, 10M, 30M Gaussians, FP16 SH
     and features), Poisson requests at --serve-rate with orbit poses and
-    resolutions drawn from 1280x720 .. 7680x4320, continuous batching."""
+    resolutions drawn from 1280x720 .. 7680x4320, admitted by buffer memory
+    and run in arrival order on one stream (frame after frame)."""
     import torch
 
     from polysplat_amd import serving as SV
@@ -79,7 +80,7 @@ def serve(args):
     sc = SV.make_resident_scenes(SV.SCENE_SIZES, device="cuda:0")
     torch.cuda.synchronize()
     log(f"serving: {len(sc)} resident scenes generated in {time.time() - t:.1f}s")
-    ex = SV.GpuExecutor(sc, slots=2)
+    ex = SV.SequentialExecutor(sc)
     est = SV.Estimates()
     t = time.time()
     SV.calibrate(ex, est, len(sc))
@@ -95,7 +96,8 @@ def serve(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t
     res = SV.summarize(done, sched.rejected, wall, batches=loop.batches)
-    res["overflow_retries"] = loop.overflows
+    res["overflow_retries"] = loop.overflows + ex.grows
+    res["executor"] = "one stream, frames in arrival order, synchronous pair count"
     res["rate_req_s"] = args.serve_rate
     res["scenes"] = list(SV.SCENE_SIZES)
     res["resolutions"] = [f"{w}x{h}" for w, h in SV.RESOLUTIONS]

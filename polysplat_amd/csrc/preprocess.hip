@@ -393,18 +393,18 @@ __device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* _
   const int count = sweep_all(cand_from(pj), area, single_ok, lane, cp.block_x, cp.block_y,
                               cp.grid_x, none, mask);
   counts[(size_t)vb * kPreThreads + tid] = count;
-  if (count > 0) {
-    if (recs) {
-      const uint32_t w = area <= kMaskTiles ? (uint32_t)(pj.rmax_x - pj.rmin_x) : 0u;
-      EmitRec r;
-      r.x0w = (uint32_t)pj.rmin_x | (w << 24);
-      r.y0 = (uint32_t)pj.rmin_y;
-      r.depth = f2u(pj.depth);
-      r.mask = mask;
-      recs[(size_t)vb * kPreThreads + tid] = r;
-    }
-    write_features<SHP>(F, g, pj, px, py, pz);
+  if (recs && g < F.P) {
+    // Written for every Gaussian (zeros when it emits nothing) so that each
+    // wave stores whole cache lines.
+    const uint32_t w = area <= kMaskTiles ? (uint32_t)(pj.rmax_x - pj.rmin_x) : 0u;
+    EmitRec r;
+    r.x0w = count > 0 ? ((uint32_t)pj.rmin_x | (w << 24)) : 0u;
+    r.y0 = count > 0 ? (uint32_t)pj.rmin_y : 0u;
+    r.depth = count > 0 ? f2u(pj.depth) : 0u;
+    r.mask = count > 0 ? mask : 0u;
+    recs[(size_t)vb * kPreThreads + tid] = r;
   }
+  if (count > 0) write_features<SHP>(F, g, pj, px, py, pz);
   int total;
   block256_excl_scan(count, scratch, total);
   if (tid == 0) {

@@ -215,7 +215,11 @@ class InFlight:
 
 
 class GpuExecutor:
-    """Runs a group of admitted requests on one stream slot: each frame is
+    """(Not used by the benchmark or the tests: a GPU run of this executor
+    faulted the device once and the cause has not been found; the
+    benchmark serves with SequentialExecutor.)
+
+    Runs a group of admitted requests on one stream slot: each frame is
     one ``ops.forward_frame`` call (preprocess, sort with tile ranges,
     persistent render; the pair count stays on the device), launched back to
     back on the slot's HIP stream, so frames of different slots overlap on
@@ -314,6 +318,134 @@ class GpuExecutor:
 
     def idle(self) -> bool:
         return all(b is None for b in self.busy)
+
+
+class SequentialExecutor:
+    """Runs admitted groups on one stream, frame after frame, each frame one
+    synchronous ``ops.forward_frame`` call (its pair count is read back
+    before the call returns).  Buffers are allocated zeroed and reused:
+    key, value and sort buffers are shared by all frames and grow when a
+    frame needs more pairs (the frame is then rendered again), the feature
+    records and the forward workspace are kept per scene, tile ranges per
+    resolution.  A group is complete when ``launch`` returns; ``poll`` hands
+    it back with its completion time on the host clock."""
+
+    slots_n = 1
+
+    def __init__(self, scenes_dev, device="cuda:0", feat_prec: str = "fp16",
+                 radius: float = ORBIT_RADIUS, margin: float = 1.05):
+        import torch
+
+        from . import _C
+
+        self.torch = torch
+        self.ops = _C.ops
+        self.dev = torch.device(device)
+        self.scenes = scenes_dev
+        self.feat_prec = {"fp32": 0, "fp16": 1, "bf16": 2}[feat_prec]
+        self.radius = radius
+        self.margin = margin
+        self.busy = [None]
+        self.t0 = None
+        self._bid = 0
+        self.cap = 0
+        self.keys = None
+        self.per_scene = {}
+        self.per_res = {}
+        self.grows = 0
+        self._empty = torch.empty(0, dtype=torch.float32, device=self.dev)
+
+    def _zeros(self, n, dtype):
+        return self.torch.zeros(n, dtype=dtype, device=self.dev)
+
+    def _ensure_keys(self, cap: int):
+        if cap <= self.cap:
+            return
+        torch = self.torch
+        cap = int(cap)
+        self.keys = None  # release the old buffers first
+        self.keys = [self._zeros(cap, torch.int64), self._zeros(cap, torch.int32),
+                     self._zeros(cap, torch.int64), self._zeros(cap, torch.int32),
+                     self._zeros(self.ops.get_sort_buffer_size(cap), torch.int8),
+                     self._zeros(1, torch.int32)]
+        self.cap = cap
+
+    def _scene_bufs(self, s: int):
+        if s not in self.per_scene:
+            torch = self.torch
+            P = self.scenes[s]["P"]
+            self.per_scene[s] = (self._zeros(24 * max(P, 1), torch.uint8),
+                                 self._zeros(self.ops.forward_workspace_size(P), torch.uint8))
+        return self.per_scene[s]
+
+    def _ranges(self, r):
+        key = (r.width, r.height)
+        if key not in self.per_res:
+            self.per_res[key] = self.torch.zeros((r.tiles, 2), dtype=self.torch.int32,
+                                                 device=self.dev)
+        return self.per_res[key]
+
+    def begin(self):
+        self.torch.cuda.synchronize(self.dev)
+        self.t0 = time.perf_counter()
+
+    def now_s(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def free_slot(self):
+        return 0 if self.busy[0] is None else None
+
+    def render(self, r, cap: int):
+        """One frame; returns (pairs, image tensor).  Grows the key buffers
+        and renders again when the frame needs more than their capacity."""
+        torch = self.torch
+        self._ensure_keys(max(int(cap), 1))
+        sc = self.scenes[r.scene]
+        rec, ws = self._scene_bufs(r.scene)
+        ranges = self._ranges(r)
+        cam = orbit_camera(r.width, r.height, r.azimuth, self.radius)
+        bg = torch.tensor(r.bg, dtype=torch.float32)
+        e = self._empty
+        while True:
+            ku, vu, ks, vs, space, curr = self.keys
+            out = torch.empty((r.height, r.width, 3), dtype=torch.uint8, device=self.dev)
+            try:
+                n = self.ops.forward_frame(sc["pos"], sc["shs"], sc["opac"], sc["cov"], r.width,
+                                           r.height, cam.position, cam.rotation, cam.focal_x,
+                                           cam.focal_y, cam.zFar, cam.zNear, 1, self.feat_prec,
+                                           curr, space, ku, vu, ks, vs, e, e, e, rec, ranges, bg,
+                                           out, ws, True)
+                return n, out
+            except RuntimeError as err:
+                if "Too many k-v pairs" not in str(err):
+                    raise
+                n = int(curr.item())  # the count the frame needed
+                self.grows += 1
+                self._ensure_keys(int(n * self.margin) + 4096)
+
+    def launch(self, reqs, nbytes, capacities, slot) -> InFlight:
+        fl = InFlight(self._bid, reqs, nbytes, list(capacities), slot)
+        self._bid += 1
+        counts = []
+        for i, r in enumerate(reqs):
+            n, out = self.render(r, capacities[i])
+            counts.append(n)
+            fl.outputs.append(out)
+            fl.capacities[i] = max(fl.capacities[i], n)  # rendered: never a retry
+        fl.host_counts = counts
+        fl.done = 1e3 * self.now_s()
+        self.busy[0] = fl
+        return fl
+
+    def poll(self):
+        fl = self.busy[0]
+        if fl is None:
+            return []
+        self.busy[0] = None
+        return [(fl, fl.done, list(fl.host_counts))]
+
+    def idle(self) -> bool:
+        return self.busy[0] is None
 
 
 class ServingLoop:
@@ -415,10 +547,12 @@ def make_resident_scenes(sizes=SCENE_SIZES, device="cuda:0", seed0: int = 101, b
     return out
 
 
-def calibrate(executor: GpuExecutor, est: Estimates, n_scenes: int, resolutions=RESOLUTIONS,
+def calibrate(executor, est: Estimates, n_scenes: int, resolutions=RESOLUTIONS,
               azimuths=(0.0, math.pi / 4), repeats: int = 3):
     """Renders every (scene, resolution) alone at a few orbit poses: warms the
     kernels and seeds the pair and time estimates."""
+    if isinstance(executor, SequentialExecutor):
+        return _calibrate_sequential(executor, est, n_scenes, resolutions, azimuths, repeats)
     torch = executor.torch
     executor.begin()
     for s in range(n_scenes):
@@ -442,4 +576,26 @@ def calibrate(executor: GpuExecutor, est: Estimates, n_scenes: int, resolutions=
                     torch.cuda.synchronize(executor.dev)
                     (fl, _, _), = executor.poll()
                     t.append(ev0.elapsed_time(fl.done))
+                est.observe(r.key, ms=float(np.median(t)))
+
+
+def _calibrate_sequential(ex: SequentialExecutor, est: Estimates, n_scenes: int, resolutions,
+                          azimuths, repeats: int):
+    torch = ex.torch
+    ex.begin()
+    for s in range(n_scenes):
+        for (w, h) in resolutions:
+            for a in azimuths:
+                r = Request(-1, s, w, h, a, 0.0)
+                n, _ = ex.render(r, ex.cap)
+                est.observe(r.key, pairs=n)
+                t = []
+                for _ in range(repeats):
+                    ev0 = torch.cuda.Event(enable_timing=True)
+                    ev1 = torch.cuda.Event(enable_timing=True)
+                    ev0.record()
+                    ex.render(r, ex.cap)
+                    ev1.record()
+                    torch.cuda.synchronize(ex.dev)
+                    t.append(ev0.elapsed_time(ev1))
                 est.observe(r.key, ms=float(np.median(t)))
