@@ -25,6 +25,7 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+VALU_PEAK_GINST = 1024 * 2.4 / 4  # wave64 FP32 VALU instructions, G/s
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 METRIC = "ms/frame + HBM GB/s @4K/6M Gaussians FP16 vs FP32; p99 latency @60Hz stream"
 
@@ -45,9 +46,8 @@ def parse():
                     help="comma-separated subset of fp16_persistent,fp32_plain")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
-    ap.add_argument("--serve-requests", type=int, default=0,
-                    help="requests in the 240 req/s serving stream (0 skips it; the stream "
-                         "is off by default until its GPU fault is understood)")
+    ap.add_argument("--serve-requests", type=int, default=1000,
+                    help="requests in the 240 req/s serving stream (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=240.0)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving stream")
     return ap.parse_args()
@@ -306,6 +306,19 @@ def main():
         t = json.loads(prof.read_text())
         roof["traffic"] = int(t["hbm_bytes_per_launch"])
         roof["traffic_source"] = "profiles/render_c3_hbm_traffic.json"
+        # The render's binding roofline is VALU issue: its wave-level VALU
+        # instructions per launch (counter profile) over this run's launch
+        # time, against 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction,
+        # and against the clock the profiled launch ran at.
+        if "valu_insts_per_launch" in t:
+            insts = float(t["valu_insts_per_launch"])
+            rate = insts / (rend_ms * 1e-3) / 1e9
+            clk = t["grbm_gui_active_per_launch"] / 8 / (t["profiled_avg_us"] * 1e-6) / 1e9
+            roof["valu"] = {"insts_per_launch": int(insts), "achieved_ginst_s": round(rate, 1),
+                            "peak_ginst_s": VALU_PEAK_GINST, "frac": round(rate / VALU_PEAK_GINST, 4),
+                            "profiled_clock_ghz": round(clk, 3),
+                            "frac_at_profiled_clock": round(
+                                insts / (1024 * t["grbm_gui_active_per_launch"] / 8 / 4), 4)}
     if Mn:
         # The render is bound by VALU issue (about a dozen VALU instructions
         # and one exp2 per pixel and Gaussian), not by HBM.

@@ -18,7 +18,7 @@ from polysplat_amd import serving as SV
 from tests import capi
 from tests.emu import frames as HF
 
-SIZES = (8_000, 30_000)
+SIZES = (6_000, 24_000)
 RES = ((256, 144), (160, 96))
 BOX = 6.0
 RADIUS = 1.5 * BOX
@@ -125,7 +125,7 @@ def test_serving_frames_equal_frames_rendered_alone(emu):
         for (w, h) in RES:
             est.observe((s, w, h), ms=1.0)
     sched = SV.Scheduler(SIZES, 1 << 30, est, max_frames=3, max_batch_ms=50.0)
-    reqs = SV.poisson_requests(10, 400.0, len(SIZES), resolutions=RES, seed=5)
+    reqs = SV.poisson_requests(8, 400.0, len(SIZES), resolutions=RES, seed=5)
     for i, r in enumerate(reqs):
         r.bg = ((i % 3) / 2.0, 0.25, 1.0 - (i % 2))
     ex = EmuExecutor(emu, scs)
