@@ -1,5 +1,5 @@
-// Tile renderers for gfx950: one 64-lane wave per 16x16 tile, 2x2 pixels per
-// lane, front-to-back alpha blending in FP32.
+// Tile renderers for gfx950: one 64-lane wave per 16x16 tile, four pixels of
+// one row per lane, front-to-back alpha blending in FP32.
 //
 // Per pixel (render.cu:228-271): d = centre - pixel (integer pixel coords),
 //   power = log2(o) + c.a*dx^2 + c.b*dx*dy + c.c*dy^2  (conic pre-scaled by
@@ -127,10 +127,13 @@ __device__ __forceinline__ void render_tile(
     float4 (*sA)[kBatch], float4 (*sB)[kBatch], float (*sC)[kBatch], int lane) {
   static_assert(kBatch % CHECK == 0, "a batch holds whole check groups");
   const int tx = tile % gx, ty = tile / gx;
-  const int px0 = tx * kTile + 2 * (lane & 7);
-  const int py0 = ty * kTile + 2 * (lane >> 3);
-  const float pxf0 = (float)px0, pxf1 = (float)(px0 + 1);
-  const float pyf0 = (float)py0, pyf1 = (float)(py0 + 1);
+  // Lane l shades pixels x0..x0+3 of row y0 (4 lanes per row): the row
+  // terms of the quadratic form are computed once per Gaussian.
+  const int px0 = tx * kTile + 4 * (lane & 3);
+  const int py0 = ty * kTile + (lane >> 2);
+  const float pxf0 = (float)px0, pxf1 = (float)(px0 + 1), pxf2 = (float)(px0 + 2),
+              pxf3 = (float)(px0 + 3);
+  const float pyf = (float)py0;
   float T[4] = {1.f, 1.f, 1.f, 1.f};
   float Cr[4] = {0.f, 0.f, 0.f, 0.f}, Cg[4] = {0.f, 0.f, 0.f, 0.f}, Cb[4] = {0.f, 0.f, 0.f, 0.f};
   const int n = end - start;
@@ -166,17 +169,16 @@ __device__ __forceinline__ void render_tile(
           const float4 A = sA[cur][k];
           const float4 B = sB[cur][k];
           const float bl = sC[cur][k];
-          // A = (x, y, w, a), B = (b, c, r, g)
-          const float dx0 = A.x - pxf0, dx1 = A.x - pxf1;
-          const float dy0 = A.y - pyf0, dy1 = A.y - pyf1;
-          const float xs0 = A.w * dx0 * dx0, xs1 = A.w * dx1 * dx1;
-          const float xc0 = B.x * dx0, xc1 = B.x * dx1;
-          const float ys0 = A.z + B.y * dy0 * dy0, ys1 = A.z + B.y * dy1 * dy1;
+          // A = (x, y, w, a), B = (b, c, r, g).  Per pixel the power is
+          // dx * (a dx + b dy) + (w + c dy^2): three operations once the row
+          // terms are known.
+          const float dy = A.y - pyf;
+          const float bdy = B.x * dy;
+          const float ys = fmaf(B.y * dy, dy, A.z);
+          const float dx[4] = {A.x - pxf0, A.x - pxf1, A.x - pxf2, A.x - pxf3};
           float pw[4];
-          pw[0] = fmaf(xc0, dy0, xs0 + ys0);
-          pw[1] = fmaf(xc1, dy0, xs1 + ys0);
-          pw[2] = fmaf(xc0, dy1, xs0 + ys1);
-          pw[3] = fmaf(xc1, dy1, xs1 + ys1);
+#pragma unroll
+          for (int p = 0; p < 4; ++p) pw[p] = fmaf(dx[p], fmaf(A.w, dx[p], bdy), ys);
 #pragma unroll
           for (int p = 0; p < 4; ++p) {
             if (DT && pw[p] < -7.994353375f) continue;
@@ -211,7 +213,7 @@ __device__ __forceinline__ void render_tile(
   }
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
-    const int px = px0 + (p & 1), py = py0 + (p >> 1);
+    const int px = px0 + p, py = py0;
     if (px < W && py < H) {
       uint8_t* o = out + ((size_t)py * W + px) * 3;
       if (n > 0) {
