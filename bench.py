@@ -226,6 +226,12 @@ def main():
                            rast.tile_ranges_buf, bg, rast._get_out_color(H, W, dev))
         ev[4].record()
 
+    staged([torch.cuda.Event(enable_timing=True) for _ in range(5)])
+    torch.cuda.synchronize()
+    lens = (rast.tile_ranges_buf[:T, 1] - rast.tile_ranges_buf[:T, 0]).cpu().numpy()
+    tile_lists = {"tiles": int(T), "nonempty": int((lens > 0).sum()),
+                  "mean_pairs": round(float(lens.mean()), 1), "p99_pairs": int(np.percentile(lens, 99)),
+                  "max_pairs": int(lens.max())}
     stage_ms = np.zeros(4)
     reps = max(3, args.steps // 2)
     for r in range(reps + 2):
@@ -302,7 +308,7 @@ def main():
             "unit": "GB/s", "frac": round(rend_bytes / (rend_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "traffic": None, "launch_ms": round(rend_ms, 4), "alg_bytes_per_launch": int(rend_bytes)}
     prof = ROOT / "profiles" / "render_c3_hbm_traffic.json"
-    if prof.exists():
+    if prof.exists() and args.config == "c3":
         t = json.loads(prof.read_text())
         roof["traffic"] = int(t["hbm_bytes_per_launch"])
         roof["traffic_source"] = "profiles/render_c3_hbm_traffic.json"
@@ -352,6 +358,7 @@ def main():
                                 "max_abs_diff": v["max_abs_diff"]}
                             for k, v in quality.items()},
         "roofline": roof,
+        "tile_lists": tile_lists,
         "cpu_baseline": cpu,
     }
     if args.serve_requests > 0:

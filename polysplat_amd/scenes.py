@@ -30,6 +30,12 @@ CONFIGS = {
     "c2": dict(P=3_000_000, box=50.0, width=1920, height=1080, mix=[(1.0, -4.5, 0.5)]),
     "c3": dict(P=6_000_000, box=50.0, width=3840, height=2160,
                mix=[(0.90, -5.0, 0.4), (0.09, -3.0, 0.4), (0.01, -1.0, 0.3)]),
+    # City: 64x64 blocks on a 2 km plane, log-normal block heights, seen by an
+    # elevated oblique camera (make_city_scene, config_camera).
+    "c4": dict(P=56_000_000, box=2000.0, width=9216, height=6912, mix=[(1.0, -4.5, 0.5)],
+               city=dict(blocks=64, extent=1000.0, footprint=0.7, height_median=25.0,
+                         height_sigma=0.6, cam_pos=(0.0, -600.0, -1400.0),
+                         cam_target=(0.0, 0.0, -100.0), z_far=5000.0)),
 }
 
 
@@ -102,7 +108,48 @@ def make_scene(P: int, box: float, mix, seed: int = 1, device="cpu",
 
 def make_config_scene(name: str, seed: int = 1, device="cpu", P: int | None = None) -> SceneArrays:
     c = CONFIGS[name]
+    if "city" in c:
+        return make_city_scene(P if P is not None else c["P"], seed=seed, device=device,
+                               **c["city"])
     return make_scene(P if P is not None else c["P"], c["box"], c["mix"], seed, device)
+
+
+def make_city_scene(P: int, seed: int = 4, device="cpu", blocks: int = 64,
+                    extent: float = 1000.0, footprint: float = 0.7,
+                    height_median: float = 25.0, height_sigma: float = 0.6,
+                    mix=((1.0, -4.5, 0.5),), chunk: int = 4_000_000, **_camera) -> SceneArrays:
+    """P Gaussians in a blocks x blocks grid over x, z in [-extent, extent]:
+    each Gaussian picks a block uniformly, lies in the central `footprint`
+    fraction of it and at a height uniform in [0, h_block], with h_block
+    log-normal per block (median height_median, log-sigma height_sigma).
+    Up is -y (the camera convention of make_camera looks along +z with +y
+    down the image).  Scales, rotations, opacities and SH as make_scene."""
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    pitch = 2.0 * extent / blocks
+    hb = torch.exp(torch.randn(blocks * blocks, generator=g, device=device) * height_sigma
+                   + math.log(height_median))
+    mean, std = mix[0][1], mix[0][2]
+    pos, shs, op, cov = [], [], [], []
+    for c0 in range(0, P, chunk):
+        n = min(chunk, P - c0)
+        b = torch.randint(0, blocks * blocks, (n,), generator=g, device=device)
+        bx = (b % blocks).float()
+        bz = torch.div(b, blocks, rounding_mode="floor").float()
+        u = torch.rand(n, 3, generator=g, device=device)
+        x = -extent + (bx + 0.5) * pitch + (u[:, 0] - 0.5) * footprint * pitch
+        z = -extent + (bz + 0.5) * pitch + (u[:, 1] - 0.5) * footprint * pitch
+        y = -u[:, 2] * hb[b]
+        ls = torch.randn(n, 3, generator=g, device=device) * std + mean
+        rot = torch.randn(n, 4, generator=g, device=device)
+        o = torch.sigmoid(torch.randn(n, generator=g, device=device) * 1.5)
+        sh = torch.randn(n, 48, generator=g, device=device) * 0.3
+        pos.append(torch.stack([x, y, z], 1).float())
+        shs.append(sh.float())
+        op.append(o.float())
+        cov.append(cov3d_from_scale_rot(torch.exp(ls), rot))
+    cat = lambda xs: torch.cat(xs, 0).contiguous()  # noqa: E731
+    return SceneArrays(cat(pos), cat(shs), cat(op), cat(cov))
 
 
 @dataclass
@@ -127,6 +174,27 @@ def make_camera(width: int, height: int, box: float, fov_deg: float = 60.0,
                       torch.eye(3, dtype=torch.float32), fx, fx)
 
 
+def look_at_camera(width: int, height: int, position, target, fov_deg: float = 60.0,
+                   z_far: float = 100.0) -> CameraSpec:
+    """Pinhole camera at `position` looking at `target`, with no roll about the
+    viewing direction (camera x stays in the world x-z plane).  The rotation
+    is the camera-to-world matrix given row by row; its third column is the
+    viewing direction."""
+    p = torch.tensor(position, dtype=torch.float64)
+    f = torch.tensor(target, dtype=torch.float64) - p
+    f = f / f.norm()
+    xa = torch.tensor([f[2].item(), 0.0, -f[0].item()], dtype=torch.float64)
+    xa = xa / xa.norm()
+    ya = torch.linalg.cross(f, xa)
+    rot = torch.stack([xa, ya, f], 1).float()  # columns: camera x, y, z in world
+    fx = width / (2.0 * math.tan(math.radians(fov_deg) / 2.0))
+    return CameraSpec(width, height, p.float(), rot, fx, fx, zFar=z_far)
+
+
 def config_camera(name: str) -> CameraSpec:
     c = CONFIGS[name]
+    if "city" in c:
+        k = c["city"]
+        return look_at_camera(c["width"], c["height"], k["cam_pos"], k["cam_target"],
+                              z_far=k["z_far"])
     return make_camera(c["width"], c["height"], c["box"])
