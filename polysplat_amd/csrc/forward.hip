@@ -29,7 +29,7 @@ int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_
                              const float* xy, const float* rgbd, const float* con, const void* rec,
                              int gathered, const uint64_t* keys, const uint32_t* order,
                              const int* ranges, int dt, const float bg[3], uint8_t* out,
-                             unsigned* queue, hipStream_t stream, const int* tile_order);
+                             unsigned* queue, hipStream_t stream);
 
 FrameDev make_frame_dev(int P, const float* positions, const void* shs, const float* opacities,
                         const float* cov3d, const ps_camera* cam, float* xy, float* rgbd,
@@ -205,13 +205,10 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
   // The sort leaves the tile ranges in tile_ranges when the renderer needs
   // them (its segment pass computes them anyway).
   const int tiles = ((cam->width + 15) / 16) * ((cam->height + 15) / 16);
-  // It also lists the tiles longest list first for the persistent renderer.
-  const int* tile_order = nullptr;
   st = sort_launch(0, d_n, fb->capacity, sort_end_bit(cam->width, cam->height, 16, 16),
                    fb->sort_space, fb->sort_space_bytes, fb->keys_unsorted, fb->values_unsorted,
                    fb->keys_sorted, fb->values_sorted, stream,
-                   need_ranges ? fb->tile_ranges : nullptr, tiles,
-                   need_ranges ? &tile_order : nullptr);
+                   need_ranges ? fb->tile_ranges : nullptr, tiles);
   if (st != PS_OK) return st;
   if (variant == PS_VARIANT_DEFAULT) {
     st = render_plain_launch(0, d_n, cam->width, cam->height, fb->points_xy, fb->rgb_depth,
@@ -223,7 +220,7 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
                                   fb->keys_sorted, fb->values_sorted,
                                   need_ranges ? fb->tile_ranges : nullptr,
                                   variant == PS_VARIANT_PERSISTENT_DT, bg, out_color, queue,
-                                  stream, tile_order);
+                                  stream);
   }
   if (st != PS_OK) return st;
   if (sync_count) {

@@ -124,23 +124,13 @@ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 // Launchers shared between translation units.
 int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles,
                             const uint64_t* keys, int* ranges, hipStream_t stream,
-                            uint32_t mask = 0xffffffffu, int* zero_counter = nullptr,
-                            int zero_words = 1);
-// Length classes of the tile order: class c holds tiles whose list length
-// has c leading zero bits (longest lists first); the last class holds empty
-// tiles.  `counters` needs 2 * kTileOrderClasses zeroed words.
-constexpr int kTileOrderClasses = 33;
-int tile_order_launch(const int* ranges, int tiles, int* order, unsigned* counters,
-                      hipStream_t stream);
+                            uint32_t mask = 0xffffffffu, int* zero_counter = nullptr);
 // Stable radix sort of (key, value) pairs over key bits [0, end_bit).  When
 // `tile_ranges` is given, the keys' high words are tile ids below `tiles`
 // and the sort also leaves each tile's [start, end) there.
-// With `tile_order`, also leaves in *tile_order a device array (inside the
-// sort space) that lists the tiles longest list first, or nullptr when the
-// tiles do not fit there.
 int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* space,
                 size_t space_bytes, const uint64_t* keys_in, const uint32_t* vals_in,
                 uint64_t* keys_out, uint32_t* vals_out, hipStream_t stream,
-                int* tile_ranges = nullptr, int tiles = 0, const int** tile_order = nullptr);
+                int* tile_ranges = nullptr, int tiles = 0);
 
 }  // namespace ps

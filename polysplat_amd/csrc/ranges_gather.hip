@@ -48,11 +48,10 @@ __global__ __launch_bounds__(kThreads) void ps_ranges_bsearch(const uint64_t* __
   ranges[t] = make_int2(start, lo);
 }
 
-// Zeroes `count` ranges and, when given, `words` (at most kThreads) counter
-// words.
+// Zeroes `count` ranges and, when given, one extra counter word.
 __global__ __launch_bounds__(kThreads) void ps_fill_int2(int2* __restrict__ p, int count,
-                                                         int* __restrict__ counter, int words) {
-  if (counter != nullptr && blockIdx.x == 0 && threadIdx.x < words) counter[threadIdx.x] = 0;
+                                                         int* __restrict__ counter) {
+  if (counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < count; i += gridDim.x * kThreads)
     p[i] = make_int2(0, 0);
 }
@@ -106,58 +105,6 @@ __global__ __launch_bounds__(kThreads) void ps_gather_rec(const int* __restrict_
     dst[i] = src[order[i]];
 }
 
-// Tile order for the persistent renderer: tiles by decreasing list-length
-// class (leading zeros of the length), so long tiles start first and the
-// persistent grid does not end waiting on one long tile taken late.  Order
-// inside a class is whatever the atomics give; every tile is rendered the
-// same way in any order.  Each workgroup takes a contiguous chunk of tiles.
-__device__ __forceinline__ int order_class(int2 r) {
-  const int len = r.y - r.x;
-  return len > 0 ? __clz(len) : kTileOrderClasses - 1;
-}
-
-__global__ __launch_bounds__(kThreads) void ps_tile_order_count(const int2* __restrict__ ranges,
-                                                                int tiles, int chunk,
-                                                                unsigned* __restrict__ totals) {
-  __shared__ unsigned hist[kTileOrderClasses];
-  const int tid = threadIdx.x;
-  if (tid < kTileOrderClasses) hist[tid] = 0;
-  __syncthreads();
-  const int t0 = blockIdx.x * chunk;
-  const int t1 = min(t0 + chunk, tiles);
-  for (int t = t0 + tid; t < t1; t += kThreads) atomicAdd(&hist[order_class(ranges[t])], 1u);
-  __syncthreads();
-  if (tid < kTileOrderClasses && hist[tid] > 0) atomicAdd(&totals[tid], hist[tid]);
-}
-
-__global__ __launch_bounds__(kThreads) void ps_tile_order_scatter(
-    const int2* __restrict__ ranges, int tiles, int chunk, const unsigned* __restrict__ totals,
-    unsigned* __restrict__ cursors, int* __restrict__ order) {
-  __shared__ unsigned hist[kTileOrderClasses];
-  __shared__ unsigned base[kTileOrderClasses];
-  __shared__ unsigned local[kTileOrderClasses];
-  const int tid = threadIdx.x;
-  if (tid < kTileOrderClasses) {
-    hist[tid] = 0;
-    local[tid] = 0;
-  }
-  __syncthreads();
-  const int t0 = blockIdx.x * chunk;
-  const int t1 = min(t0 + chunk, tiles);
-  for (int t = t0 + tid; t < t1; t += kThreads) atomicAdd(&hist[order_class(ranges[t])], 1u);
-  __syncthreads();
-  if (tid < kTileOrderClasses) {
-    unsigned start = 0;
-    for (int c = 0; c < tid; ++c) start += totals[c];
-    base[tid] = hist[tid] > 0 ? start + atomicAdd(&cursors[tid], hist[tid]) : 0u;
-  }
-  __syncthreads();
-  for (int t = t0 + tid; t < t1; t += kThreads) {
-    const int c = order_class(ranges[t]);
-    order[base[c] + atomicAdd(&local[c], 1u)] = t;
-  }
-}
-
 inline int grid_for(int n) {
   int g = (n + kThreads - 1) / kThreads;
   if (g > kMaxGrid) g = kMaxGrid;
@@ -168,10 +115,9 @@ inline int grid_for(int n) {
 
 int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles,
                             const uint64_t* keys, int* ranges, hipStream_t stream, uint32_t mask,
-                            int* zero_counter, int zero_words) {
-  PS_REQUIRE(zero_words >= 0 && zero_words <= kThreads, "bad counter word count");
+                            int* zero_counter) {
   hipLaunchKernelGGL(ps_fill_int2, dim3(grid_for(tiles)), dim3(kThreads), 0, stream,
-                     reinterpret_cast<int2*>(ranges), tiles, zero_counter, zero_words);
+                     reinterpret_cast<int2*>(ranges), tiles, zero_counter);
   PS_LAUNCH_CHECK();
   const int n = d_n ? max_pairs : n_host;
   if (n > 0) {
@@ -179,22 +125,6 @@ int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles
                        n_host, max_pairs, tiles, mask, reinterpret_cast<int2*>(ranges));
     PS_LAUNCH_CHECK();
   }
-  return PS_OK;
-}
-
-int tile_order_launch(const int* ranges, int tiles, int* order, unsigned* counters,
-                      hipStream_t stream) {
-  if (tiles <= 0) return PS_OK;
-  int grid = (tiles + 1023) / 1024;
-  if (grid > 256) grid = 256;
-  const int chunk = (tiles + grid - 1) / grid;
-  const int2* r = reinterpret_cast<const int2*>(ranges);
-  hipLaunchKernelGGL(ps_tile_order_count, dim3(grid), dim3(kThreads), 0, stream, r, tiles, chunk,
-                     counters);
-  PS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(ps_tile_order_scatter, dim3(grid), dim3(kThreads), 0, stream, r, tiles,
-                     chunk, (const unsigned*)counters, counters + kTileOrderClasses, order);
-  PS_LAUNCH_CHECK();
   return PS_OK;
 }
 

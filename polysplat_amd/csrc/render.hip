@@ -250,7 +250,6 @@ __global__ __launch_bounds__(kWave, 8) void ps_render_plain(
 }
 
 // Persistent: tiles from the queue.  RANGES: read tile_ranges, else search.
-// tile_order (optional) maps queue positions to tiles (longest lists first).
 template <int PREC, bool RANGES, bool GATHERED, int CHECK, bool DT>
 __global__ __launch_bounds__(kWave, 8) void ps_render_persistent(
     const uint64_t* __restrict__ keys, const int* __restrict__ d_n, int n_host,
@@ -258,15 +257,14 @@ __global__ __launch_bounds__(kWave, 8) void ps_render_persistent(
     const uint32_t* __restrict__ order, const float2* __restrict__ xy,
     const float4* __restrict__ rgbd, const float4* __restrict__ con,
     const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
-    unsigned* __restrict__ queue, const int* __restrict__ tile_order) {
+    unsigned* __restrict__ queue) {
   __shared__ float4 sA[2][kBatch];
   __shared__ float4 sB[2][kBatch];
   __shared__ float sC[2][kBatch];
   const int lane = threadIdx.x;
   const int n = RANGES ? 0 : (d_n ? *d_n : n_host);
-  int pos = blockIdx.x;
-  while (pos < tiles) {
-    const int tile = tile_order ? tile_order[pos] : pos;
+  int tile = blockIdx.x;
+  while (tile < tiles) {
     int start, end;
     if (RANGES) {
       const int2 r = ranges[tile];
@@ -280,7 +278,7 @@ __global__ __launch_bounds__(kWave, 8) void ps_render_persistent(
                                            bg, out, sA, sB, sC, lane);
     int next = 0;
     if (lane == 0) next = (int)atomicAdd(&queue[0], 1u) + (int)gridDim.x;
-    pos = bcast(next, 0);
+    tile = bcast(next, 0);
   }
   // The last workgroup out resets the queue for the next launch.
   if (lane == 0) {
@@ -369,7 +367,7 @@ template <int PREC, bool RANGES, bool GATHERED, int CHECK, bool DT>
 int launch_persistent(const uint64_t* keys, const int* d_n, int n_host, const int* ranges, int W,
                       int H, const uint32_t* order, const float* xy, const float* rgbd,
                       const float* con, const void* rec, float3 bg, uint8_t* out, unsigned* queue,
-                      const int* tile_order, hipStream_t stream) {
+                      hipStream_t stream) {
   const int gx = (W + kTile - 1) / kTile, gy = (H + kTile - 1) / kTile;
   const int tiles = gx * gy;
   auto k = ps_render_persistent<PREC, RANGES, GATHERED, CHECK, DT>;
@@ -377,7 +375,7 @@ int launch_persistent(const uint64_t* keys, const int* d_n, int n_host, const in
   hipLaunchKernelGGL(k, dim3(grid), dim3(kWave), 0, stream, keys, d_n, n_host,
                      reinterpret_cast<const int2*>(ranges), W, H, gx, tiles, order,
                      (const float2*)xy, (const float4*)rgbd, (const float4*)con,
-                     (const FeatRec16*)rec, bg, out, queue, RANGES ? tile_order : nullptr);
+                     (const FeatRec16*)rec, bg, out, queue);
   PS_LAUNCH_CHECK();
   return PS_OK;
 }
@@ -386,17 +384,17 @@ template <int PREC, bool RANGES, bool GATHERED>
 int dispatch_dt(bool dt, const uint64_t* keys, const int* d_n, int n_host, const int* ranges,
                 int W, int H, const uint32_t* order, const float* xy, const float* rgbd,
                 const float* con, const void* rec, float3 bg, uint8_t* out, unsigned* queue,
-                const int* tile_order, hipStream_t s) {
+                hipStream_t s) {
   // Cadence of the saturation test: every 8 Gaussians with precomputed
   // ranges (render.cu:1814), every 16 with the in-kernel search (render.cu:2124).
   constexpr int kCheck = RANGES ? 8 : 16;
   if (dt)
     return launch_persistent<PREC, RANGES, GATHERED, kCheck, true>(keys, d_n, n_host, ranges, W, H,
                                                                    order, xy, rgbd, con, rec, bg,
-                                                                   out, queue, tile_order, s);
+                                                                   out, queue, s);
   return launch_persistent<PREC, RANGES, GATHERED, kCheck, false>(keys, d_n, n_host, ranges, W, H,
                                                                   order, xy, rgbd, con, rec, bg,
-                                                                  out, queue, tile_order, s);
+                                                                  out, queue, s);
 }
 
 template <int PREC>
@@ -404,19 +402,19 @@ int dispatch_persistent(bool ranges_mode, bool gathered, bool dt, const uint64_t
                         const int* d_n, int n_host, const int* ranges, int W, int H,
                         const uint32_t* order, const float* xy, const float* rgbd,
                         const float* con, const void* rec, float3 bg, uint8_t* out,
-                        unsigned* queue, const int* tile_order, hipStream_t s) {
+                        unsigned* queue, hipStream_t s) {
   if (ranges_mode) {
     if (gathered)
       return dispatch_dt<PREC, true, true>(dt, keys, d_n, n_host, ranges, W, H, order, xy, rgbd,
-                                           con, rec, bg, out, queue, tile_order, s);
+                                           con, rec, bg, out, queue, s);
     return dispatch_dt<PREC, true, false>(dt, keys, d_n, n_host, ranges, W, H, order, xy, rgbd,
-                                          con, rec, bg, out, queue, tile_order, s);
+                                          con, rec, bg, out, queue, s);
   }
   if (gathered)
     return dispatch_dt<PREC, false, true>(dt, keys, d_n, n_host, ranges, W, H, order, xy, rgbd,
-                                          con, rec, bg, out, queue, tile_order, s);
+                                          con, rec, bg, out, queue, s);
   return dispatch_dt<PREC, false, false>(dt, keys, d_n, n_host, ranges, W, H, order, xy, rgbd, con,
-                                         rec, bg, out, queue, tile_order, s);
+                                         rec, bg, out, queue, s);
 }
 
 }  // namespace
@@ -464,7 +462,7 @@ int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_
                              const float* xy, const float* rgbd, const float* con, const void* rec,
                              int gathered, const uint64_t* keys, const uint32_t* order,
                              const int* ranges, int dt, const float bg[3], uint8_t* out,
-                             unsigned* queue, hipStream_t stream, const int* tile_order) {
+                             unsigned* queue, hipStream_t stream) {
   PS_REQUIRE(W > 0 && H > 0, "bad image size");
   PS_REQUIRE(out != nullptr && bg != nullptr && queue != nullptr, "NULL output/bg/queue");
   PS_REQUIRE(ranges != nullptr || keys != nullptr, "need tile_ranges or keys_sorted");
@@ -476,15 +474,15 @@ int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_
     case PS_PREC_F32:
       PS_REQUIRE(xy && rgbd && con, "f32 feature arrays are NULL");
       return dispatch_persistent<kFeatF32>(rm, g, d, keys, d_n, n_host, ranges, W, H, order, xy,
-                                           rgbd, con, rec, b, out, queue, tile_order, stream);
+                                           rgbd, con, rec, b, out, queue, stream);
     case PS_PREC_F16:
       PS_REQUIRE(rec != nullptr, "records are NULL");
       return dispatch_persistent<kFeatF16>(rm, g, d, keys, d_n, n_host, ranges, W, H, order, xy,
-                                           rgbd, con, rec, b, out, queue, tile_order, stream);
+                                           rgbd, con, rec, b, out, queue, stream);
     case PS_PREC_BF16:
       PS_REQUIRE(rec != nullptr, "records are NULL");
       return dispatch_persistent<kFeatBF16>(rm, g, d, keys, d_n, n_host, ranges, W, H, order, xy,
-                                            rgbd, con, rec, b, out, queue, tile_order, stream);
+                                            rgbd, con, rec, b, out, queue, stream);
     default:
       set_last_error("unknown feature precision %d", feat_prec);
       return PS_ERR_INVALID;
@@ -514,5 +512,5 @@ extern "C" int ps_render_16x16_persistent(int num_rendered, const int* d_num_ren
   return ps::render_persistent_launch(num_rendered, d_num_rendered, width, height, feat_prec,
                                       points_xy, rgb_depth, conic_opacity, records, gathered,
                                       keys_sorted, values_sorted, tile_ranges, dt, bg, out_color,
-                                      queue, reinterpret_cast<hipStream_t>(stream), nullptr);
+                                      queue, reinterpret_cast<hipStream_t>(stream));
 }

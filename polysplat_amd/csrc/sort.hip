@@ -546,8 +546,8 @@ inline size_t sort_layout(int cap, char* base, SortWs* w) {
   off += align_up((size_t)kRadix * sizeof(uint32_t), 256);
   if (w) w->seg_ranges = reinterpret_cast<int2*>(base + off);
   off += align_up((size_t)kSegMaxSegs * sizeof(int2), 256);
-  if (w) w->big_count = reinterpret_cast<int*>(base + off);  // + tile-order counters
-  off += 512;
+  if (w) w->big_count = reinterpret_cast<int*>(base + off);
+  off += 256;
   if (w) w->big_list = reinterpret_cast<int*>(base + off);
   off += align_up(big_list_cap(c) * sizeof(int), 256);
   return off;
@@ -594,7 +594,7 @@ int digit_passes(int n_host, const int* d_n, int max_pairs, int bit0, int bit1, 
 int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* space,
                 size_t space_bytes, const uint64_t* keys_in, const uint32_t* vals_in,
                 uint64_t* keys_out, uint32_t* vals_out, hipStream_t stream, int* tile_ranges,
-                int tiles, const int** tile_order) {
+                int tiles) {
   PS_REQUIRE(max_pairs >= 0, "max_pairs must be >= 0");
   PS_REQUIRE(d_n != nullptr || (n_host >= 0 && n_host <= max_pairs),
              "num_rendered %d outside [0, %d]", n_host, max_pairs);
@@ -606,37 +606,19 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
   PS_REQUIRE(tile_ranges == nullptr || tiles > 0, "tile ranges need a tile count");
   SortWs w;
   sort_layout(max_pairs, static_cast<char*>(space), &w);
-  // The tile order (longest lists first) lives in the digit-count area,
-  // which is free once the digit passes are done; its counters follow the
-  // big-segment counter and are zeroed with it.
-  if (tile_order) *tile_order = nullptr;
-  const bool want_order = tile_order != nullptr && tile_ranges != nullptr &&
-                          (size_t)tiles <= (size_t)kRadix * kSortBlocks;
-  unsigned* order_counters = reinterpret_cast<unsigned*>(w.big_count + 1);
-  int* order = reinterpret_cast<int*>(w.counts);
-  const int zero_words = want_order ? 1 + 2 * kTileOrderClasses : 1;
-  auto finish_order = [&]() -> int {
-    if (!want_order) return PS_OK;
-    const int st = tile_order_launch(tile_ranges, tiles, order, order_counters, stream);
-    if (st == PS_OK) *tile_order = order;
-    return st;
-  };
   const int seg_bits = end_bit - 32;
   const bool segmented = use_segmented() && seg_bits > 0 &&
                          (tile_ranges != nullptr || seg_bits <= 18);
   if (!segmented) {
     if (d_n == nullptr && n_host == 0) {
-      if (!tile_ranges) return PS_OK;
-      const int st = tile_ranges_scan_launch(0, nullptr, 0, tiles, keys_out, tile_ranges, stream,
-                                             0xffffffffu, w.big_count, zero_words);
-      return st != PS_OK ? st : finish_order();
+      if (tile_ranges) return tile_ranges_scan_launch(0, nullptr, 0, tiles, keys_out,
+                                                      tile_ranges, stream);
+      return PS_OK;
     }
     int st = digit_passes(n_host, d_n, max_pairs, 0, end_bit, w, keys_in, vals_in, keys_out,
                           vals_out, stream);
     if (st != PS_OK || tile_ranges == nullptr) return st;
-    st = tile_ranges_scan_launch(n_host, d_n, max_pairs, tiles, keys_out, tile_ranges, stream,
-                                 0xffffffffu, w.big_count, zero_words);
-    return st != PS_OK ? st : finish_order();
+    return tile_ranges_scan_launch(n_host, d_n, max_pairs, tiles, keys_out, tile_ranges, stream);
   }
   // Digit passes over the tile bits, then segment ranges, then the segment
   // pass in place in the output.
@@ -654,9 +636,7 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
                       vals_out, stream);
   if (st != PS_OK) return st;
   st = tile_ranges_scan_launch(n_host, d_n, max_pairs, segs, keys_out, ranges, stream, mask,
-                               w.big_count, zero_words);
-  if (st != PS_OK) return st;
-  st = finish_order();
+                               w.big_count);
   if (st != PS_OK) return st;
   if (d_n == nullptr && n_host == 0) return PS_OK;
   const int big_cap = (int)big_list_cap((size_t)(max_pairs > 0 ? max_pairs : 1));
