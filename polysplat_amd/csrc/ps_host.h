@@ -124,7 +124,8 @@ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 // Launchers shared between translation units.
 int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles,
                             const uint64_t* keys, int* ranges, hipStream_t stream,
-                            uint32_t mask = 0xffffffffu, int* zero_counter = nullptr);
+                            uint32_t mask = 0xffffffffu, int* zero_counter = nullptr,
+                            int zero_words = 1);
 // Stable radix sort of (key, value) pairs over key bits [0, end_bit).  When
 // `tile_ranges` is given, the keys' high words are tile ids below `tiles`
 // and the sort also leaves each tile's [start, end) there.

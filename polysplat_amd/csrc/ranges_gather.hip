@@ -48,10 +48,11 @@ __global__ __launch_bounds__(kThreads) void ps_ranges_bsearch(const uint64_t* __
   ranges[t] = make_int2(start, lo);
 }
 
-// Zeroes `count` ranges and, when given, one extra counter word.
+// Zeroes `count` ranges and, when given, `words` (at most kThreads) counter
+// words.
 __global__ __launch_bounds__(kThreads) void ps_fill_int2(int2* __restrict__ p, int count,
-                                                         int* __restrict__ counter) {
-  if (counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
+                                                         int* __restrict__ counter, int words) {
+  if (counter != nullptr && blockIdx.x == 0 && threadIdx.x < words) counter[threadIdx.x] = 0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < count; i += gridDim.x * kThreads)
     p[i] = make_int2(0, 0);
 }
@@ -115,9 +116,10 @@ inline int grid_for(int n) {
 
 int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles,
                             const uint64_t* keys, int* ranges, hipStream_t stream, uint32_t mask,
-                            int* zero_counter) {
+                            int* zero_counter, int zero_words) {
+  PS_REQUIRE(zero_words >= 0 && zero_words <= kThreads, "bad counter word count");
   hipLaunchKernelGGL(ps_fill_int2, dim3(grid_for(tiles)), dim3(kThreads), 0, stream,
-                     reinterpret_cast<int2*>(ranges), tiles, zero_counter);
+                     reinterpret_cast<int2*>(ranges), tiles, zero_counter, zero_words);
   PS_LAUNCH_CHECK();
   const int n = d_n ? max_pairs : n_host;
   if (n > 0) {

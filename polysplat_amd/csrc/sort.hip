@@ -412,6 +412,149 @@ __global__ __launch_bounds__(kSegWaves * kWave) void ps_seg_small(
   }
 }
 
+// Listed segments of up to kMidCap pairs whose keys agree above the sorted
+// bits: a 512-thread workgroup per segment holds it in registers (item
+// w*1024 + r*64 + l in round r of lane l of wave w) and runs one stable
+// counting pass per non-constant 8-bit digit of the low word: each wave ranks
+// its items against its own LDS digit counters (ballot match), a block scan
+// over (digit, wave) turns the counts into starts, and the pairs are
+// scattered through LDS and read back in the new order.  Other listed
+// segments go on to the second list for ps_seg_big.
+constexpr int kMidWaves = 8;
+constexpr int kMidRounds = 16;
+constexpr int kMidCap = kMidWaves * kMidRounds * kWave;  // 8192
+
+__global__ __launch_bounds__(kMidWaves * kWave) void ps_seg_mid(
+    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, const int2* __restrict__ ranges,
+    const int* __restrict__ big_list, const int* __restrict__ big_count, int big_cap,
+    int* __restrict__ big_list2, int* __restrict__ big_count2) {
+  __shared__ uint32_t s_lo[kMidCap];
+  __shared__ uint32_t s_v[kMidCap];
+  __shared__ uint32_t wcnt[kMidWaves][kRadix];
+  __shared__ int s_wsum[kMidWaves];
+  __shared__ uint32_t s_diff;
+  __shared__ int s_mixed;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t below = lanes_below(lane);
+  const int nbig = min(*big_count, big_cap);
+  for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+    const int seg = big_list[b];
+    const int2 rg = ranges[seg];
+    const int n = rg.y - rg.x;
+    if (n > kMidCap) {  // uniform
+      if (tid == 0) big_list2[atomicAdd(big_count2, 1)] = seg;
+      continue;
+    }
+    const uint64_t k0 = keys[rg.x];
+    const uint32_t hi0 = (uint32_t)(k0 >> 32);
+    if (tid == 0) {
+      s_diff = 0u;
+      s_mixed = 0;
+    }
+    __syncthreads();
+    uint32_t lo[kMidRounds], v[kMidRounds];
+    bool same = true;
+    uint32_t diff = 0;
+#pragma unroll
+    for (int r = 0; r < kMidRounds; ++r) {
+      const int j = wave * (kMidRounds * kWave) + r * kWave + lane;
+      lo[r] = 0;
+      v[r] = 0;
+      if (j < n) {
+        const uint64_t k = keys[rg.x + j];
+        same = same && (uint32_t)(k >> 32) == hi0;
+        lo[r] = (uint32_t)k;
+        v[r] = vals[rg.x + j];
+        diff |= lo[r] ^ (uint32_t)k0;
+      }
+    }
+    if (!same) s_mixed = 1;
+    if (diff) atomicOr(&s_diff, diff);
+    __syncthreads();
+    const bool mixed = s_mixed != 0;
+    const uint32_t all_diff = s_diff;
+    __syncthreads();  // s_mixed / s_diff are rewritten for the next segment
+    if (mixed) {  // uniform
+      if (tid == 0) big_list2[atomicAdd(big_count2, 1)] = seg;
+      continue;
+    }
+    for (int p = 0; p < 4; ++p) {
+      const int shift = 8 * p;
+      if (((all_diff >> shift) & 0xffu) == 0) continue;  // uniform: constant digit
+      for (int i = lane; i < kRadix; i += kWave) wcnt[wave][i] = 0;
+      __builtin_amdgcn_wave_barrier();
+      int dg[kMidRounds];
+      uint32_t pos[kMidRounds];
+#pragma unroll
+      for (int r = 0; r < kMidRounds; ++r) {
+        const int j = wave * (kMidRounds * kWave) + r * kWave + lane;
+        const bool ok = j < n;
+        const int d = (int)((lo[r] >> shift) & 0xffu);
+        const uint64_t peers = match_digit(d, ok);
+        const uint32_t rank = (uint32_t)popc64(peers & below);
+        const uint32_t base = ok ? wcnt[wave][d] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (ok && rank == 0) wcnt[wave][d] = base + (uint32_t)popc64(peers);
+        __builtin_amdgcn_wave_barrier();
+        dg[r] = ok ? d : -1;
+        pos[r] = base + rank;
+      }
+      __syncthreads();
+      // Digit starts over (digit, wave): threads 0..255 own one digit each.
+      uint32_t c[kMidWaves];
+      uint32_t tot = 0;
+      if (tid < kRadix) {
+#pragma unroll
+        for (int w = 0; w < kMidWaves; ++w) {
+          c[w] = wcnt[w][tid];
+          tot += c[w];
+        }
+      }
+      const int incl = wave_incl_scan((int)tot, lane);
+      if (lane == 63) s_wsum[wave] = incl;
+      __syncthreads();
+      if (tid < kRadix) {
+        int before = 0;
+        for (int w = 0; w < wave; ++w) before += s_wsum[w];
+        uint32_t start = (uint32_t)(before + incl) - tot;
+#pragma unroll
+        for (int w = 0; w < kMidWaves; ++w) {
+          wcnt[w][tid] = start;
+          start += c[w];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < kMidRounds; ++r) {
+        if (dg[r] >= 0) {
+          const uint32_t at = wcnt[wave][dg[r]] + pos[r];
+          s_lo[at] = lo[r];
+          s_v[at] = v[r];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < kMidRounds; ++r) {
+        const int j = wave * (kMidRounds * kWave) + r * kWave + lane;
+        if (j < n) {
+          lo[r] = s_lo[j];
+          v[r] = s_v[j];
+        }
+      }
+      __syncthreads();
+    }
+    const uint64_t hi = (uint64_t)hi0 << 32;
+#pragma unroll
+    for (int r = 0; r < kMidRounds; ++r) {
+      const int j = wave * (kMidRounds * kWave) + r * kWave + lane;
+      if (j < n) {
+        keys[rg.x + j] = hi | lo[r];
+        vals[rg.x + j] = v[r];
+      }
+    }
+  }
+}
+
 // Listed segments: a workgroup per segment, four stable counting passes by
 // the low 32 bits through global memory (keys <-> tmp, ending in keys).
 // Chunks of 1024 pairs: wave w ranks items [256w, 256w + 256) of the chunk
@@ -519,8 +662,9 @@ struct SortWs {
   uint32_t* offsets;
   uint32_t* totals;
   int2* seg_ranges;  // segment ranges when the caller gives none
-  int* big_list;     // segments for ps_seg_big
-  int* big_count;
+  int* big_list;     // segments longer than kSegCap (or mixed high words)
+  int* big_count;    // [0]: entries of big_list; [1]: entries of big_list2
+  int* big_list2;    // segments for ps_seg_big
 };
 
 // Segments (values of the sorted high bits) the internal range table holds.
@@ -549,6 +693,8 @@ inline size_t sort_layout(int cap, char* base, SortWs* w) {
   if (w) w->big_count = reinterpret_cast<int*>(base + off);
   off += 256;
   if (w) w->big_list = reinterpret_cast<int*>(base + off);
+  off += align_up(big_list_cap(c) * sizeof(int), 256);
+  if (w) w->big_list2 = reinterpret_cast<int*>(base + off);
   off += align_up(big_list_cap(c) * sizeof(int), 256);
   return off;
 }
@@ -636,7 +782,7 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
                       vals_out, stream);
   if (st != PS_OK) return st;
   st = tile_ranges_scan_launch(n_host, d_n, max_pairs, segs, keys_out, ranges, stream, mask,
-                               w.big_count);
+                               w.big_count, 2);
   if (st != PS_OK) return st;
   if (d_n == nullptr && n_host == 0) return PS_OK;
   const int big_cap = (int)big_list_cap((size_t)(max_pairs > 0 ? max_pairs : 1));
@@ -646,9 +792,13 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
                      vals_out, reinterpret_cast<const int2*>(ranges), segs, w.big_list,
                      w.big_count, big_cap);
   PS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(ps_seg_mid, dim3(1024), dim3(kMidWaves * kWave), 0, stream, keys_out,
+                     vals_out, reinterpret_cast<const int2*>(ranges), w.big_list, w.big_count,
+                     big_cap, w.big_list2, w.big_count + 1);
+  PS_LAUNCH_CHECK();
   hipLaunchKernelGGL(ps_seg_big, dim3(512), dim3(kSortThreads), 0, stream, keys_out, vals_out,
-                     w.alt_keys, w.alt_vals, reinterpret_cast<const int2*>(ranges), w.big_list,
-                     w.big_count, big_cap);
+                     w.alt_keys, w.alt_vals, reinterpret_cast<const int2*>(ranges), w.big_list2,
+                     w.big_count + 1, big_cap);
   PS_LAUNCH_CHECK();
   return PS_OK;
 }

@@ -51,10 +51,11 @@ def _tie_keys(counts, rng):
 
 
 @pytest.mark.parametrize("mode", ["segmented", "lsd"])
-def test_sort_segments_around_the_lds_limit(emu, mode, monkeypatch):
+def test_sort_segments_around_the_lds_limits(emu, mode, monkeypatch):
     if mode == "lsd":
         monkeypatch.setenv("POLYSPLAT_SORT", "lsd")
-    counts = [0, 1, 2, 63, 64, 65, 895, 896, 897, 1024, 1025, 3755, 0, 7]
+    # per-wave sort (<= 896), per-workgroup sort (<= 8192), global passes
+    counts = [0, 1, 2, 63, 64, 65, 895, 896, 897, 1024, 1025, 3755, 0, 7, 8191, 8192, 8193, 9000]
     rng = np.random.default_rng(99)
     keys = _tie_keys(counts, rng)
     vals = rng.permutation(len(keys)).astype(np.uint32)

@@ -119,11 +119,12 @@ def _keys_with_counts(counts, rng, tiles):
 
 @pytest.mark.parametrize("mode", ["segmented", "lsd"])
 def test_sort_segment_sizes_around_the_lds_limit(gpu, mode, monkeypatch):
-    # segments of 0, 1, 2, 63..65, 895..897, 1023..1025 and 5000 pairs: wave path,
-    # its limit and the workgroup path for long segments
+    # segments of 0, 1, 2, 63..65, 895..897, 1023..1025, 5000 and 8191..9000
+    # pairs: the per-wave path and its limit, the per-workgroup LDS path and
+    # its limit, and the global-memory path beyond it
     if mode == "lsd":
         monkeypatch.setenv("POLYSPLAT_SORT", "lsd")
-    counts = [0, 1, 2, 63, 64, 65, 895, 896, 897, 1024, 1025, 5000, 0, 7]
+    counts = [0, 1, 2, 63, 64, 65, 895, 896, 897, 1024, 1025, 5000, 0, 7, 8191, 8192, 8193, 9000]
     rng = np.random.default_rng(99)
     keys = _keys_with_counts(counts, rng, len(counts))
     n = len(keys)
