@@ -413,8 +413,9 @@ __global__ __launch_bounds__(kSegWaves * kWave) void ps_seg_small(
 }
 
 // Listed segments of up to kMidCap pairs whose keys agree above the sorted
-// bits: a 512-thread workgroup per segment holds it in registers (item
-// w*1024 + r*64 + l in round r of lane l of wave w) and runs one stable
+// bits: a 512-thread workgroup per segment holds it in registers (wave w
+// takes the contiguous chunk [w*c, w*c + c), c = n / 8 rounded up to whole
+// rounds of 64; item w*c + r*64 + l is round r of lane l) and runs one stable
 // counting pass per non-constant 8-bit digit of the low word: each wave ranks
 // its items against its own LDS digit counters (ballot match), a block scan
 // over (digit, wave) turns the counts into starts, and the pairs are
@@ -447,6 +448,9 @@ __global__ __launch_bounds__(kMidWaves * kWave) void ps_seg_mid(
     }
     const uint64_t k0 = keys[rg.x];
     const uint32_t hi0 = (uint32_t)(k0 >> 32);
+    const int rounds = (n + kMidWaves * kWave - 1) / (kMidWaves * kWave);  // uniform, <= 16
+    const int chunk = rounds * kWave;
+    const int first = wave * chunk;
     if (tid == 0) {
       s_diff = 0u;
       s_mixed = 0;
@@ -457,10 +461,10 @@ __global__ __launch_bounds__(kMidWaves * kWave) void ps_seg_mid(
     uint32_t diff = 0;
 #pragma unroll
     for (int r = 0; r < kMidRounds; ++r) {
-      const int j = wave * (kMidRounds * kWave) + r * kWave + lane;
+      const int j = first + r * kWave + lane;
       lo[r] = 0;
       v[r] = 0;
-      if (j < n) {
+      if (r < rounds && j < n) {
         const uint64_t k = keys[rg.x + j];
         same = same && (uint32_t)(k >> 32) == hi0;
         lo[r] = (uint32_t)k;
@@ -487,7 +491,10 @@ __global__ __launch_bounds__(kMidWaves * kWave) void ps_seg_mid(
       uint32_t pos[kMidRounds];
 #pragma unroll
       for (int r = 0; r < kMidRounds; ++r) {
-        const int j = wave * (kMidRounds * kWave) + r * kWave + lane;
+        dg[r] = -1;
+        pos[r] = 0;
+        if (r >= rounds) continue;  // uniform
+        const int j = first + r * kWave + lane;
         const bool ok = j < n;
         const int d = (int)((lo[r] >> shift) & 0xffu);
         const uint64_t peers = match_digit(d, ok);
@@ -535,8 +542,8 @@ __global__ __launch_bounds__(kMidWaves * kWave) void ps_seg_mid(
       __syncthreads();
 #pragma unroll
       for (int r = 0; r < kMidRounds; ++r) {
-        const int j = wave * (kMidRounds * kWave) + r * kWave + lane;
-        if (j < n) {
+        const int j = first + r * kWave + lane;
+        if (r < rounds && j < n) {
           lo[r] = s_lo[j];
           v[r] = s_v[j];
         }
@@ -546,8 +553,8 @@ __global__ __launch_bounds__(kMidWaves * kWave) void ps_seg_mid(
     const uint64_t hi = (uint64_t)hi0 << 32;
 #pragma unroll
     for (int r = 0; r < kMidRounds; ++r) {
-      const int j = wave * (kMidRounds * kWave) + r * kWave + lane;
-      if (j < n) {
+      const int j = first + r * kWave + lane;
+      if (r < rounds && j < n) {
         keys[rg.x + j] = hi | lo[r];
         vals[rg.x + j] = v[r];
       }
