@@ -34,6 +34,10 @@ constexpr int kSortTile = kSortThreads * kSortItems;  // 3840 pairs
 constexpr int kWaveItems = kWave * kSortItems;        // 960 pairs per wave
 constexpr int kSortBlocks = 1024;
 constexpr int kRadix = 256;
+// Digit passes take 8-bit digits, or 9-bit ones (512 buckets) when that saves
+// a pass (18 tile bits at 9216x6912: two passes instead of three).
+constexpr int kMaxDigitBits = 9;
+constexpr int kMaxRadix = 1 << kMaxDigitBits;
 
 struct SortPlan {
   int n, ntiles, tiles_per_block;
@@ -49,7 +53,7 @@ __device__ __forceinline__ SortPlan sort_plan(const int* d_n, int n_host, int ma
   return p;
 }
 
-// A digit is `width` (1..8) key bits from `shift`, packed as
+// A digit is `width` (1..9) key bits from `shift`, packed as
 // shift | width << 8.  The last digit pass of a sort over [0, end_bit) takes
 // only the bits below end_bit, so higher key bits never affect the order.
 __host__ __device__ constexpr int digit_spec(int shift, int width) { return shift | (width << 8); }
@@ -66,19 +70,25 @@ __device__ __forceinline__ int tile_item(int wave, int r, int lane) {
 }
 
 // counts layout: [digit][block].  Every block histograms the same contiguous
-// run of tiles the downsweep will give it.
+// run of tiles the downsweep will give it.  DB: digit bits of the kernel
+// (8 or 9); a digit of fewer bits uses the low buckets.
+template <int DB>
 __global__ __launch_bounds__(kSortThreads) void ps_sort_upsweep(const uint64_t* __restrict__ keys,
                                                                 const int* __restrict__ d_n,
                                                                 int n_host, int max_pairs,
                                                                 int spec,
                                                                 uint32_t* __restrict__ counts) {
-  __shared__ uint32_t hist[4][kRadix];
+  constexpr int D = 1 << DB, DPT = D / kSortThreads;
+  __shared__ uint32_t hist[4][D];
   const SortPlan pl = sort_plan(d_n, n_host, max_pairs);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, b = blockIdx.x;
-  hist[0][tid] = 0;
-  hist[1][tid] = 0;
-  hist[2][tid] = 0;
-  hist[3][tid] = 0;
+#pragma unroll
+  for (int q = 0; q < DPT; ++q) {
+    hist[0][tid + q * kSortThreads] = 0;
+    hist[1][tid + q * kSortThreads] = 0;
+    hist[2][tid + q * kSortThreads] = 0;
+    hist[3][tid + q * kSortThreads] = 0;
+  }
   __syncthreads();
   const int t_begin = b * pl.tiles_per_block;
   int t_end = t_begin + pl.tiles_per_block;
@@ -97,7 +107,11 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_upsweep(const uint64_t* 
       if (d[r] >= 0) atomicAdd(&hist[wave][d[r]], 1u);
   }
   __syncthreads();
-  counts[(size_t)tid * kSortBlocks + b] = hist[0][tid] + hist[1][tid] + hist[2][tid] + hist[3][tid];
+#pragma unroll
+  for (int q = 0; q < DPT; ++q) {
+    const int dd = tid + q * kSortThreads;
+    counts[(size_t)dd * kSortBlocks + b] = hist[0][dd] + hist[1][dd] + hist[2][dd] + hist[3][dd];
+  }
 }
 
 // One workgroup per digit: exclusive scan of counts[d][0..kSortBlocks) into
@@ -151,18 +165,20 @@ __device__ __forceinline__ void load_tile(const uint64_t* __restrict__ keys,
 // digit run is written contiguously.  The next tile's keys are loaded into
 // registers while the current one is scanned, staged and written (about 210
 // VGPRs: two workgroups per CU, no spills).
+template <int DB>
 __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
     const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint64_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, const int* __restrict__ d_n,
     int n_host, int max_pairs, int spec, const uint32_t* __restrict__ offsets,
     const uint32_t* __restrict__ totals) {
+  constexpr int D = 1 << DB, DPT = D / kSortThreads;
   __shared__ uint64_t skey[kSortTile];
   __shared__ uint32_t sval[kSortTile];
   // wcnt: per-wave digit counters while ranking, then (same slots) the
   // tile-local start of each (wave, digit) run; wcnt[0][d] is digit d's
-  // tile-local start.
-  __shared__ uint32_t wcnt[4][kRadix];
-  __shared__ uint32_t gbase[kRadix];
+  // tile-local start.  Thread t owns digits t*DPT .. t*DPT + DPT - 1.
+  __shared__ uint32_t wcnt[4][D];
+  __shared__ uint32_t gbase[D];
   __shared__ int scratch[4];
 
   const SortPlan pl = sort_plan(d_n, n_host, max_pairs);
@@ -172,13 +188,27 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
   if (t_end > pl.ntiles) t_end = pl.ntiles;
   if (t_begin >= t_end) return;  // uniform across the block
 
-  int dtotal;
-  const int dbase = block256_excl_scan((int)totals[tid], scratch, dtotal);
-  gbase[tid] = (uint32_t)dbase + offsets[(size_t)tid * kSortBlocks + b];
-  wcnt[0][tid] = 0;
-  wcnt[1][tid] = 0;
-  wcnt[2][tid] = 0;
-  wcnt[3][tid] = 0;
+  {
+    uint32_t tv[DPT];
+    int local = 0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      tv[q] = totals[tid * DPT + q];
+      local += (int)tv[q];
+    }
+    int dtotal;
+    uint32_t run = (uint32_t)block256_excl_scan(local, scratch, dtotal);
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const int dd = tid * DPT + q;
+      gbase[dd] = run + offsets[(size_t)dd * kSortBlocks + b];
+      run += tv[q];
+      wcnt[0][dd] = 0;
+      wcnt[1][dd] = 0;
+      wcnt[2][dd] = 0;
+      wcnt[3][dd] = 0;
+    }
+  }
   const uint64_t below = lanes_below(lane);
 
   uint64_t k[kSortItems];
@@ -197,7 +227,7 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
       dg[r] = d;
       uint64_t peers = ballot(d >= 0);
 #pragma unroll
-      for (int bit = 0; bit < 8; ++bit) {
+      for (int bit = 0; bit < DB; ++bit) {
         const bool set = d >= 0 && ((d >> bit) & 1);
         const uint64_t bb = ballot(set);
         peers &= set ? bb : ~bb;
@@ -218,14 +248,29 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
         (t + 1 < t_end) ? ((pl.n - ntile0 < kSortTile) ? (int)(pl.n - ntile0) : kSortTile) : 0;
     if (ntile_n > 0) load_tile(keys_in, vals_in, ntile0, ntile_n, wave, lane, kn, vn);
     __syncthreads();
-    const uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
-    const uint32_t tot = c0 + c1 + c2 + c3;
+    uint32_t c0[DPT], c1[DPT], c2[DPT], c3[DPT], tot[DPT];
+    int local = 0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const int dd = tid * DPT + q;
+      c0[q] = wcnt[0][dd];
+      c1[q] = wcnt[1][dd];
+      c2[q] = wcnt[2][dd];
+      c3[q] = wcnt[3][dd];
+      tot[q] = c0[q] + c1[q] + c2[q] + c3[q];
+      local += (int)tot[q];
+    }
     int tile_total;
-    const uint32_t ls = (uint32_t)block256_excl_scan((int)tot, scratch, tile_total);
-    wcnt[0][tid] = ls;
-    wcnt[1][tid] = ls + c0;
-    wcnt[2][tid] = ls + c0 + c1;
-    wcnt[3][tid] = ls + c0 + c1 + c2;
+    uint32_t ls = (uint32_t)block256_excl_scan(local, scratch, tile_total);
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const int dd = tid * DPT + q;
+      wcnt[0][dd] = ls;
+      wcnt[1][dd] = ls + c0[q];
+      wcnt[2][dd] = ls + c0[q] + c1[q];
+      wcnt[3][dd] = ls + c0[q] + c1[q] + c2[q];
+      ls += tot[q];
+    }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
@@ -246,11 +291,15 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
       }
     }
     __syncthreads();
-    gbase[tid] += tot;
-    wcnt[0][tid] = 0;
-    wcnt[1][tid] = 0;
-    wcnt[2][tid] = 0;
-    wcnt[3][tid] = 0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const int dd = tid * DPT + q;
+      gbase[dd] += tot[q];
+      wcnt[0][dd] = 0;
+      wcnt[1][dd] = 0;
+      wcnt[2][dd] = 0;
+      wcnt[3][dd] = 0;
+    }
     tile0 = ntile0;
     tile_n = ntile_n;
 #pragma unroll
@@ -690,11 +739,11 @@ inline size_t sort_layout(int cap, char* base, SortWs* w) {
   if (w) w->alt_vals = reinterpret_cast<uint32_t*>(base + off);
   off += align_up(c * sizeof(uint32_t), 256);
   if (w) w->counts = reinterpret_cast<uint32_t*>(base + off);
-  off += align_up((size_t)kRadix * kSortBlocks * sizeof(uint32_t), 256);
+  off += align_up((size_t)kMaxRadix * kSortBlocks * sizeof(uint32_t), 256);
   if (w) w->offsets = reinterpret_cast<uint32_t*>(base + off);
-  off += align_up((size_t)kRadix * kSortBlocks * sizeof(uint32_t), 256);
+  off += align_up((size_t)kMaxRadix * kSortBlocks * sizeof(uint32_t), 256);
   if (w) w->totals = reinterpret_cast<uint32_t*>(base + off);
-  off += align_up((size_t)kRadix * sizeof(uint32_t), 256);
+  off += align_up((size_t)kMaxRadix * sizeof(uint32_t), 256);
   if (w) w->seg_ranges = reinterpret_cast<int2*>(base + off);
   off += align_up((size_t)kSegMaxSegs * sizeof(int2), 256);
   if (w) w->big_count = reinterpret_cast<int*>(base + off);
@@ -714,28 +763,42 @@ inline bool use_segmented() {
   return !(e && e[0] == 'l');
 }
 
+template <int DB>
+void launch_digit_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout,
+                       int n_host, const int* d_n, int max_pairs, int spec, const SortWs& w,
+                       hipStream_t stream) {
+  hipLaunchKernelGGL(ps_sort_upsweep<DB>, dim3(kSortBlocks), dim3(kSortThreads), 0, stream, kin,
+                     d_n, n_host, max_pairs, spec, w.counts);
+  hipLaunchKernelGGL(ps_sort_scan, dim3(1 << DB), dim3(kSortThreads), 0, stream, w.counts,
+                     w.offsets, w.totals);
+  hipLaunchKernelGGL(ps_sort_downsweep<DB>, dim3(kSortBlocks), dim3(kSortThreads), 0, stream,
+                     kin, vin, kout, vout, d_n, n_host, max_pairs, spec, w.offsets, w.totals);
+}
+
+// Stable LSD passes over key bits [bit0, bit1): 8-bit digits, or 9-bit ones
+// when that needs fewer passes; the bits are spread evenly over the passes.
 int digit_passes(int n_host, const int* d_n, int max_pairs, int bit0, int bit1, const SortWs& w,
                  const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
                  uint32_t* vals_out, hipStream_t stream) {
-  const int passes = (bit1 - bit0 + 7) / 8;
+  const int bits = bit1 - bit0;
+  const int p8 = (bits + 7) / 8, p9 = (bits + kMaxDigitBits - 1) / kMaxDigitBits;
+  const int passes = p9 < p8 ? p9 : p8;
   const uint64_t* kin = keys_in;
   const uint32_t* vin = vals_in;
+  int lo = bit0;
   for (int p = 0; p < passes; ++p) {
     // Ping-pong so that the last pass lands in the caller's output.
     const bool to_out = ((passes - 1 - p) % 2) == 0;
     uint64_t* kout = to_out ? keys_out : w.alt_keys;
     uint32_t* vout = to_out ? vals_out : w.alt_vals;
-    const int lo = bit0 + 8 * p;
-    const int spec = digit_spec(lo, (bit1 - lo < 8) ? bit1 - lo : 8);
-    hipLaunchKernelGGL(ps_sort_upsweep, dim3(kSortBlocks), dim3(kSortThreads), 0, stream, kin,
-                       d_n, n_host, max_pairs, spec, w.counts);
+    const int width = (bit1 - lo + (passes - p) - 1) / (passes - p);  // <= 9
+    const int spec = digit_spec(lo, width);
+    if (width > 8)
+      launch_digit_pass<9>(kin, vin, kout, vout, n_host, d_n, max_pairs, spec, w, stream);
+    else
+      launch_digit_pass<8>(kin, vin, kout, vout, n_host, d_n, max_pairs, spec, w, stream);
     PS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(ps_sort_scan, dim3(kRadix), dim3(kSortThreads), 0, stream, w.counts,
-                       w.offsets, w.totals);
-    PS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(ps_sort_downsweep, dim3(kSortBlocks), dim3(kSortThreads), 0, stream, kin,
-                       vin, kout, vout, d_n, n_host, max_pairs, spec, w.offsets, w.totals);
-    PS_LAUNCH_CHECK();
+    lo += width;
     kin = kout;
     vin = vout;
   }

@@ -86,6 +86,24 @@ def test_sort_orders_only_bits_below_end_bit(emu):
         np.testing.assert_array_equal(k[:n], keys[order], err_msg=mode)
 
 
+@pytest.mark.parametrize("mode", ["segmented", "lsd"])
+def test_sort_with_nine_bit_digits(emu, mode, monkeypatch):
+    # 9216x6912 has 248832 tiles: 18 tile bits, two 9-bit digit passes.
+    if mode == "lsd":
+        monkeypatch.setenv("POLYSPLAT_SORT", "lsd")
+    rng = np.random.default_rng(17)
+    n = 30000
+    t = rng.integers(0, 248832, n).astype(np.uint64)
+    t[: n // 3] = rng.integers(0, 40, n // 3)  # some long tiles as well
+    d = rng.integers(0x3f000000, 0x42000000, n).astype(np.uint64)
+    keys = (t << np.uint64(32)) | d
+    vals = rng.permutation(n).astype(np.uint32)
+    k, v = _sort(emu, keys, vals, 9216, 6912)
+    rk, rv = O.sort_pairs(keys, vals)
+    np.testing.assert_array_equal(k[:n], rk)
+    np.testing.assert_array_equal(v[:n], rv)
+
+
 def _scene():
     return scenes.make_scene(3000, 10.0, [(0.8, -4.0, 0.5), (0.2, -1.8, 0.4)], seed=3)
 
