@@ -1,5 +1,5 @@
-// Tile renderers for gfx950: one 64-lane wave per 16x16 tile, four pixels of
-// one row per lane, front-to-back alpha blending in FP32.
+// Tile renderers for gfx950: one 64-lane wave per 16x16 tile, one pixel per
+// lane in each 8x8 quadrant, front-to-back alpha blending in FP32.
 //
 // Per pixel (render.cu:228-271): d = centre - pixel (integer pixel coords),
 //   power = log2(o) + c.a*dx^2 + c.b*dx*dy + c.c*dy^2  (conic pre-scaled by
@@ -114,11 +114,56 @@ __device__ __forceinline__ uint8_t encode(float x) {
   return (uint8_t)fminf(fmaxf(0.0f, x * 255.0f), 255.0f);
 }
 
+// Quadrant bit k (k = qy * 2 + qx) of a Gaussian: set when its alpha can
+// reach 2^kQuadSkipPower at some pixel of quadrant k (8x8 pixels) of the tile
+// whose first pixel is (tx0, ty0).  The power is w + q(d) with q a negative
+// semidefinite quadratic form, and q(d) <= lmax |d|^2 for lmax the eigenvalue
+// of q nearest zero (taken with a relative rounding allowance), so
+// w + lmax * dist^2(centre, quadrant) bounds the power over the quadrant.
+// A Gaussian left out of a quadrant would have added less than 2^-32 of
+// colour and left T unchanged in FP32.  The neutral padding Gaussian (power
+// -inf) has no bits; a NaN keeps every bit.
+constexpr float kQuadSkipPower = -32.0f;
+
+__device__ __forceinline__ float dist_to_span(float v, float lo, float hi) {
+  return v < lo ? lo - v : (v > hi ? v - hi : 0.0f);
+}
+
+__device__ __forceinline__ uint32_t quad_mask(const Feat& f, float tx0, float ty0) {
+  const float h = 0.5f * (f.a - f.c);
+  const float root = __builtin_sqrtf(h * h + 0.25f * f.b * f.b);
+  float lmax = 0.5f * (f.a + f.c) + root;
+  lmax += 1e-6f * (fabsf(f.a) + fabsf(f.b) + fabsf(f.c));
+  lmax = fminf(lmax, 0.0f);
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float x0 = tx0 + 8.0f * (float)(k & 1), y0 = ty0 + 8.0f * (float)(k >> 1);
+    const float ddx = dist_to_span(f.x, x0, x0 + 7.0f);
+    const float ddy = dist_to_span(f.y, y0, y0 + 7.0f);
+    const float bound = f.w + lmax * (ddx * ddx + ddy * ddy);
+    if (!(bound < kQuadSkipPower)) m |= 1u << k;
+  }
+  return m;
+}
+
+// Estimated VALU instructions of the two shading paths (per Gaussian for the
+// dense one, per live (Gaussian, quadrant) for the sparse one).
+constexpr int kDenseCost = 43;
+constexpr int kSparseCost = 14;
+constexpr int kSparseGroupCost = 2;
+
 // Shared per-tile body.  Returns nothing; writes the tile's pixels.
-// PERSIST: tile ids come from the queue; RANGES: ranges from tile_ranges,
-// else from a wave search over keys; GATHERED: features indexed by pair
-// position (no id indirection); CHECK: early-exit test cadence; DT: skip
-// pixels whose alpha is below 1/255.
+// GATHERED: features indexed by pair position (no id indirection); CHECK:
+// early-exit test cadence; DT: skip pixels whose alpha is below 1/255.
+//
+// Lane l holds one pixel in each 8x8 quadrant of the tile: (l % 8, l / 8)
+// plus (8 qx, 8 qy).  Every CHECK Gaussians of the list the wave picks the
+// cheaper of two paths from the group's quadrant masks: dense (each Gaussian
+// shades all four pixels of every lane; the per-row and per-column terms of
+// the quadratic form are shared) or sparse (for each quadrant, only the
+// Gaussians live there, one pixel per lane).  Both evaluate the same
+// expression per pixel, in list order, so the path never changes a result.
 template <int PREC, bool GATHERED, int CHECK, bool DT>
 __device__ __forceinline__ void render_tile(
     int tile, int start, int end, int W, int H, int gx, const uint32_t* __restrict__ order,
@@ -127,13 +172,9 @@ __device__ __forceinline__ void render_tile(
     float4 (*sA)[kBatch], float4 (*sB)[kBatch], float (*sC)[kBatch], int lane) {
   static_assert(kBatch % CHECK == 0, "a batch holds whole check groups");
   const int tx = tile % gx, ty = tile / gx;
-  // Lane l shades pixels x0..x0+3 of row y0 (4 lanes per row): the row
-  // terms of the quadratic form are computed once per Gaussian.
-  const int px0 = tx * kTile + 4 * (lane & 3);
-  const int py0 = ty * kTile + (lane >> 2);
-  const float pxf0 = (float)px0, pxf1 = (float)(px0 + 1), pxf2 = (float)(px0 + 2),
-              pxf3 = (float)(px0 + 3);
-  const float pyf = (float)py0;
+  const float tx0 = (float)(tx * kTile), ty0 = (float)(ty * kTile);
+  const float pxf[2] = {tx0 + (float)(lane & 7), tx0 + (float)(lane & 7) + 8.0f};
+  const float pyf[2] = {ty0 + (float)(lane >> 3), ty0 + (float)(lane >> 3) + 8.0f};
   float T[4] = {1.f, 1.f, 1.f, 1.f};
   float Cr[4] = {0.f, 0.f, 0.f, 0.f}, Cg[4] = {0.f, 0.f, 0.f, 0.f}, Cb[4] = {0.f, 0.f, 0.f, 0.f};
   const int n = end - start;
@@ -150,6 +191,12 @@ __device__ __forceinline__ void render_tile(
     sA[0][lane] = make_float4(pre.x, pre.y, pre.w, pre.a);
     sB[0][lane] = make_float4(pre.b, pre.c, pre.r, pre.g);
     sC[0][lane] = pre.bl;
+    uint64_t live[4];
+    {
+      const uint32_t m = quad_mask(pre, tx0, ty0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) live[q] = ballot((m >> q) & 1u);
+    }
     pre = neutral_feat();
     if (kBatch + lane < n) {
       const int idx = GATHERED ? start + kBatch + lane : (int)order[start + kBatch + lane];
@@ -161,33 +208,62 @@ __device__ __forceinline__ void render_tile(
       const int groups = (cnt + CHECK - 1) / CHECK;
       bool done = false;
       for (int g = 0; g < groups && !done; ++g) {
-        // Two Gaussians per loop trip: LDS addresses as immediate offsets and
-        // no per-Gaussian loop overhead, within 64 VGPRs (8 waves per SIMD).
+        const uint64_t gm = (CHECK >= 64 ? ~0ull : ((1ull << CHECK) - 1ull)) << (g * CHECK);
+        const uint64_t l0 = live[0] & gm, l1 = live[1] & gm, l2 = live[2] & gm,
+                       l3 = live[3] & gm;
+        const int nlive = popc64(l0) + popc64(l1) + popc64(l2) + popc64(l3);
+        if (kSparseCost * nlive + kSparseGroupCost * CHECK < kDenseCost * CHECK) {
+          // Sparse: per quadrant, its live Gaussians, one pixel per lane.
+          const uint64_t lq[4] = {l0, l1, l2, l3};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float px = pxf[q & 1], py = pyf[q >> 1];
+            uint64_t bits = lq[q];
+            while (bits) {
+              const int k = __builtin_ctzll(bits);
+              bits &= bits - 1;
+              const float4 A = sA[cur][k];
+              const float4 B = sB[cur][k];
+              const float bl = sC[cur][k];
+              // A = (x, y, w, a), B = (b, c, r, g)
+              const float dx = A.x - px, dy = A.y - py;
+              const float pw = fmaf(dx, fmaf(A.w, dx, B.x * dy), fmaf(B.y * dy, dy, A.z));
+              if (DT && pw < -7.994353375f) continue;
+              const float alpha = __builtin_amdgcn_exp2f(pw);
+              const float wgt = alpha * T[q];
+              Cr[q] = fmaf(B.z, wgt, Cr[q]);
+              Cg[q] = fmaf(B.w, wgt, Cg[q]);
+              Cb[q] = fmaf(bl, wgt, Cb[q]);
+              T[q] = T[q] - wgt;
+            }
+          }
+        } else {
+          // Dense: every Gaussian of the group on all four pixels.
 #pragma unroll 2
-        for (int j = 0; j < CHECK; ++j) {
-          const int k = g * CHECK + j;
-          const float4 A = sA[cur][k];
-          const float4 B = sB[cur][k];
-          const float bl = sC[cur][k];
-          // A = (x, y, w, a), B = (b, c, r, g).  Per pixel the power is
-          // dx * (a dx + b dy) + (w + c dy^2): three operations once the row
-          // terms are known.
-          const float dy = A.y - pyf;
-          const float bdy = B.x * dy;
-          const float ys = fmaf(B.y * dy, dy, A.z);
-          const float dx[4] = {A.x - pxf0, A.x - pxf1, A.x - pxf2, A.x - pxf3};
-          float pw[4];
+          for (int j = 0; j < CHECK; ++j) {
+            const int k = g * CHECK + j;
+            const float4 A = sA[cur][k];
+            const float4 B = sB[cur][k];
+            const float bl = sC[cur][k];
+            const float dy0 = A.y - pyf[0], dy1 = A.y - pyf[1];
+            const float bdy0 = B.x * dy0, bdy1 = B.x * dy1;
+            const float ys0 = fmaf(B.y * dy0, dy0, A.z), ys1 = fmaf(B.y * dy1, dy1, A.z);
+            const float dx0 = A.x - pxf[0], dx1 = A.x - pxf[1];
+            float pw[4];
+            pw[0] = fmaf(dx0, fmaf(A.w, dx0, bdy0), ys0);
+            pw[1] = fmaf(dx1, fmaf(A.w, dx1, bdy0), ys0);
+            pw[2] = fmaf(dx0, fmaf(A.w, dx0, bdy1), ys1);
+            pw[3] = fmaf(dx1, fmaf(A.w, dx1, bdy1), ys1);
 #pragma unroll
-          for (int p = 0; p < 4; ++p) pw[p] = fmaf(dx[p], fmaf(A.w, dx[p], bdy), ys);
-#pragma unroll
-          for (int p = 0; p < 4; ++p) {
-            if (DT && pw[p] < -7.994353375f) continue;
-            const float alpha = __builtin_amdgcn_exp2f(pw[p]);
-            const float wgt = alpha * T[p];
-            Cr[p] = fmaf(B.z, wgt, Cr[p]);
-            Cg[p] = fmaf(B.w, wgt, Cg[p]);
-            Cb[p] = fmaf(bl, wgt, Cb[p]);
-            T[p] = T[p] - wgt;
+            for (int p = 0; p < 4; ++p) {
+              if (DT && pw[p] < -7.994353375f) continue;
+              const float alpha = __builtin_amdgcn_exp2f(pw[p]);
+              const float wgt = alpha * T[p];
+              Cr[p] = fmaf(B.z, wgt, Cr[p]);
+              Cg[p] = fmaf(B.w, wgt, Cg[p]);
+              Cb[p] = fmaf(bl, wgt, Cb[p]);
+              T[p] = T[p] - wgt;
+            }
           }
         }
         // Saturation test after every CHECK Gaussians of the tile (the
@@ -202,6 +278,11 @@ __device__ __forceinline__ void render_tile(
       sA[cur][lane] = make_float4(pre.x, pre.y, pre.w, pre.a);
       sB[cur][lane] = make_float4(pre.b, pre.c, pre.r, pre.g);
       sC[cur][lane] = pre.bl;
+      {
+        const uint32_t m = quad_mask(pre, tx0, ty0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) live[q] = ballot((m >> q) & 1u);
+      }
       pre = neutral_feat();
       const int nb = b0 + 2 * kBatch + lane;
       if (nb < n) {
@@ -213,7 +294,7 @@ __device__ __forceinline__ void render_tile(
   }
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
-    const int px = px0 + p, py = py0;
+    const int px = tx * kTile + 8 * (p & 1) + (lane & 7), py = ty * kTile + 8 * (p >> 1) + (lane >> 3);
     if (px < W && py < H) {
       uint8_t* o = out + ((size_t)py * W + px) * 3;
       if (n > 0) {
@@ -232,7 +313,7 @@ __device__ __forceinline__ void render_tile(
 
 // One workgroup per tile.
 template <int PREC, int CHECK>
-__global__ __launch_bounds__(kWave, 8) void ps_render_plain(
+__global__ __launch_bounds__(kWave, 6) void ps_render_plain(
     const uint64_t* __restrict__ keys, const int* __restrict__ d_n, int n_host, int W, int H,
     int gx, const uint32_t* __restrict__ order, const float2* __restrict__ xy,
     const float4* __restrict__ rgbd, const float4* __restrict__ con,
@@ -251,7 +332,7 @@ __global__ __launch_bounds__(kWave, 8) void ps_render_plain(
 
 // Persistent: tiles from the queue.  RANGES: read tile_ranges, else search.
 template <int PREC, bool RANGES, bool GATHERED, int CHECK, bool DT>
-__global__ __launch_bounds__(kWave, 8) void ps_render_persistent(
+__global__ __launch_bounds__(kWave, 6) void ps_render_persistent(
     const uint64_t* __restrict__ keys, const int* __restrict__ d_n, int n_host,
     const int2* __restrict__ ranges, int W, int H, int gx, int tiles,
     const uint32_t* __restrict__ order, const float2* __restrict__ xy,
@@ -296,7 +377,7 @@ __global__ __launch_bounds__(kWave, 8) void ps_render_persistent(
 constexpr int kMaxBatch = PS_MAX_BATCH_FRAMES;
 
 template <int PREC, int CHECK>
-__global__ __launch_bounds__(kWave, 8) void ps_render_persistent_batch(
+__global__ __launch_bounds__(kWave, 6) void ps_render_persistent_batch(
     const FrameDev* __restrict__ frames, int nf, const int2* __restrict__ ranges, int tiles,
     const uint32_t* __restrict__ order, unsigned* __restrict__ queue) {
   __shared__ float4 sA[2][kBatch];

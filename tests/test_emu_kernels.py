@@ -173,6 +173,26 @@ def test_renderer_pads_partial_groups_neutrally(emu):
     assert np.abs(out.astype(int) - ref2["image"].astype(int)).max() <= 1
 
 
+def test_renderer_sparse_and_dense_groups_match_the_oracle(emu):
+    # A city crop: sub-pixel splats (the sparse quadrant path) over a few
+    # large ones (the dense path), and a mixed-size scene; images within one
+    # 8-bit level of the oracle, with almost every pixel identical.
+    city = scenes.make_config_scene("c4", seed=4, P=300_000)
+    cam = scenes.config_camera("c4")
+    cam.width, cam.height = 320, 240
+    mixed = scenes.make_scene(6000, 8.0, [(0.7, -4.5, 0.5), (0.25, -2.2, 0.4), (0.05, -0.8, 0.3)],
+                              seed=13)
+    for sc, spec in ((city, cam), (mixed, scenes.make_camera(200, 136, 8.0, distance=10.0))):
+        ref = oracle_frame(sc, spec, sh_type="fp16", feat_prec="fp16", bg=(0.1, 0.0, 0.3),
+                           check_every=8, fp64=False)
+        f = HF.HostFrame(emu, sc, spec, capacity=ref["pre"]["M"] + 64, garbage=True, seed=9)
+        st, n, img = f.forward((0.1, 0.0, 0.3))
+        capi.check_with(emu, st)
+        assert n == ref["pre"]["M"]
+        d = np.abs(img.astype(int) - ref["image"].astype(int))
+        assert d.max() <= 1 and (d > 0).mean() < 1e-3, (d.max(), (d > 0).mean())
+
+
 # ------------------------------------------------------------------ key emission
 def _emission_scene():
     # Mostly small splats plus a few that cover hundreds of tiles: most
