@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c3")
+    ap.add_argument("--scene-gaussians", type=int, default=0,
+                    help="override the config's Gaussian count (profiling other sizes)")
+    ap.add_argument("--resolution", default="",
+                    help="override the config's WxH (profiling other sizes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--paths", default="fp16_persistent,fp32_plain",
@@ -138,10 +142,16 @@ def main():
     dev = torch.device("cuda:0")
     cfg = scenes.CONFIGS[args.config]
     spec = scenes.config_camera(args.config)
+    if args.resolution:
+        w_, h_ = (int(x) for x in args.resolution.lower().split("x"))
+        scale = w_ / spec.width
+        spec.width, spec.height = w_, h_
+        spec.focal_x, spec.focal_y = spec.focal_x * scale, spec.focal_y * scale
     W, H = spec.width, spec.height
     T = ((W + 15) // 16) * ((H + 15) // 16)
     t0 = time.time()
-    arrays = scenes.make_config_scene(args.config, seed=1, device=dev)
+    arrays = scenes.make_config_scene(args.config, seed=1, device=dev,
+                                      P=args.scene_gaussians or None)
     torch.cuda.synchronize()
     log(f"scene {args.config}: P={arrays.num_vertex} {W}x{H} generated in {time.time() - t0:.1f}s")
     scene = polysplat.Scene.from_arrays(arrays, dev)

@@ -654,17 +654,30 @@ __global__ __launch_bounds__(kSortThreads) void ps_seg_big(
       wcnt[2][tid] = 0;
       wcnt[3][tid] = 0;
       __syncthreads();
+      // Chunk c + 1 is loaded while chunk c is ranked and written.
+      uint64_t k[kPer];
+      uint32_t v[kPer];
+#pragma unroll
+      for (int q = 0; q < kPer; ++q) {
+        const int i = wave * (kWave * kPer) + q * kWave + lane;
+        k[q] = i < n ? sk[i] : 0ull;
+        v[q] = i < n ? sv[i] : 0u;
+      }
       for (int c0 = 0; c0 < n; c0 += kChunk) {
-        uint64_t k[kPer];
-        uint32_t v[kPer], pos[kPer];
+        uint32_t pos[kPer];
         int dg[kPer];
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
           const int i = c0 + wave * (kWave * kPer) + q * kWave + lane;
-          const bool ok = i < n;
-          k[q] = ok ? sk[i] : 0ull;
-          v[q] = ok ? sv[i] : 0u;
-          dg[q] = ok ? digit_of(k[q], spec) : -1;
+          dg[q] = i < n ? digit_of(k[q], spec) : -1;
+        }
+        uint64_t kn[kPer];
+        uint32_t vn[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+          const int i = c0 + kChunk + wave * (kWave * kPer) + q * kWave + lane;
+          kn[q] = i < n ? sk[i] : 0ull;
+          vn[q] = i < n ? sv[i] : 0u;
         }
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
@@ -702,6 +715,11 @@ __global__ __launch_bounds__(kSortThreads) void ps_seg_big(
         wcnt[1][tid] = 0;
         wcnt[2][tid] = 0;
         wcnt[3][tid] = 0;
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+          k[q] = kn[q];
+          v[q] = vn[q];
+        }
         __syncthreads();
       }
       // the pass's writes must be visible to the next pass's reads
@@ -866,7 +884,9 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
                      vals_out, reinterpret_cast<const int2*>(ranges), w.big_list, w.big_count,
                      big_cap, w.big_list2, w.big_count + 1);
   PS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(ps_seg_big, dim3(512), dim3(kSortThreads), 0, stream, keys_out, vals_out,
+  // One workgroup per listed segment at a time; enough of them to fill the
+  // device (about seven per CU fit), since each walks its segment serially.
+  hipLaunchKernelGGL(ps_seg_big, dim3(2048), dim3(kSortThreads), 0, stream, keys_out, vals_out,
                      w.alt_keys, w.alt_vals, reinterpret_cast<const int2*>(ranges), w.big_list2,
                      w.big_count + 1, big_cap);
   PS_LAUNCH_CHECK();
