@@ -317,8 +317,9 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
 // by tile (segment) and keep their input order inside each segment; what is
 // left is a stable sort of every segment by the low 32 bits.  That needs no
 // global passes: a wave sorts a segment of up to kSegCap pairs in its own
-// LDS (four 8-bit counting passes); larger segments are listed and a
-// workgroup sorts each with four counting passes through global memory.
+// LDS (four 8-bit counting passes); larger segments are listed and sorted
+// by a workgroup each: in registers and LDS up to kLargeCap pairs, beyond
+// that with four counting passes through global memory.
 // The result equals the full LSD sort bit for bit: sorting by (high, low)
 // stably is sorting by the whole key stably.
 
@@ -469,7 +470,7 @@ __global__ __launch_bounds__(kSegWaves * kWave) void ps_seg_small(
 // its items against its own LDS digit counters (ballot match), a block scan
 // over (digit, wave) turns the counts into starts, and the pairs are
 // scattered through LDS and read back in the new order.  Other listed
-// segments go on to the second list for ps_seg_big.
+// segments go on to the second list for ps_seg_large.
 constexpr int kMidWaves = 8;
 constexpr int kMidRounds = 16;
 constexpr int kMidCap = kMidWaves * kMidRounds * kWave;  // 8192
@@ -611,6 +612,166 @@ __global__ __launch_bounds__(kMidWaves * kWave) void ps_seg_mid(
   }
 }
 
+// Listed segments of kMidCap + 1 .. kLargeCap pairs whose keys agree above
+// the sorted bits: a 1024-thread workgroup per segment, as ps_seg_mid but
+// with items held compactly so that 32 rounds fit in registers: the low key
+// word, and the item's index in the segment (15 bits) packed with its
+// destination in the current pass (15 bits).  The wave's last rounds are
+// padded up to whole rounds with items whose low word is the segment's
+// largest; a stable sort places them after every real item (their indices
+// are larger), so the passes need no per-item masks.  Each pass scatters the
+// low words through one LDS buffer and reads them back, then does the same
+// with the indices.  At the end the values are gathered by index through
+// LDS (every read of the segment's old values completes before the first
+// write) and written with the sorted keys.  Other listed segments go on to
+// the third list for ps_seg_big.
+constexpr int kLargeWaves = 16;
+constexpr int kLargeRounds = 32;
+constexpr int kLargeCap = kLargeWaves * kLargeRounds * kWave;  // 32768
+
+__global__ __launch_bounds__(kLargeWaves * kWave) void ps_seg_large(
+    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, const int2* __restrict__ ranges,
+    const int* __restrict__ big_list, const int* __restrict__ big_count, int big_cap,
+    int* __restrict__ big_list3, int* __restrict__ big_count3) {
+  constexpr int kThreads = kLargeWaves * kWave;
+  __shared__ uint32_t s_buf[kLargeCap];
+  __shared__ uint32_t wcnt[kLargeWaves][kRadix];
+  __shared__ int s_wsum[kLargeWaves];
+  __shared__ uint32_t s_diff, s_max;
+  __shared__ int s_mixed;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t below = lanes_below(lane);
+  const int nbig = min(*big_count, big_cap);
+  for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+    const int seg = big_list[b];
+    const int2 rg = ranges[seg];
+    const int n = rg.y - rg.x;
+    if (n > kLargeCap) {  // uniform
+      if (tid == 0) big_list3[atomicAdd(big_count3, 1)] = seg;
+      continue;
+    }
+    const uint64_t k0 = keys[rg.x];
+    const uint32_t hi0 = (uint32_t)(k0 >> 32);
+    if (tid == 0) {
+      s_diff = 0u;
+      s_max = 0u;
+      s_mixed = 0;
+    }
+    __syncthreads();
+    bool same = true;
+    uint32_t diff = 0, mx = 0;
+    for (int i = tid; i < n; i += kThreads) {
+      const uint64_t k = keys[rg.x + i];
+      same = same && (uint32_t)(k >> 32) == hi0;
+      s_buf[i] = (uint32_t)k;
+      diff |= (uint32_t)k ^ (uint32_t)k0;
+      mx = max(mx, (uint32_t)k);
+    }
+    if (!same) s_mixed = 1;
+    if (diff) atomicOr(&s_diff, diff);
+    if (mx) atomicMax(&s_max, mx);
+    __syncthreads();
+    const bool mixed = s_mixed != 0;
+    const uint32_t all_diff = s_diff, pad = s_max;
+    __syncthreads();  // s_mixed / s_diff / s_max are rewritten for the next segment
+    if (mixed) {  // uniform
+      if (tid == 0) big_list3[atomicAdd(big_count3, 1)] = seg;
+      continue;
+    }
+    const int rounds = (n + kThreads - 1) / kThreads;  // uniform
+    // Item r of this lane is item mine + 64 r of the segment.
+    const int mine = wave * rounds * kWave + lane;
+    const int lim = n - mine;  // round r holds a real item when 64 r < lim
+    uint32_t* const my_buf = s_buf + mine;
+    // lo: low key word; pk: index in the segment (bits 0..14), destination of
+    // the current pass (bits 16..30).
+    uint32_t lo[kLargeRounds], pk[kLargeRounds];
+#pragma unroll
+    for (int r = 0; r < kLargeRounds; ++r) {
+      pk[r] = (uint32_t)(mine + r * kWave);
+      lo[r] = r < rounds && r * kWave < lim ? my_buf[r * kWave] : pad;
+    }
+    __syncthreads();
+    for (int p = 0; p < 4; ++p) {
+      const int shift = 8 * p;
+      if (((all_diff >> shift) & 0xffu) == 0) continue;  // uniform: constant digit
+      for (int i = lane; i < kRadix; i += kWave) wcnt[wave][i] = 0;
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int r = 0; r < kLargeRounds; ++r) {
+        if (r >= rounds) continue;  // uniform
+        const int d = (int)((lo[r] >> shift) & 0xffu);
+        const uint64_t peers = match_digit(d, true);
+        const uint32_t rank = (uint32_t)popc64(peers & below);
+        const uint32_t base = wcnt[wave][d];
+        __builtin_amdgcn_wave_barrier();
+        if (rank == 0) wcnt[wave][d] = base + (uint32_t)popc64(peers);
+        __builtin_amdgcn_wave_barrier();
+        pk[r] = (pk[r] & 0x7fffu) | ((base + rank) << 16);
+      }
+      __syncthreads();
+      // Digit starts over (digit, wave): threads 0..255 own one digit each.
+      uint32_t c[kLargeWaves];
+      uint32_t tot = 0;
+      if (tid < kRadix) {
+#pragma unroll
+        for (int w = 0; w < kLargeWaves; ++w) {
+          c[w] = wcnt[w][tid];
+          tot += c[w];
+        }
+      }
+      const int incl = wave_incl_scan((int)tot, lane);
+      if (lane == 63) s_wsum[wave] = incl;
+      __syncthreads();
+      if (tid < kRadix) {
+        int before = 0;
+        for (int w = 0; w < wave; ++w) before += s_wsum[w];
+        uint32_t start = (uint32_t)(before + incl) - tot;
+#pragma unroll
+        for (int w = 0; w < kLargeWaves; ++w) {
+          wcnt[w][tid] = start;
+          start += c[w];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < kLargeRounds; ++r) {
+        if (r >= rounds) continue;  // uniform
+        const uint32_t at = wcnt[wave][(lo[r] >> shift) & 0xffu] + (pk[r] >> 16);
+        pk[r] = (pk[r] & 0x7fffu) | (at << 16);
+        s_buf[at] = lo[r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < kLargeRounds; ++r)
+        if (r < rounds) lo[r] = my_buf[r * kWave];
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < kLargeRounds; ++r)
+        if (r < rounds) s_buf[pk[r] >> 16] = pk[r] & 0x7fffu;
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < kLargeRounds; ++r)
+        if (r < rounds) pk[r] = my_buf[r * kWave];
+      __syncthreads();
+    }
+    const uint64_t hi = (uint64_t)hi0 << 32;
+    uint64_t* const my_keys = keys + rg.x + mine;
+#pragma unroll
+    for (int r = 0; r < kLargeRounds; ++r) {
+      if (r < rounds && r * kWave < lim) {
+        my_keys[r * kWave] = hi | lo[r];
+        my_buf[r * kWave] = pk[r] & 0x7fffu;
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += kThreads) s_buf[i] = vals[rg.x + (int)s_buf[i]];
+    __syncthreads();
+    for (int i = tid; i < n; i += kThreads) vals[rg.x + i] = s_buf[i];
+    __syncthreads();  // s_buf is rewritten for the next segment
+  }
+}
+
 // Listed segments: a workgroup per segment, four stable counting passes by
 // the low 32 bits through global memory (keys <-> tmp, ending in keys).
 // Chunks of 1024 pairs: wave w ranks items [256w, 256w + 256) of the chunk
@@ -737,8 +898,9 @@ struct SortWs {
   uint32_t* totals;
   int2* seg_ranges;  // segment ranges when the caller gives none
   int* big_list;     // segments longer than kSegCap (or mixed high words)
-  int* big_count;    // [0]: entries of big_list; [1]: entries of big_list2
-  int* big_list2;    // segments for ps_seg_big
+  int* big_count;    // [0]: entries of big_list; [1]: of big_list2; [2]: of big_list3
+  int* big_list2;    // segments for ps_seg_large
+  int* big_list3;    // segments for ps_seg_big
 };
 
 // Segments (values of the sorted high bits) the internal range table holds.
@@ -769,6 +931,8 @@ inline size_t sort_layout(int cap, char* base, SortWs* w) {
   if (w) w->big_list = reinterpret_cast<int*>(base + off);
   off += align_up(big_list_cap(c) * sizeof(int), 256);
   if (w) w->big_list2 = reinterpret_cast<int*>(base + off);
+  off += align_up(big_list_cap(c) * sizeof(int), 256);
+  if (w) w->big_list3 = reinterpret_cast<int*>(base + off);
   off += align_up(big_list_cap(c) * sizeof(int), 256);
   return off;
 }
@@ -870,7 +1034,7 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
                       vals_out, stream);
   if (st != PS_OK) return st;
   st = tile_ranges_scan_launch(n_host, d_n, max_pairs, segs, keys_out, ranges, stream, mask,
-                               w.big_count, 2);
+                               w.big_count, 3);
   if (st != PS_OK) return st;
   if (d_n == nullptr && n_host == 0) return PS_OK;
   const int big_cap = (int)big_list_cap((size_t)(max_pairs > 0 ? max_pairs : 1));
@@ -884,11 +1048,16 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
                      vals_out, reinterpret_cast<const int2*>(ranges), w.big_list, w.big_count,
                      big_cap, w.big_list2, w.big_count + 1);
   PS_LAUNCH_CHECK();
+  // One workgroup per CU (the LDS buffer takes most of a CU's 160 KiB).
+  hipLaunchKernelGGL(ps_seg_large, dim3(256), dim3(kLargeWaves * kWave), 0, stream, keys_out,
+                     vals_out, reinterpret_cast<const int2*>(ranges), w.big_list2,
+                     w.big_count + 1, big_cap, w.big_list3, w.big_count + 2);
+  PS_LAUNCH_CHECK();
   // One workgroup per listed segment at a time; enough of them to fill the
   // device (about seven per CU fit), since each walks its segment serially.
   hipLaunchKernelGGL(ps_seg_big, dim3(2048), dim3(kSortThreads), 0, stream, keys_out, vals_out,
-                     w.alt_keys, w.alt_vals, reinterpret_cast<const int2*>(ranges), w.big_list2,
-                     w.big_count + 1, big_cap);
+                     w.alt_keys, w.alt_vals, reinterpret_cast<const int2*>(ranges), w.big_list3,
+                     w.big_count + 2, big_cap);
   PS_LAUNCH_CHECK();
   return PS_OK;
 }

@@ -182,6 +182,14 @@ template <typename T>
 inline T atomicOr(T* p, T v) {
   return std::atomic_ref<T>(*p).fetch_or(v);
 }
+template <typename T>
+inline T atomicMax(T* p, T v) {
+  std::atomic_ref<T> a(*p);
+  T cur = a.load();
+  while (cur < v && !a.compare_exchange_weak(cur, v)) {
+  }
+  return cur;
+}
 
 // Scoped atomics, sequentially consistent on the host (stronger than the
 // orders the kernels ask for).

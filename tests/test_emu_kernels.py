@@ -67,6 +67,30 @@ def test_sort_segments_around_the_lds_limits(emu, mode, monkeypatch):
     assert (k[n:] == np.uint64(2**64 - 1)).all()
 
 
+@pytest.mark.parametrize("threads", [1, 8])
+def test_sort_long_segments_with_full_depth_words(emu, threads, monkeypatch):
+    # Segments past the 512-thread sort (8192) up to and past the 1024-thread
+    # one (32768), with depths over the full 32 bits (every digit pass runs)
+    # and many ties at the segment's largest depth (the padding value).
+    monkeypatch.setenv("EMU_THREADS", str(threads))
+    counts = [8193, 20000, 32767, 32768, 32769, 40000]
+    rng = np.random.default_rng(123)
+    t = np.repeat(np.arange(len(counts), dtype=np.uint64), counts)
+    rng.shuffle(t)
+    d = rng.integers(0, 2**32, len(t), dtype=np.uint64)
+    top = rng.random(len(t)) < 0.05
+    d[top] = np.uint64(0xFFFFFFF0)  # the largest depth of each segment, repeated
+    d[rng.random(len(t)) < 0.05] = np.uint64(0)
+    keys = (t << np.uint64(32)) | d
+    vals = rng.permutation(len(keys)).astype(np.uint32)
+    k, v = _sort(emu, keys, vals, 64, 64, cap=len(keys) + 5)
+    rk, rv = O.sort_pairs(keys, vals)
+    n = len(keys)
+    np.testing.assert_array_equal(k[:n], rk)
+    np.testing.assert_array_equal(v[:n], rv)
+    assert (k[n:] == np.uint64(2**64 - 1)).all()
+
+
 def test_sort_orders_only_bits_below_end_bit(emu):
     rng = np.random.default_rng(5)
     n = 6000

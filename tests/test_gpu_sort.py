@@ -136,6 +136,30 @@ def test_sort_segment_sizes_around_the_lds_limit(gpu, mode, monkeypatch):
     np.testing.assert_array_equal(v[:n], rv)
 
 
+@pytest.mark.parametrize("mode", ["segmented", "lsd"])
+def test_sort_long_segments_with_full_depth_words(gpu, mode, monkeypatch):
+    # Segments from past the 512-thread LDS sort (8192) to past the
+    # 1024-thread one (32768), depths over all 32 bits (every digit pass
+    # runs), with ties at the segment's largest depth (the padding value).
+    if mode == "lsd":
+        monkeypatch.setenv("POLYSPLAT_SORT", "lsd")
+    counts = [8193, 20000, 32767, 32768, 32769, 40000, 100000]
+    rng = np.random.default_rng(123)
+    t = np.repeat(np.arange(len(counts), dtype=np.uint64), counts)
+    rng.shuffle(t)
+    d = rng.integers(0, 2**32, len(t), dtype=np.uint64)
+    d[rng.random(len(t)) < 0.05] = np.uint64(0xFFFFFFF0)
+    d[rng.random(len(t)) < 0.05] = np.uint64(0)
+    keys = (t << np.uint64(32)) | d
+    n = len(keys)
+    vals = rng.permutation(n).astype(np.uint32)
+    say("sort long segments", mode, n)
+    k, v = _run_sort(keys, vals, 64, 64, cap=n + 5)
+    rk, rv = O.sort_pairs(keys, vals)
+    np.testing.assert_array_equal(k[:n], rk)
+    np.testing.assert_array_equal(v[:n], rv)
+
+
 def test_sort_ignores_key_bits_above_end_bit(gpu):
     # The sort orders by bits [0, 32 + bit_length(tiles)) only, like the
     # reference's radix sort; higher bits ride along.  Segments whose keys
