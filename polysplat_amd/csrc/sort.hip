@@ -316,14 +316,14 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
 // After the digit passes over the tile bits [32, end_bit), pairs are grouped
 // by tile (segment) and keep their input order inside each segment; what is
 // left is a stable sort of every segment by the low 32 bits.  That needs no
-// global passes: a wave sorts a segment of up to kSegCap pairs in its own
-// LDS (four 8-bit counting passes); larger segments are listed and sorted
-// by a workgroup each: in registers and LDS up to kLargeCap pairs, beyond
-// that with four counting passes through global memory.
-// The result equals the full LSD sort bit for bit: sorting by (high, low)
-// stably is sorting by the whole key stably.
+// global passes for segments of up to 32768 pairs: a wave (up to 896
+// pairs), or a workgroup of 8 waves (8192) or 16 waves (32768) sorts each in
+// registers and LDS; longer segments are listed and a workgroup sorts each
+// with four counting passes through global memory.  The result equals the
+// full LSD sort bit for bit: sorting by (high, low) stably is sorting by the
+// whole key stably.
 
-constexpr int kSegCap = 896;    // pairs per wave-sorted segment (LDS stays under 64 KiB)
+constexpr int kSegCap = 896;    // pairs per wave-sorted segment
 constexpr int kSegWaves = 4;    // waves per workgroup of the small-segment kernel
 
 // Lanes of the wave whose 8-bit digit equals this lane's (valid lanes only).
@@ -340,7 +340,7 @@ __device__ __forceinline__ uint64_t match_digit(int d, bool ok) {
 
 // Each wave takes segments seg = wave id, wave id + total waves, ...  A
 // segment longer than kSegCap, or whose keys' high words differ (bits above
-// the sorted ones), goes to the list for ps_seg_big.  The wave keeps its
+// the sorted ones), goes to the list for the workgroup kernels.  The wave keeps its
 // segment in registers (item j in round j / 64 of lane j % 64) and runs one
 // stable counting pass per 8-bit digit of the low word: rank every item
 // among the segment's items with the same digit (ballot match per round,
@@ -462,192 +462,60 @@ __global__ __launch_bounds__(kSegWaves * kWave) void ps_seg_small(
   }
 }
 
-// Listed segments of up to kMidCap pairs whose keys agree above the sorted
-// bits: a 512-thread workgroup per segment holds it in registers (wave w
-// takes the contiguous chunk [w*c, w*c + c), c = n / 8 rounded up to whole
-// rounds of 64; item w*c + r*64 + l is round r of lane l) and runs one stable
-// counting pass per non-constant 8-bit digit of the low word: each wave ranks
-// its items against its own LDS digit counters (ballot match), a block scan
-// over (digit, wave) turns the counts into starts, and the pairs are
-// scattered through LDS and read back in the new order.  Other listed
-// segments go on to the second list for ps_seg_large.
+// Segments whose keys agree above the sorted bits, sorted by one workgroup
+// of W waves each (ps_seg_group<W, RD>: up to W * RD * 64 pairs).
+// Wave w takes the contiguous chunk of `rounds` rounds of 64 items starting at
+// item w * rounds * 64 (rounds = n / (64 W) rounded up); items are held
+// compactly in registers: the low key word, and the item's index in the
+// segment (15 bits) packed with its destination in the current pass (15
+// bits).  The wave's last rounds are padded up to whole rounds with items
+// whose low word is the segment's largest; a stable sort places them after
+// every real item (their indices are larger), so the passes need no per-item
+// masks.  One stable counting pass per non-constant 8-bit digit of the low
+// word: each wave ranks its items against its own LDS digit counters (ballot
+// match), a block scan over (digit, wave) turns the counts into starts, and
+// the low words are scattered through one LDS buffer and read back in the
+// new order, then the indices the same way.  At the end the values are
+// gathered by index through LDS (every read of the segment's old values
+// completes before the first write) and written with the sorted keys.
+// Longer (or mixed) segments go on to the next list: 8 waves x 16 rounds
+// (8192 pairs) first, then 16 waves x 32 rounds (32768, most of a CU's
+// LDS), then ps_seg_big.
 constexpr int kMidWaves = 8;
 constexpr int kMidRounds = 16;
-constexpr int kMidCap = kMidWaves * kMidRounds * kWave;  // 8192
-
-__global__ __launch_bounds__(kMidWaves * kWave) void ps_seg_mid(
-    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, const int2* __restrict__ ranges,
-    const int* __restrict__ big_list, const int* __restrict__ big_count, int big_cap,
-    int* __restrict__ big_list2, int* __restrict__ big_count2) {
-  __shared__ uint32_t s_lo[kMidCap];
-  __shared__ uint32_t s_v[kMidCap];
-  __shared__ uint32_t wcnt[kMidWaves][kRadix];
-  __shared__ int s_wsum[kMidWaves];
-  __shared__ uint32_t s_diff;
-  __shared__ int s_mixed;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint64_t below = lanes_below(lane);
-  const int nbig = min(*big_count, big_cap);
-  for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
-    const int seg = big_list[b];
-    const int2 rg = ranges[seg];
-    const int n = rg.y - rg.x;
-    if (n > kMidCap) {  // uniform
-      if (tid == 0) big_list2[atomicAdd(big_count2, 1)] = seg;
-      continue;
-    }
-    const uint64_t k0 = keys[rg.x];
-    const uint32_t hi0 = (uint32_t)(k0 >> 32);
-    const int rounds = (n + kMidWaves * kWave - 1) / (kMidWaves * kWave);  // uniform, <= 16
-    const int chunk = rounds * kWave;
-    const int first = wave * chunk;
-    if (tid == 0) {
-      s_diff = 0u;
-      s_mixed = 0;
-    }
-    __syncthreads();
-    uint32_t lo[kMidRounds], v[kMidRounds];
-    bool same = true;
-    uint32_t diff = 0;
-#pragma unroll
-    for (int r = 0; r < kMidRounds; ++r) {
-      const int j = first + r * kWave + lane;
-      lo[r] = 0;
-      v[r] = 0;
-      if (r < rounds && j < n) {
-        const uint64_t k = keys[rg.x + j];
-        same = same && (uint32_t)(k >> 32) == hi0;
-        lo[r] = (uint32_t)k;
-        v[r] = vals[rg.x + j];
-        diff |= lo[r] ^ (uint32_t)k0;
-      }
-    }
-    if (!same) s_mixed = 1;
-    if (diff) atomicOr(&s_diff, diff);
-    __syncthreads();
-    const bool mixed = s_mixed != 0;
-    const uint32_t all_diff = s_diff;
-    __syncthreads();  // s_mixed / s_diff are rewritten for the next segment
-    if (mixed) {  // uniform
-      if (tid == 0) big_list2[atomicAdd(big_count2, 1)] = seg;
-      continue;
-    }
-    for (int p = 0; p < 4; ++p) {
-      const int shift = 8 * p;
-      if (((all_diff >> shift) & 0xffu) == 0) continue;  // uniform: constant digit
-      for (int i = lane; i < kRadix; i += kWave) wcnt[wave][i] = 0;
-      __builtin_amdgcn_wave_barrier();
-      int dg[kMidRounds];
-      uint32_t pos[kMidRounds];
-#pragma unroll
-      for (int r = 0; r < kMidRounds; ++r) {
-        dg[r] = -1;
-        pos[r] = 0;
-        if (r >= rounds) continue;  // uniform
-        const int j = first + r * kWave + lane;
-        const bool ok = j < n;
-        const int d = (int)((lo[r] >> shift) & 0xffu);
-        const uint64_t peers = match_digit(d, ok);
-        const uint32_t rank = (uint32_t)popc64(peers & below);
-        const uint32_t base = ok ? wcnt[wave][d] : 0u;
-        __builtin_amdgcn_wave_barrier();
-        if (ok && rank == 0) wcnt[wave][d] = base + (uint32_t)popc64(peers);
-        __builtin_amdgcn_wave_barrier();
-        dg[r] = ok ? d : -1;
-        pos[r] = base + rank;
-      }
-      __syncthreads();
-      // Digit starts over (digit, wave): threads 0..255 own one digit each.
-      uint32_t c[kMidWaves];
-      uint32_t tot = 0;
-      if (tid < kRadix) {
-#pragma unroll
-        for (int w = 0; w < kMidWaves; ++w) {
-          c[w] = wcnt[w][tid];
-          tot += c[w];
-        }
-      }
-      const int incl = wave_incl_scan((int)tot, lane);
-      if (lane == 63) s_wsum[wave] = incl;
-      __syncthreads();
-      if (tid < kRadix) {
-        int before = 0;
-        for (int w = 0; w < wave; ++w) before += s_wsum[w];
-        uint32_t start = (uint32_t)(before + incl) - tot;
-#pragma unroll
-        for (int w = 0; w < kMidWaves; ++w) {
-          wcnt[w][tid] = start;
-          start += c[w];
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int r = 0; r < kMidRounds; ++r) {
-        if (dg[r] >= 0) {
-          const uint32_t at = wcnt[wave][dg[r]] + pos[r];
-          s_lo[at] = lo[r];
-          s_v[at] = v[r];
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int r = 0; r < kMidRounds; ++r) {
-        const int j = first + r * kWave + lane;
-        if (r < rounds && j < n) {
-          lo[r] = s_lo[j];
-          v[r] = s_v[j];
-        }
-      }
-      __syncthreads();
-    }
-    const uint64_t hi = (uint64_t)hi0 << 32;
-#pragma unroll
-    for (int r = 0; r < kMidRounds; ++r) {
-      const int j = first + r * kWave + lane;
-      if (r < rounds && j < n) {
-        keys[rg.x + j] = hi | lo[r];
-        vals[rg.x + j] = v[r];
-      }
-    }
-  }
-}
-
-// Listed segments of kMidCap + 1 .. kLargeCap pairs whose keys agree above
-// the sorted bits: a 1024-thread workgroup per segment, as ps_seg_mid but
-// with items held compactly so that 32 rounds fit in registers: the low key
-// word, and the item's index in the segment (15 bits) packed with its
-// destination in the current pass (15 bits).  The wave's last rounds are
-// padded up to whole rounds with items whose low word is the segment's
-// largest; a stable sort places them after every real item (their indices
-// are larger), so the passes need no per-item masks.  Each pass scatters the
-// low words through one LDS buffer and reads them back, then does the same
-// with the indices.  At the end the values are gathered by index through
-// LDS (every read of the segment's old values completes before the first
-// write) and written with the sorted keys.  Other listed segments go on to
-// the third list for ps_seg_big.
 constexpr int kLargeWaves = 16;
 constexpr int kLargeRounds = 32;
-constexpr int kLargeCap = kLargeWaves * kLargeRounds * kWave;  // 32768
 
-__global__ __launch_bounds__(kLargeWaves * kWave) void ps_seg_large(
+// Appends a segment to the next kernel's list (dropped past its capacity,
+// which only inputs with more segments than the workspace is sized for reach).
+__device__ __forceinline__ void pass_on(int* list, int* count, int cap, int seg) {
+  const int at = atomicAdd(count, 1);
+  if (at < cap) list[at] = seg;
+}
+
+// WPS: waves per SIMD the kernel is compiled for (its register budget).
+template <int W, int RD, int WPS>
+__global__ __launch_bounds__(W * kWave, WPS) void ps_seg_group(
     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, const int2* __restrict__ ranges,
-    const int* __restrict__ big_list, const int* __restrict__ big_count, int big_cap,
-    int* __restrict__ big_list3, int* __restrict__ big_count3) {
-  constexpr int kThreads = kLargeWaves * kWave;
-  __shared__ uint32_t s_buf[kLargeCap];
-  __shared__ uint32_t wcnt[kLargeWaves][kRadix];
-  __shared__ int s_wsum[kLargeWaves];
+    const int* __restrict__ big_list, const int* __restrict__ big_count, int total,
+    int* __restrict__ next_list, int* __restrict__ next_count, int next_cap) {
+  static_assert(W * kWave >= kRadix, "digit starts need a thread per digit");
+  constexpr int kThreads = W * kWave;
+  constexpr int kCap = kThreads * RD;
+  __shared__ uint32_t s_buf[kCap];
+  __shared__ uint32_t wcnt[W][kRadix];
+  __shared__ int s_wsum[W];
   __shared__ uint32_t s_diff, s_max;
   __shared__ int s_mixed;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t below = lanes_below(lane);
-  const int nbig = min(*big_count, big_cap);
-  for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+  const int nseg = min(*big_count, total);
+  for (int b = blockIdx.x; b < nseg; b += gridDim.x) {
     const int seg = big_list[b];
     const int2 rg = ranges[seg];
     const int n = rg.y - rg.x;
-    if (n > kLargeCap) {  // uniform
-      if (tid == 0) big_list3[atomicAdd(big_count3, 1)] = seg;
+    if (n > kCap) {  // uniform
+      if (tid == 0) pass_on(next_list, next_count, next_cap, seg);
       continue;
     }
     const uint64_t k0 = keys[rg.x];
@@ -660,12 +528,24 @@ __global__ __launch_bounds__(kLargeWaves * kWave) void ps_seg_large(
     __syncthreads();
     bool same = true;
     uint32_t diff = 0, mx = 0;
-    for (int i = tid; i < n; i += kThreads) {
-      const uint64_t k = keys[rg.x + i];
-      same = same && (uint32_t)(k >> 32) == hi0;
-      s_buf[i] = (uint32_t)k;
-      diff |= (uint32_t)k ^ (uint32_t)k0;
-      mx = max(mx, (uint32_t)k);
+    // Strided copies go kBatch items per thread at a time, so that that many
+    // global accesses are in flight.
+    constexpr int kBatch = 8;
+    for (int i0 = tid; i0 < n; i0 += kBatch * kThreads) {
+      uint64_t k[kBatch];
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) {
+        const int i = i0 + q * kThreads;
+        k[q] = i < n ? keys[rg.x + i] : k0;
+      }
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) {
+        const int i = i0 + q * kThreads;
+        same = same && (uint32_t)(k[q] >> 32) == hi0;
+        diff |= (uint32_t)k[q] ^ (uint32_t)k0;
+        mx = max(mx, (uint32_t)k[q]);
+        if (i < n) s_buf[i] = (uint32_t)k[q];
+      }
     }
     if (!same) s_mixed = 1;
     if (diff) atomicOr(&s_diff, diff);
@@ -675,7 +555,7 @@ __global__ __launch_bounds__(kLargeWaves * kWave) void ps_seg_large(
     const uint32_t all_diff = s_diff, pad = s_max;
     __syncthreads();  // s_mixed / s_diff / s_max are rewritten for the next segment
     if (mixed) {  // uniform
-      if (tid == 0) big_list3[atomicAdd(big_count3, 1)] = seg;
+      if (tid == 0) pass_on(next_list, next_count, next_cap, seg);
       continue;
     }
     const int rounds = (n + kThreads - 1) / kThreads;  // uniform
@@ -685,9 +565,9 @@ __global__ __launch_bounds__(kLargeWaves * kWave) void ps_seg_large(
     uint32_t* const my_buf = s_buf + mine;
     // lo: low key word; pk: index in the segment (bits 0..14), destination of
     // the current pass (bits 16..30).
-    uint32_t lo[kLargeRounds], pk[kLargeRounds];
+    uint32_t lo[RD], pk[RD];
 #pragma unroll
-    for (int r = 0; r < kLargeRounds; ++r) {
+    for (int r = 0; r < RD; ++r) {
       pk[r] = (uint32_t)(mine + r * kWave);
       lo[r] = r < rounds && r * kWave < lim ? my_buf[r * kWave] : pad;
     }
@@ -698,7 +578,7 @@ __global__ __launch_bounds__(kLargeWaves * kWave) void ps_seg_large(
       for (int i = lane; i < kRadix; i += kWave) wcnt[wave][i] = 0;
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
-      for (int r = 0; r < kLargeRounds; ++r) {
+      for (int r = 0; r < RD; ++r) {
         if (r >= rounds) continue;  // uniform
         const int d = (int)((lo[r] >> shift) & 0xffu);
         const uint64_t peers = match_digit(d, true);
@@ -711,11 +591,11 @@ __global__ __launch_bounds__(kLargeWaves * kWave) void ps_seg_large(
       }
       __syncthreads();
       // Digit starts over (digit, wave): threads 0..255 own one digit each.
-      uint32_t c[kLargeWaves];
+      uint32_t c[W];
       uint32_t tot = 0;
       if (tid < kRadix) {
 #pragma unroll
-        for (int w = 0; w < kLargeWaves; ++w) {
+        for (int w = 0; w < W; ++w) {
           c[w] = wcnt[w][tid];
           tot += c[w];
         }
@@ -728,14 +608,14 @@ __global__ __launch_bounds__(kLargeWaves * kWave) void ps_seg_large(
         for (int w = 0; w < wave; ++w) before += s_wsum[w];
         uint32_t start = (uint32_t)(before + incl) - tot;
 #pragma unroll
-        for (int w = 0; w < kLargeWaves; ++w) {
+        for (int w = 0; w < W; ++w) {
           wcnt[w][tid] = start;
           start += c[w];
         }
       }
       __syncthreads();
 #pragma unroll
-      for (int r = 0; r < kLargeRounds; ++r) {
+      for (int r = 0; r < RD; ++r) {
         if (r >= rounds) continue;  // uniform
         const uint32_t at = wcnt[wave][(lo[r] >> shift) & 0xffu] + (pk[r] >> 16);
         pk[r] = (pk[r] & 0x7fffu) | (at << 16);
@@ -743,29 +623,41 @@ __global__ __launch_bounds__(kLargeWaves * kWave) void ps_seg_large(
       }
       __syncthreads();
 #pragma unroll
-      for (int r = 0; r < kLargeRounds; ++r)
+      for (int r = 0; r < RD; ++r)
         if (r < rounds) lo[r] = my_buf[r * kWave];
       __syncthreads();
 #pragma unroll
-      for (int r = 0; r < kLargeRounds; ++r)
+      for (int r = 0; r < RD; ++r)
         if (r < rounds) s_buf[pk[r] >> 16] = pk[r] & 0x7fffu;
       __syncthreads();
 #pragma unroll
-      for (int r = 0; r < kLargeRounds; ++r)
+      for (int r = 0; r < RD; ++r)
         if (r < rounds) pk[r] = my_buf[r * kWave];
       __syncthreads();
     }
     const uint64_t hi = (uint64_t)hi0 << 32;
     uint64_t* const my_keys = keys + rg.x + mine;
 #pragma unroll
-    for (int r = 0; r < kLargeRounds; ++r) {
+    for (int r = 0; r < RD; ++r) {
       if (r < rounds && r * kWave < lim) {
         my_keys[r * kWave] = hi | lo[r];
         my_buf[r * kWave] = pk[r] & 0x7fffu;
       }
     }
     __syncthreads();
-    for (int i = tid; i < n; i += kThreads) s_buf[i] = vals[rg.x + (int)s_buf[i]];
+    for (int i0 = tid; i0 < n; i0 += kBatch * kThreads) {
+      uint32_t g[kBatch];
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) {
+        const int i = i0 + q * kThreads;
+        g[q] = i < n ? vals[rg.x + (int)s_buf[i]] : 0u;
+      }
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) {
+        const int i = i0 + q * kThreads;
+        if (i < n) s_buf[i] = g[q];
+      }
+    }
     __syncthreads();
     for (int i = tid; i < n; i += kThreads) vals[rg.x + i] = s_buf[i];
     __syncthreads();  // s_buf is rewritten for the next segment
@@ -899,7 +791,7 @@ struct SortWs {
   int2* seg_ranges;  // segment ranges when the caller gives none
   int* big_list;     // segments longer than kSegCap (or mixed high words)
   int* big_count;    // [0]: entries of big_list; [1]: of big_list2; [2]: of big_list3
-  int* big_list2;    // segments for ps_seg_large
+  int* big_list2;    // segments for the 1024-thread ps_seg_group
   int* big_list3;    // segments for ps_seg_big
 };
 
@@ -1044,14 +936,18 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
                      vals_out, reinterpret_cast<const int2*>(ranges), segs, w.big_list,
                      w.big_count, big_cap);
   PS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(ps_seg_mid, dim3(1024), dim3(kMidWaves * kWave), 0, stream, keys_out,
-                     vals_out, reinterpret_cast<const int2*>(ranges), w.big_list, w.big_count,
-                     big_cap, w.big_list2, w.big_count + 1);
+  // Two 512-thread workgroups per CU.
+  hipLaunchKernelGGL((ps_seg_group<kMidWaves, kMidRounds, 4>), dim3(1024),
+                     dim3(kMidWaves * kWave), 0, stream, keys_out, vals_out,
+                     reinterpret_cast<const int2*>(ranges), w.big_list, w.big_count, big_cap,
+                     w.big_list2, w.big_count + 1, big_cap);
   PS_LAUNCH_CHECK();
-  // One workgroup per CU (the LDS buffer takes most of a CU's 160 KiB).
-  hipLaunchKernelGGL(ps_seg_large, dim3(256), dim3(kLargeWaves * kWave), 0, stream, keys_out,
-                     vals_out, reinterpret_cast<const int2*>(ranges), w.big_list2,
-                     w.big_count + 1, big_cap, w.big_list3, w.big_count + 2);
+  // One 1024-thread workgroup per CU (its LDS buffer takes most of the CU's
+  // 160 KiB).
+  hipLaunchKernelGGL((ps_seg_group<kLargeWaves, kLargeRounds, 4>), dim3(256),
+                     dim3(kLargeWaves * kWave), 0, stream, keys_out, vals_out,
+                     reinterpret_cast<const int2*>(ranges), w.big_list2, w.big_count + 1, big_cap,
+                     w.big_list3, w.big_count + 2, big_cap);
   PS_LAUNCH_CHECK();
   // One workgroup per listed segment at a time; enough of them to fill the
   // device (about seven per CU fit), since each walks its segment serially.

@@ -203,6 +203,7 @@ inline void __hip_atomic_store(T* p, V v, int, int) {
   std::atomic_ref<T>(*p).store((T)v);
 }
 inline void __builtin_amdgcn_s_sleep(int) { emu::pause(); }
+inline void __builtin_amdgcn_sched_barrier(int) {}
 
 // ---------------------------------------------------------------- runtime
 typedef struct ihipStream_t* hipStream_t;

@@ -69,11 +69,12 @@ def test_sort_segments_around_the_lds_limits(emu, mode, monkeypatch):
 
 @pytest.mark.parametrize("threads", [1, 8])
 def test_sort_long_segments_with_full_depth_words(emu, threads, monkeypatch):
-    # Segments past the 512-thread sort (8192) up to and past the 1024-thread
-    # one (32768), with depths over the full 32 bits (every digit pass runs)
-    # and many ties at the segment's largest depth (the padding value).
+    # Segments for the 512-thread sort (up to 8192), the 1024-thread one (up
+    # to 32768) and past it, with depths over the full 32 bits (every digit
+    # pass runs) and many ties at the segment's largest depth (the padding
+    # value).
     monkeypatch.setenv("EMU_THREADS", str(threads))
-    counts = [8193, 20000, 32767, 32768, 32769, 40000]
+    counts = [1000, 5000, 8192, 8193, 20000, 32767, 32768, 32769, 40000]
     rng = np.random.default_rng(123)
     t = np.repeat(np.arange(len(counts), dtype=np.uint64), counts)
     rng.shuffle(t)
