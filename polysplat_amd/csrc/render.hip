@@ -116,9 +116,9 @@ __device__ __forceinline__ uint8_t encode(float x) {
 
 // Quadrant bit k (k = qy * 2 + qx) of a Gaussian: set when its alpha can
 // reach 2^kQuadSkipPower at some pixel of quadrant k (8x8 pixels) of the tile
-// whose first pixel is (tx0, ty0).  The power is w + q(d) with q a negative
-// semidefinite quadratic form, and q(d) <= lmax |d|^2 for lmax the eigenvalue
-// of q nearest zero (taken with a relative rounding allowance), so
+// centred on (cx, cy) (its first pixel + 7.5).  The power is w + q(d) with q
+// a negative semidefinite quadratic form, and q(d) <= lmax |d|^2 for lmax the
+// eigenvalue of q nearest zero (taken with a relative rounding allowance), so
 // w + lmax * dist^2(centre, quadrant) bounds the power over the quadrant.
 // A Gaussian left out of a quadrant would have added less than 2^-32 of
 // colour and left T unchanged in FP32.  The neutral padding Gaussian (power
@@ -129,7 +129,7 @@ __device__ __forceinline__ float dist_to_span(float v, float lo, float hi) {
   return v < lo ? lo - v : (v > hi ? v - hi : 0.0f);
 }
 
-__device__ __forceinline__ uint32_t quad_mask(const Feat& f, float tx0, float ty0) {
+__device__ __forceinline__ uint32_t quad_mask(const Feat& f, float cx, float cy) {
   const float h = 0.5f * (f.a - f.c);
   const float root = __builtin_sqrtf(h * h + 0.25f * f.b * f.b);
   float lmax = 0.5f * (f.a + f.c) + root;
@@ -138,7 +138,8 @@ __device__ __forceinline__ uint32_t quad_mask(const Feat& f, float tx0, float ty
   uint32_t m = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const float x0 = tx0 + 8.0f * (float)(k & 1), y0 = ty0 + 8.0f * (float)(k >> 1);
+    const float x0 = cx + (8.0f * (float)(k & 1) - 7.5f);
+    const float y0 = cy + (8.0f * (float)(k >> 1) - 7.5f);
     const float ddx = dist_to_span(f.x, x0, x0 + 7.0f);
     const float ddy = dist_to_span(f.y, y0, y0 + 7.0f);
     const float bound = f.w + lmax * (ddx * ddx + ddy * ddy);
@@ -147,10 +148,47 @@ __device__ __forceinline__ uint32_t quad_mask(const Feat& f, float tx0, float ty
   return m;
 }
 
+// The power as a polynomial in the pixel's offset (u, v) from the tile's
+// centre (cx, cy) = tile origin + 7.5: with X = x - cx, Y = y - cy,
+//   power = F + u (D + a u + b v) + v (E + c v),
+//   F = w + X (a X + b Y) + c Y^2,  D = -(2 a X + b Y),  E = -(b X + 2 c Y),
+// which is w + a dx^2 + b dx dy + c dy^2 for dx = X - u, dy = Y - v expanded.
+// Staged once per (Gaussian, tile), it costs five FMAs per pixel, and for a
+// lane's four pixels the row terms (per v) and column terms (per u) are
+// shared.  Rounding: F is the power at the tile centre; its magnitude is
+// bounded by the conic (at most about 2.4 per square pixel after the
+// reference's 0.3 low-pass dilation) times the squared distance, so the
+// cancellation error stays near 1e-4 in log2 units, far below 1/255 of
+// colour.  The neutral Gaussian stages F = -inf with D = E = 0.
+struct Staged {
+  float4 A;  // (F, D, E, a)
+  float4 B;  // (b, c, r, g)
+  float bl;
+};
+
+__device__ __forceinline__ Staged stage_feat(const Feat& f, float cx, float cy) {
+  const float X = f.x - cx, Y = f.y - cy;
+  Staged st;
+  st.A.x = fmaf(X, fmaf(f.a, X, f.b * Y), fmaf(f.c * Y, Y, f.w));
+  st.A.y = -fmaf(2.0f * f.a, X, f.b * Y);
+  st.A.z = -fmaf(f.b, X, 2.0f * f.c * Y);
+  st.A.w = f.a;
+  st.B = make_float4(f.b, f.c, f.r, f.g);
+  st.bl = f.bl;
+  return st;
+}
+
+// The power at offset (u, v) from the tile centre; the dense and the sparse
+// path evaluate exactly this sequence of operations.
+__device__ __forceinline__ float staged_power(const float4& A, const float4& B, float u,
+                                              float v) {
+  return fmaf(u, fmaf(B.x, v, fmaf(A.w, u, A.y)), fmaf(fmaf(B.y, v, A.z), v, A.x));
+}
+
 // Estimated VALU instructions of the two shading paths (per Gaussian for the
 // dense one, per live (Gaussian, quadrant) for the sparse one).
-constexpr int kDenseCost = 43;
-constexpr int kSparseCost = 14;
+constexpr int kDenseCost = 39;
+constexpr int kSparseCost = 12;
 constexpr int kSparseGroupCost = 2;
 
 // Shared per-tile body.  Returns nothing; writes the tile's pixels.
@@ -172,9 +210,10 @@ __device__ __forceinline__ void render_tile(
     float4 (*sA)[kBatch], float4 (*sB)[kBatch], float (*sC)[kBatch], int lane) {
   static_assert(kBatch % CHECK == 0, "a batch holds whole check groups");
   const int tx = tile % gx, ty = tile / gx;
-  const float tx0 = (float)(tx * kTile), ty0 = (float)(ty * kTile);
-  const float pxf[2] = {tx0 + (float)(lane & 7), tx0 + (float)(lane & 7) + 8.0f};
-  const float pyf[2] = {ty0 + (float)(lane >> 3), ty0 + (float)(lane >> 3) + 8.0f};
+  // The tile's centre, and this lane's pixel offsets from it.
+  const float cx = (float)(tx * kTile) + 7.5f, cy = (float)(ty * kTile) + 7.5f;
+  const float uf[2] = {(float)(lane & 7) - 7.5f, (float)(lane & 7) + 0.5f};
+  const float vf[2] = {(float)(lane >> 3) - 7.5f, (float)(lane >> 3) + 0.5f};
   float T[4] = {1.f, 1.f, 1.f, 1.f};
   float Cr[4] = {0.f, 0.f, 0.f, 0.f}, Cg[4] = {0.f, 0.f, 0.f, 0.f}, Cb[4] = {0.f, 0.f, 0.f, 0.f};
   const int n = end - start;
@@ -188,12 +227,15 @@ __device__ __forceinline__ void render_tile(
       pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
     }
     int cur = 0;
-    sA[0][lane] = make_float4(pre.x, pre.y, pre.w, pre.a);
-    sB[0][lane] = make_float4(pre.b, pre.c, pre.r, pre.g);
-    sC[0][lane] = pre.bl;
+    {
+      const Staged st = stage_feat(pre, cx, cy);
+      sA[0][lane] = st.A;
+      sB[0][lane] = st.B;
+      sC[0][lane] = st.bl;
+    }
     uint64_t live[4];
     {
-      const uint32_t m = quad_mask(pre, tx0, ty0);
+      const uint32_t m = quad_mask(pre, cx, cy);
 #pragma unroll
       for (int q = 0; q < 4; ++q) live[q] = ballot((m >> q) & 1u);
     }
@@ -217,7 +259,7 @@ __device__ __forceinline__ void render_tile(
           const uint64_t lq[4] = {l0, l1, l2, l3};
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            const float px = pxf[q & 1], py = pyf[q >> 1];
+            const float u = uf[q & 1], v = vf[q >> 1];
             uint64_t bits = lq[q];
             while (bits) {
               const int k = __builtin_ctzll(bits);
@@ -225,9 +267,7 @@ __device__ __forceinline__ void render_tile(
               const float4 A = sA[cur][k];
               const float4 B = sB[cur][k];
               const float bl = sC[cur][k];
-              // A = (x, y, w, a), B = (b, c, r, g)
-              const float dx = A.x - px, dy = A.y - py;
-              const float pw = fmaf(dx, fmaf(A.w, dx, B.x * dy), fmaf(B.y * dy, dy, A.z));
+              const float pw = staged_power(A, B, u, v);
               if (DT && pw < -7.994353375f) continue;
               const float alpha = __builtin_amdgcn_exp2f(pw);
               const float wgt = alpha * T[q];
@@ -245,15 +285,16 @@ __device__ __forceinline__ void render_tile(
             const float4 A = sA[cur][k];
             const float4 B = sB[cur][k];
             const float bl = sC[cur][k];
-            const float dy0 = A.y - pyf[0], dy1 = A.y - pyf[1];
-            const float bdy0 = B.x * dy0, bdy1 = B.x * dy1;
-            const float ys0 = fmaf(B.y * dy0, dy0, A.z), ys1 = fmaf(B.y * dy1, dy1, A.z);
-            const float dx0 = A.x - pxf[0], dx1 = A.x - pxf[1];
+            // staged_power for the lane's four pixels, with the per-row
+            // (v) and per-column (u) terms computed once.
+            const float rv0 = fmaf(fmaf(B.y, vf[0], A.z), vf[0], A.x);
+            const float rv1 = fmaf(fmaf(B.y, vf[1], A.z), vf[1], A.x);
+            const float du0 = fmaf(A.w, uf[0], A.y), du1 = fmaf(A.w, uf[1], A.y);
             float pw[4];
-            pw[0] = fmaf(dx0, fmaf(A.w, dx0, bdy0), ys0);
-            pw[1] = fmaf(dx1, fmaf(A.w, dx1, bdy0), ys0);
-            pw[2] = fmaf(dx0, fmaf(A.w, dx0, bdy1), ys1);
-            pw[3] = fmaf(dx1, fmaf(A.w, dx1, bdy1), ys1);
+            pw[0] = fmaf(uf[0], fmaf(B.x, vf[0], du0), rv0);
+            pw[1] = fmaf(uf[1], fmaf(B.x, vf[0], du1), rv0);
+            pw[2] = fmaf(uf[0], fmaf(B.x, vf[1], du0), rv1);
+            pw[3] = fmaf(uf[1], fmaf(B.x, vf[1], du1), rv1);
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
               if (DT && pw[p] < -7.994353375f) continue;
@@ -275,11 +316,14 @@ __device__ __forceinline__ void render_tile(
       // Publish the prefetched batch, start loading the one after it.
       __syncthreads();
       cur ^= 1;
-      sA[cur][lane] = make_float4(pre.x, pre.y, pre.w, pre.a);
-      sB[cur][lane] = make_float4(pre.b, pre.c, pre.r, pre.g);
-      sC[cur][lane] = pre.bl;
       {
-        const uint32_t m = quad_mask(pre, tx0, ty0);
+        const Staged st = stage_feat(pre, cx, cy);
+        sA[cur][lane] = st.A;
+        sB[cur][lane] = st.B;
+        sC[cur][lane] = st.bl;
+      }
+      {
+        const uint32_t m = quad_mask(pre, cx, cy);
 #pragma unroll
         for (int q = 0; q < 4; ++q) live[q] = ballot((m >> q) & 1u);
       }
