@@ -159,8 +159,9 @@ int ps_gather_records(int num_rendered, const int* d_num_rendered, int max_pairs
  * pybind.cpp:691-745.  feat_prec selects f32 arrays (points_xy, rgb_depth,
  * conic_opacity) or 24-byte records (records).  `gathered` != 0 means the
  * features are already in sorted-pair order (index = pair position).
- * queue: 2 u32 of device memory, zero before the first use; every launch
- * leaves them zero again.  dt != 0 skips pixels with alpha < 1/255. */
+ * queue: 2 KiB of device memory (the tile queue's claim counters), zero
+ * before the first use; every launch leaves it zero again.  dt != 0 skips
+ * pixels with alpha < 1/255. */
 int ps_render_16x16(int num_rendered, const int* d_num_rendered, int width, int height,
                     const float* points_xy, const float* rgb_depth, const float* conic_opacity,
                     const uint64_t* keys_sorted, const uint32_t* values_sorted,

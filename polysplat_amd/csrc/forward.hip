@@ -81,7 +81,7 @@ PinnedSlot& pinned() {
   return s;
 }
 
-constexpr size_t kQueueBytes = 256;
+constexpr size_t kQueueBytes = 2048;
 
 }  // namespace
 

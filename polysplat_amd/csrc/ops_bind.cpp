@@ -285,7 +285,7 @@ void persistent_common(int num_rendered, int width, int height, Tensor* xy, Tens
   }
   check_out(out, width, height);
   auto bg = host_floats(bg_color, 3, "bg_color");
-  Tensor q = cached_workspace(out.device(), 256, "queue");
+  Tensor q = cached_workspace(out.device(), 2048, "queue");
   check(ps_render_16x16_persistent(
             num_rendered, nullptr, width, height, prec, xy ? ptr<float>(*xy) : nullptr,
             rgbd ? ptr<float>(*rgbd) : nullptr, con ? ptr<float>(*con) : nullptr,

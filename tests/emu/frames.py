@@ -90,7 +90,7 @@ class HostFrame:
         self.sort_space = alloc((self.sort_bytes,), np.uint8)
         self.ws_bytes = L.ps_forward_workspace_size(self.P)
         self.ws = alloc((self.ws_bytes,), np.uint8)
-        self.ws[:256] = 0  # persistent-queue words start at zero
+        self.ws[:2048] = 0  # persistent-queue words start at zero
         self.tiles = ((spec.width + 15) // 16) * ((spec.height + 15) // 16)
         self.ranges = alloc((self.tiles, 2), np.int32)
         self.out = alloc((spec.height, spec.width, 3), np.uint8)
@@ -134,7 +134,7 @@ class HostBatch:
         P = (C.c_int * len(frames))(*[f.P for f in frames])
         self.ws_bytes = L.ps_forward_batch_workspace_size(len(frames), P)
         self.ws = alloc((self.ws_bytes,), np.uint8)
-        self.ws[:256] = 0
+        self.ws[:2048] = 0
 
     def run(self, bgs, sync=True):
         fr = (capi.PsFrame * len(self.frames))()

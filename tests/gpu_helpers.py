@@ -53,7 +53,7 @@ class Gpu:
         self.sort_space = torch.zeros(self.sort_bytes, device=self.dev, dtype=torch.uint8)
         self.ws_bytes = L.ps_forward_workspace_size(self.P)
         self.ws = torch.zeros(self.ws_bytes, device=self.dev, dtype=torch.uint8)
-        self.queue = torch.zeros(64, device=self.dev, dtype=torch.int32)
+        self.queue = torch.zeros(512, device=self.dev, dtype=torch.int32)
         gx, gy = (spec.width + 15) // 16, (spec.height + 15) // 16
         self.tiles = gx * gy
         self.ranges = torch.zeros(self.tiles, 2, device=self.dev, dtype=torch.int32)

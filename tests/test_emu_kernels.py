@@ -149,6 +149,25 @@ def test_frame_bit_exact_against_the_oracle(emu, garbage):
     assert np.abs(img.astype(int) - ref["image"].astype(int)).max() <= 1
 
 
+@pytest.mark.parametrize("threads", [1, 8])
+@pytest.mark.parametrize("size", [(16, 16), (48, 16), (100, 60), (400, 240)])
+def test_persistent_queue_covers_every_tile(emu, size, threads, monkeypatch):
+    # 1, 3, 28 and 375 tiles: per-XCD queues that are empty, single-tile or
+    # uneven, chunked claims and stealing; two frames on one workspace (the
+    # queue words must be left zero).  Every pixel must match the oracle.
+    monkeypatch.setenv("EMU_THREADS", str(threads))
+    sc = _scene()
+    spec = scenes.make_camera(size[0], size[1], 10.0, distance=12.0)
+    ref = oracle_frame(sc, spec, sh_type="fp16", feat_prec="fp16", bg=(0.5, 0.25, 0.125),
+                       check_every=8, fp64=False)
+    f = HF.HostFrame(emu, sc, spec, capacity=ref["pre"]["M"] + 100)
+    for _ in range(2):
+        st, n, img = f.forward((0.5, 0.25, 0.125))
+        capi.check_with(emu, st)
+        assert np.abs(img.astype(int) - ref["image"].astype(int)).max() <= 1
+        assert not f.ws[:2048].any()
+
+
 def test_batched_frames_equal_single_frames(emu):
     a = _scene()
     b = scenes.make_scene(1500, 6.0, [(0.7, -3.5, 0.5), (0.3, -1.5, 0.3)], seed=8)
