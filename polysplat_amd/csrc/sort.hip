@@ -317,11 +317,11 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
 // by tile (segment) and keep their input order inside each segment; what is
 // left is a stable sort of every segment by the low 32 bits.  That needs no
 // global passes for segments of up to 32768 pairs: a wave (up to 896
-// pairs), or a workgroup of 8 waves (8192) or 16 waves (32768) sorts each in
-// registers and LDS; longer segments are listed and a workgroup sorts each
-// with four counting passes through global memory.  The result equals the
-// full LSD sort bit for bit: sorting by (high, low) stably is sorting by the
-// whole key stably.
+// pairs), or a workgroup of 4, 8 or 16 waves (4096, 8192, 32768 pairs)
+// sorts each in registers and LDS; longer segments are listed and a
+// workgroup sorts each with four counting passes through global memory.  The
+// result equals the full LSD sort bit for bit: sorting by (high, low) stably
+// is sorting by the whole key stably.
 
 constexpr int kSegCap = 896;    // pairs per wave-sorted segment
 constexpr int kSegWaves = 4;    // waves per workgroup of the small-segment kernel
@@ -478,13 +478,15 @@ __global__ __launch_bounds__(kSegWaves * kWave) void ps_seg_small(
 // new order, then the indices the same way.  At the end the values are
 // gathered by index through LDS (every read of the segment's old values
 // completes before the first write) and written with the sorted keys.
-// Longer (or mixed) segments go on to the next list: 8 waves x 16 rounds
-// (8192 pairs) first, then 16 waves x 32 rounds (32768, most of a CU's
-// LDS), then ps_seg_big.
-constexpr int kMidWaves = 8;
-constexpr int kMidRounds = 16;
-constexpr int kLargeWaves = 16;
-constexpr int kLargeRounds = 32;
+// Longer (or mixed) segments go on to the next list: 4 and 8 waves x 16
+// rounds (4096, 8192 pairs), then 16 waves x 32 rounds (32768, most of a
+// CU's LDS), then ps_seg_big.  Fewer waves per segment mean fewer waves
+// waiting at each barrier and more segments in flight per CU (4 waves
+// instead of 8 for segments of 897..4096 pairs: 893 -> 563 us for a
+// 30M-Gaussian frame at 3840x2160); a 2-wave route for up to 2048 pairs
+// was measured slower (one more launch, no faster per segment).
+constexpr int kGroupRounds = 16;  // 4 and 8 waves: 4096, 8192 pairs
+constexpr int kLargeRounds = 32;  // 16 waves: 32768 pairs
 
 // Appends a segment to the next kernel's list (dropped past its capacity,
 // which only inputs with more segments than the workspace is sized for reach).
@@ -499,8 +501,10 @@ __global__ __launch_bounds__(W * kWave, WPS) void ps_seg_group(
     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, const int2* __restrict__ ranges,
     const int* __restrict__ big_list, const int* __restrict__ big_count, int total,
     int* __restrict__ next_list, int* __restrict__ next_count, int next_cap) {
-  static_assert(W * kWave >= kRadix, "digit starts need a thread per digit");
   constexpr int kThreads = W * kWave;
+  constexpr int DPT = kThreads >= kRadix ? 1 : kRadix / kThreads;  // digits per owner
+  constexpr int kOwners = kRadix / DPT;
+  static_assert(kOwners * DPT == kRadix && kOwners <= kThreads, "digit owners");
   constexpr int kCap = kThreads * RD;
   __shared__ uint32_t s_buf[kCap];
   __shared__ uint32_t wcnt[W][kRadix];
@@ -590,28 +594,33 @@ __global__ __launch_bounds__(W * kWave, WPS) void ps_seg_group(
         pk[r] = (pk[r] & 0x7fffu) | ((base + rank) << 16);
       }
       __syncthreads();
-      // Digit starts over (digit, wave): threads 0..255 own one digit each.
-      uint32_t c[W];
+      // Digit starts over (digit, wave): thread t < kOwners owns digits
+      // DPT t .. DPT t + DPT - 1.
+      uint32_t c[DPT][W];
       uint32_t tot = 0;
-      if (tid < kRadix) {
+      if (tid < kOwners) {
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-          c[w] = wcnt[w][tid];
-          tot += c[w];
-        }
+        for (int k = 0; k < DPT; ++k)
+#pragma unroll
+          for (int w = 0; w < W; ++w) {
+            c[k][w] = wcnt[w][DPT * tid + k];
+            tot += c[k][w];
+          }
       }
       const int incl = wave_incl_scan((int)tot, lane);
       if (lane == 63) s_wsum[wave] = incl;
       __syncthreads();
-      if (tid < kRadix) {
+      if (tid < kOwners) {
         int before = 0;
         for (int w = 0; w < wave; ++w) before += s_wsum[w];
         uint32_t start = (uint32_t)(before + incl) - tot;
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-          wcnt[w][tid] = start;
-          start += c[w];
-        }
+        for (int k = 0; k < DPT; ++k)
+#pragma unroll
+          for (int w = 0; w < W; ++w) {
+            wcnt[w][DPT * tid + k] = start;
+            start += c[k][w];
+          }
       }
       __syncthreads();
 #pragma unroll
@@ -790,9 +799,9 @@ struct SortWs {
   uint32_t* totals;
   int2* seg_ranges;  // segment ranges when the caller gives none
   int* big_list;     // segments longer than kSegCap (or mixed high words)
-  int* big_count;    // [0]: entries of big_list; [1]: of big_list2; [2]: of big_list3
-  int* big_list2;    // segments for the 1024-thread ps_seg_group
-  int* big_list3;    // segments for ps_seg_big
+  int* big_count;    // entries of big_list, big_list2, big_list3, big_list (second use)
+  int* big_list2;    // segments for the 512-thread ps_seg_group
+  int* big_list3;    // segments for the 1024-thread ps_seg_group
 };
 
 // Segments (values of the sorted high bits) the internal range table holds.
@@ -926,7 +935,7 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
                       vals_out, stream);
   if (st != PS_OK) return st;
   st = tile_ranges_scan_launch(n_host, d_n, max_pairs, segs, keys_out, ranges, stream, mask,
-                               w.big_count, 3);
+                               w.big_count, 4);
   if (st != PS_OK) return st;
   if (d_n == nullptr && n_host == 0) return PS_OK;
   const int big_cap = (int)big_list_cap((size_t)(max_pairs > 0 ? max_pairs : 1));
@@ -936,24 +945,31 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
                      vals_out, reinterpret_cast<const int2*>(ranges), segs, w.big_list,
                      w.big_count, big_cap);
   PS_LAUNCH_CHECK();
+  // Listed segments go through the workgroup routes in turn; each passes on
+  // what it cannot hold.  Lists: big_list -> big_list2 -> big_list3 ->
+  // big_list -> ps_seg_big (a list is reused once its reader is done),
+  // counters 0..3.
+  const int2* rg = reinterpret_cast<const int2*>(ranges);
+  // 256-thread workgroups, five waves per SIMD.
+  hipLaunchKernelGGL((ps_seg_group<4, kGroupRounds, 5>), dim3(2048), dim3(4 * kWave), 0, stream,
+                     keys_out, vals_out, rg, w.big_list, w.big_count, big_cap, w.big_list2,
+                     w.big_count + 1, big_cap);
+  PS_LAUNCH_CHECK();
   // Two 512-thread workgroups per CU.
-  hipLaunchKernelGGL((ps_seg_group<kMidWaves, kMidRounds, 4>), dim3(1024),
-                     dim3(kMidWaves * kWave), 0, stream, keys_out, vals_out,
-                     reinterpret_cast<const int2*>(ranges), w.big_list, w.big_count, big_cap,
-                     w.big_list2, w.big_count + 1, big_cap);
+  hipLaunchKernelGGL((ps_seg_group<8, kGroupRounds, 4>), dim3(1024), dim3(8 * kWave), 0, stream,
+                     keys_out, vals_out, rg, w.big_list2, w.big_count + 1, big_cap, w.big_list3,
+                     w.big_count + 2, big_cap);
   PS_LAUNCH_CHECK();
   // One 1024-thread workgroup per CU (its LDS buffer takes most of the CU's
   // 160 KiB).
-  hipLaunchKernelGGL((ps_seg_group<kLargeWaves, kLargeRounds, 4>), dim3(256),
-                     dim3(kLargeWaves * kWave), 0, stream, keys_out, vals_out,
-                     reinterpret_cast<const int2*>(ranges), w.big_list2, w.big_count + 1, big_cap,
-                     w.big_list3, w.big_count + 2, big_cap);
+  hipLaunchKernelGGL((ps_seg_group<16, kLargeRounds, 4>), dim3(256), dim3(16 * kWave), 0, stream,
+                     keys_out, vals_out, rg, w.big_list3, w.big_count + 2, big_cap, w.big_list,
+                     w.big_count + 3, big_cap);
   PS_LAUNCH_CHECK();
   // One workgroup per listed segment at a time; enough of them to fill the
   // device (about seven per CU fit), since each walks its segment serially.
   hipLaunchKernelGGL(ps_seg_big, dim3(2048), dim3(kSortThreads), 0, stream, keys_out, vals_out,
-                     w.alt_keys, w.alt_vals, reinterpret_cast<const int2*>(ranges), w.big_list3,
-                     w.big_count + 2, big_cap);
+                     w.alt_keys, w.alt_vals, rg, w.big_list, w.big_count + 3, big_cap);
   PS_LAUNCH_CHECK();
   return PS_OK;
 }

@@ -74,7 +74,7 @@ def test_sort_long_segments_with_full_depth_words(emu, threads, monkeypatch):
     # pass runs) and many ties at the segment's largest depth (the padding
     # value).
     monkeypatch.setenv("EMU_THREADS", str(threads))
-    counts = [1000, 5000, 8192, 8193, 20000, 32767, 32768, 32769, 40000]
+    counts = [1000, 4096, 4097, 5000, 8192, 8193, 20000, 32767, 32768, 32769, 40000]
     rng = np.random.default_rng(123)
     t = np.repeat(np.arange(len(counts), dtype=np.uint64), counts)
     rng.shuffle(t)

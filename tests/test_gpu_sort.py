@@ -143,7 +143,7 @@ def test_sort_long_segments_with_full_depth_words(gpu, mode, monkeypatch):
     # runs), with ties at the segment's largest depth (the padding value).
     if mode == "lsd":
         monkeypatch.setenv("POLYSPLAT_SORT", "lsd")
-    counts = [1000, 5000, 8192, 8193, 20000, 32767, 32768, 32769, 40000, 100000]
+    counts = [1000, 4096, 4097, 5000, 8192, 8193, 20000, 32767, 32768, 32769, 40000, 100000]
     rng = np.random.default_rng(123)
     t = np.repeat(np.arange(len(counts), dtype=np.uint64), counts)
     rng.shuffle(t)
