@@ -26,6 +26,8 @@
 #include "ps_host.h"
 #include "ps_wave.h"
 
+#include <stdlib.h>
+
 
 namespace ps {
 namespace {
@@ -355,6 +357,105 @@ __device__ __forceinline__ void render_tile(
   }
 }
 
+// One 8x8 quadrant q of a tile (q = qy * 2 + qx), one pixel per lane: the
+// sparse path of render_tile alone, with its own saturation test.  For
+// frames with fewer tiles than waves in flight, so that each tile's work is
+// split four ways (a frame's longest tile otherwise sets the render time).
+// Per pixel the same operations in the same order as render_tile; the
+// Gaussians left out are those whose alpha stays below 2^-32 in this
+// quadrant, and the quadrant stops on its own pixels' transmittance, where a
+// tile waits for all four (pixels past saturation add < 1e-4 of colour).
+template <int PREC, bool GATHERED, int CHECK, bool DT>
+__device__ __forceinline__ void render_quad(
+    int tile, int q, int start, int end, int W, int H, int gx, const uint32_t* __restrict__ order,
+    const float2* __restrict__ xy, const float4* __restrict__ rgbd, const float4* __restrict__ con,
+    const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
+    float4 (*sA)[kBatch], float4 (*sB)[kBatch], float (*sC)[kBatch], int lane) {
+  const int tx = tile % gx, ty = tile / gx;
+  const float cx = (float)(tx * kTile) + 7.5f, cy = (float)(ty * kTile) + 7.5f;
+  const float u = (float)(lane & 7) + 8.0f * (float)(q & 1) - 7.5f;
+  const float v = (float)(lane >> 3) + 8.0f * (float)(q >> 1) - 7.5f;
+  float T = 1.f, Cr = 0.f, Cg = 0.f, Cb = 0.f;
+  const int n = end - start;
+  if (n > 0) {
+    Feat pre = neutral_feat();
+    if (lane < n) {
+      const int idx = GATHERED ? start + lane : (int)order[start + lane];
+      pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
+    }
+    int cur = 0;
+    {
+      const Staged st = stage_feat(pre, cx, cy);
+      sA[0][lane] = st.A;
+      sB[0][lane] = st.B;
+      sC[0][lane] = st.bl;
+    }
+    uint64_t live = ballot((quad_mask(pre, cx, cy) >> q) & 1u);
+    pre = neutral_feat();
+    if (kBatch + lane < n) {
+      const int idx = GATHERED ? start + kBatch + lane : (int)order[start + kBatch + lane];
+      pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
+    }
+    __syncthreads();
+    for (int b0 = 0;; b0 += kBatch) {
+      const int cnt = (n - b0 < kBatch) ? (n - b0) : kBatch;
+      const int groups = (cnt + CHECK - 1) / CHECK;
+      bool done = false;
+      for (int g = 0; g < groups && !done; ++g) {
+        const uint64_t gm = (CHECK >= 64 ? ~0ull : ((1ull << CHECK) - 1ull)) << (g * CHECK);
+        uint64_t bits = live & gm;
+        while (bits) {
+          const int k = __builtin_ctzll(bits);
+          bits &= bits - 1;
+          const float4 A = sA[cur][k];
+          const float4 B = sB[cur][k];
+          const float bl = sC[cur][k];
+          const float pw = staged_power(A, B, u, v);
+          if (DT && pw < -7.994353375f) continue;
+          const float alpha = __builtin_amdgcn_exp2f(pw);
+          const float wgt = alpha * T;
+          Cr = fmaf(B.z, wgt, Cr);
+          Cg = fmaf(B.w, wgt, Cg);
+          Cb = fmaf(bl, wgt, Cb);
+          T = T - wgt;
+        }
+        done = ballot(!(T < kTMin)) == 0;
+      }
+      if (done || b0 + kBatch >= n) break;
+      __syncthreads();
+      cur ^= 1;
+      {
+        const Staged st = stage_feat(pre, cx, cy);
+        sA[cur][lane] = st.A;
+        sB[cur][lane] = st.B;
+        sC[cur][lane] = st.bl;
+      }
+      live = ballot((quad_mask(pre, cx, cy) >> q) & 1u);
+      pre = neutral_feat();
+      const int nb = b0 + 2 * kBatch + lane;
+      if (nb < n) {
+        const int idx = GATHERED ? start + nb : (int)order[start + nb];
+        pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
+      }
+      __syncthreads();
+    }
+  }
+  const int px = tx * kTile + 8 * (q & 1) + (lane & 7), py = ty * kTile + 8 * (q >> 1) + (lane >> 3);
+  if (px < W && py < H) {
+    uint8_t* o = out + ((size_t)py * W + px) * 3;
+    if (n > 0) {
+      const float t = T < kTMin ? 0.0f : T;
+      o[0] = encode(Cr + t * bg.x);
+      o[1] = encode(Cg + t * bg.y);
+      o[2] = encode(Cb + t * bg.z);
+    } else {
+      o[0] = encode(bg.x);
+      o[1] = encode(bg.y);
+      o[2] = encode(bg.z);
+    }
+  }
+}
+
 // One workgroup per tile.
 template <int PREC, int CHECK>
 __global__ __launch_bounds__(kWave, 6) void ps_render_plain(
@@ -387,6 +488,7 @@ __global__ __launch_bounds__(kWave, 6) void ps_render_plain(
 // (56M-Gaussian city at 9216x6912: 6.0 vs 3.2 ms).  The last workgroup to
 // finish zeroes every word, so the next launch needs no reset.
 constexpr int kQueueLines = 8;
+constexpr int kQuadPairs = 512;  // mean pairs per tile above which quadrants pay
 constexpr int kQueueStride = 32;                         // words between counters
 constexpr int kQueueDone = kQueueLines * kQueueStride;  // finished workgroups
 
@@ -421,10 +523,13 @@ struct TileQueue {
 };
 
 // Persistent: tiles from the queue.  RANGES: read tile_ranges, else search.
+// quads = 1: the queue's items are (tile, quadrant) pairs, item 4 t + q,
+// each rendered by render_quad; 2: so when the frame averages at least
+// kQuadPairs pairs per tile, else whole tiles; 0: whole tiles.
 template <int PREC, bool RANGES, bool GATHERED, int CHECK, bool DT>
 __global__ __launch_bounds__(kWave, 6) void ps_render_persistent(
     const uint64_t* __restrict__ keys, const int* __restrict__ d_n, int n_host,
-    const int2* __restrict__ ranges, int W, int H, int gx, int tiles,
+    const int2* __restrict__ ranges, int W, int H, int gx, int tiles, int quads,
     const uint32_t* __restrict__ order, const float2* __restrict__ xy,
     const float4* __restrict__ rgbd, const float4* __restrict__ con,
     const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
@@ -433,9 +538,12 @@ __global__ __launch_bounds__(kWave, 6) void ps_render_persistent(
   __shared__ float4 sB[2][kBatch];
   __shared__ float sC[2][kBatch];
   const int lane = threadIdx.x;
-  const int n = RANGES ? 0 : (d_n ? *d_n : n_host);
-  TileQueue tq(queue, tiles, lane);
-  for (int tile = tq.claim(); tile >= 0; tile = tq.claim()) {
+  const int pairs = d_n ? *d_n : n_host;
+  const int n = RANGES ? 0 : pairs;
+  if (quads == 2) quads = (long long)pairs >= (long long)kQuadPairs * tiles ? 1 : 0;
+  TileQueue tq(queue, quads ? 4 * tiles : tiles, lane);
+  for (int item = tq.claim(); item >= 0; item = tq.claim()) {
+    const int tile = quads ? item >> 2 : item;
     int start, end;
     if (RANGES) {
       const int2 r = ranges[tile];
@@ -445,8 +553,12 @@ __global__ __launch_bounds__(kWave, 6) void ps_render_persistent(
       start = wave_bound(keys, n, (uint32_t)tile, false, lane);
       end = wave_bound(keys, n, (uint32_t)tile, true, lane);
     }
-    render_tile<PREC, GATHERED, CHECK, DT>(tile, start, end, W, H, gx, order, xy, rgbd, con, rec,
-                                           bg, out, sA, sB, sC, lane);
+    if (quads)
+      render_quad<PREC, GATHERED, CHECK, DT>(tile, item & 3, start, end, W, H, gx, order, xy,
+                                             rgbd, con, rec, bg, out, sA, sB, sC, lane);
+    else
+      render_tile<PREC, GATHERED, CHECK, DT>(tile, start, end, W, H, gx, order, xy, rgbd, con,
+                                             rec, bg, out, sA, sB, sC, lane);
   }
   tq.finish();
 }
@@ -524,6 +636,21 @@ int persistent_grid(K kernel, int tiles) {
   return (int)(g < 1 ? 1 : g);
 }
 
+// Work items of the persistent renderer (see ps_render_persistent): quadrants
+// when the frame has fewer tiles than waves in flight (every tile then has a
+// wave to itself and the longest tile sets the render time), the kernel's
+// choice from the pair count when it has fewer than twice as many, whole
+// tiles otherwise.  Measured (render ms, whole tiles -> quadrants): 1280x720
+// with 3M / 30M Gaussians 0.52 -> 0.32 / 1.27 -> 0.78; 1920x1080 with 10M
+// (1532 pairs per tile) 1.14 -> 0.91 but with 1M (153) 0.15 -> 0.20;
+// 3840x2160 at C3 0.59 -> 0.67.  POLYSPLAT_RENDER_QUADS=0 or 1 forces it.
+int quadrant_items(int tiles, int waves) {
+  const char* e = getenv("POLYSPLAT_RENDER_QUADS");
+  if (e && (e[0] == '0' || e[0] == '1')) return e[0] == '1';
+  if (tiles < waves) return 1;
+  return tiles < 2 * waves ? 2 : 0;
+}
+
 template <int PREC, bool RANGES, bool GATHERED, int CHECK, bool DT>
 int launch_persistent(const uint64_t* keys, const int* d_n, int n_host, const int* ranges, int W,
                       int H, const uint32_t* order, const float* xy, const float* rgbd,
@@ -532,9 +659,12 @@ int launch_persistent(const uint64_t* keys, const int* d_n, int n_host, const in
   const int gx = (W + kTile - 1) / kTile, gy = (H + kTile - 1) / kTile;
   const int tiles = gx * gy;
   auto k = ps_render_persistent<PREC, RANGES, GATHERED, CHECK, DT>;
-  const int grid = persistent_grid(k, tiles);
+  const int waves = persistent_grid(k, 1 << 30);
+  const int quads = quadrant_items(tiles, waves);
+  const int items = quads == 1 ? 4 * tiles : tiles;  // quads == 2: tiles >= waves
+  const int grid = waves < items ? waves : items;
   hipLaunchKernelGGL(k, dim3(grid), dim3(kWave), 0, stream, keys, d_n, n_host,
-                     reinterpret_cast<const int2*>(ranges), W, H, gx, tiles, order,
+                     reinterpret_cast<const int2*>(ranges), W, H, gx, tiles, quads, order,
                      (const float2*)xy, (const float4*)rgbd, (const float4*)con,
                      (const FeatRec16*)rec, bg, out, queue);
   PS_LAUNCH_CHECK();

@@ -149,13 +149,19 @@ def test_frame_bit_exact_against_the_oracle(emu, garbage):
     assert np.abs(img.astype(int) - ref["image"].astype(int)).max() <= 1
 
 
+@pytest.mark.parametrize("quads", ["0", "1", "auto"])
 @pytest.mark.parametrize("threads", [1, 8])
-@pytest.mark.parametrize("size", [(16, 16), (48, 16), (100, 60), (400, 240)])
-def test_persistent_queue_covers_every_tile(emu, size, threads, monkeypatch):
-    # 1, 3, 28 and 375 tiles: per-XCD queues that are empty, single-tile or
-    # uneven, chunked claims and stealing; two frames on one workspace (the
-    # queue words must be left zero).  Every pixel must match the oracle.
+@pytest.mark.parametrize("size", [(16, 16), (48, 16), (64, 32), (100, 60), (400, 240)])
+def test_persistent_queue_covers_every_tile(emu, size, threads, quads, monkeypatch):
+    # 1, 3, 8, 28 and 375 tiles (the emulated device keeps 8 waves in
+    # flight): per-XCD queues that are empty, single-tile or uneven, and
+    # stealing; items that are whole tiles or (tile, quadrant) pairs, forced
+    # or chosen (8 tiles: by the kernel, from the pair count); two frames on
+    # one workspace (the queue words must be left zero).  Every pixel must
+    # match the oracle.
     monkeypatch.setenv("EMU_THREADS", str(threads))
+    if quads != "auto":
+        monkeypatch.setenv("POLYSPLAT_RENDER_QUADS", quads)
     sc = _scene()
     spec = scenes.make_camera(size[0], size[1], 10.0, distance=12.0)
     ref = oracle_frame(sc, spec, sh_type="fp16", feat_prec="fp16", bg=(0.5, 0.25, 0.125),
@@ -217,10 +223,13 @@ def test_renderer_pads_partial_groups_neutrally(emu):
     assert np.abs(out.astype(int) - ref2["image"].astype(int)).max() <= 1
 
 
-def test_renderer_sparse_and_dense_groups_match_the_oracle(emu):
+@pytest.mark.parametrize("quads", ["0", "1"])
+def test_renderer_sparse_and_dense_groups_match_the_oracle(emu, quads, monkeypatch):
     # A city crop: sub-pixel splats (the sparse quadrant path) over a few
     # large ones (the dense path), and a mixed-size scene; images within one
-    # 8-bit level of the oracle, with almost every pixel identical.
+    # 8-bit level of the oracle, with almost every pixel identical; with
+    # whole-tile and with quadrant work items.
+    monkeypatch.setenv("POLYSPLAT_RENDER_QUADS", quads)
     city = scenes.make_config_scene("c4", seed=4, P=300_000)
     cam = scenes.config_camera("c4")
     cam.width, cam.height = 320, 240

@@ -93,8 +93,12 @@ def test_sort_and_ranges_bit_exact(case):
     np.testing.assert_array_equal(g.ranges_scan(), O.tile_ranges(ref["keys_sorted"], T, 1))
 
 
-@pytest.mark.parametrize("mode", ["plain", "persistent", "fused", "dt"])
-def test_render_fp32_parity(case, mode):
+@pytest.mark.parametrize("mode", ["plain", "persistent", "persistent_tiles", "fused", "dt"])
+def test_render_fp32_parity(case, mode, monkeypatch):
+    # These frames have fewer tiles than the persistent renderer's waves, so
+    # it takes (tile, quadrant) items; "persistent_tiles" forces whole tiles.
+    if mode == "persistent_tiles":
+        monkeypatch.setenv("POLYSPLAT_RENDER_QUADS", "0")
     name, scene, spec, ref = case
     g = Gpu(scene, spec, capacity=_cap(ref))
     g.preprocess()
@@ -104,7 +108,7 @@ def test_render_fp32_parity(case, mode):
     if mode == "plain":
         img = g.render_plain(BG)
         cad, dt = 2, False
-    elif mode == "persistent":
+    elif mode in ("persistent", "persistent_tiles"):
         img = g.render_persistent(BG, use_ranges=True)
         cad, dt = 8, False
     elif mode == "fused":
