@@ -149,16 +149,17 @@ def test_frame_bit_exact_against_the_oracle(emu, garbage):
     assert np.abs(img.astype(int) - ref["image"].astype(int)).max() <= 1
 
 
-@pytest.mark.parametrize("quads", ["0", "1", "auto"])
-@pytest.mark.parametrize("threads", [1, 8])
-@pytest.mark.parametrize("size", [(16, 16), (48, 16), (64, 32), (100, 60), (400, 240)])
-def test_persistent_queue_covers_every_tile(emu, size, threads, quads, monkeypatch):
+@pytest.mark.parametrize("size,threads,quads,frames", [
+    ((16, 16), 1, "0", 1), ((16, 16), 1, "1", 1), ((48, 16), 8, "0", 1), ((48, 16), 8, "1", 1),
+    ((64, 32), 1, "auto", 1), ((100, 60), 8, "0", 1), ((100, 60), 8, "1", 2),
+    ((400, 240), 8, "0", 2)])
+def test_persistent_queue_covers_every_tile(emu, size, threads, quads, frames, monkeypatch):
     # 1, 3, 8, 28 and 375 tiles (the emulated device keeps 8 waves in
     # flight): per-XCD queues that are empty, single-tile or uneven, and
     # stealing; items that are whole tiles or (tile, quadrant) pairs, forced
-    # or chosen (8 tiles: by the kernel, from the pair count); two frames on
-    # one workspace (the queue words must be left zero).  Every pixel must
-    # match the oracle.
+    # or chosen (8 tiles: by the kernel, from the pair count); a second frame
+    # on the same workspace (the queue words must be left zero).  Every pixel
+    # must match the oracle.
     monkeypatch.setenv("EMU_THREADS", str(threads))
     if quads != "auto":
         monkeypatch.setenv("POLYSPLAT_RENDER_QUADS", quads)
@@ -167,7 +168,7 @@ def test_persistent_queue_covers_every_tile(emu, size, threads, quads, monkeypat
     ref = oracle_frame(sc, spec, sh_type="fp16", feat_prec="fp16", bg=(0.5, 0.25, 0.125),
                        check_every=8, fp64=False)
     f = HF.HostFrame(emu, sc, spec, capacity=ref["pre"]["M"] + 100)
-    for _ in range(2):
+    for _ in range(frames):
         st, n, img = f.forward((0.5, 0.25, 0.125))
         capi.check_with(emu, st)
         assert np.abs(img.astype(int) - ref["image"].astype(int)).max() <= 1
