@@ -243,7 +243,12 @@ __device__ int sweep_fragmented(const Cand& me, bool is_frag, int lane, int bx, 
     if (tv) {
       const int w = q.rx1 - q.rx0;
       const int local = t - o_excl;
-      ty = local / w;
+      // local / w for local < 64, w < 64: the reciprocal is within one ulp
+      // of 1 / w, so local * rcp(w) is off by under 2^-16, and the 1/128
+      // bias lifts exact quotients without crossing the next integer (the
+      // fraction of an inexact one is at most 1 - 1/63); checked for every
+      // (local, w) and both one-ulp neighbours of the reciprocal.
+      ty = (int)((float)local * __builtin_amdgcn_rcpf((float)w) + (1.0f / 128.0f));
       tx = local - ty * w;
       ok = tile_hits(q.rx0 + tx, q.ry0 + ty, proj_from(q), bx, by);
     }
@@ -613,11 +618,14 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_emit_rec(
   const int w = count > 0 ? (int)(r.x0w >> 24) : 0;
   if (count > 0 && w > 0) {
     const int x0 = (int)(r.x0w & 0xffffffu), y0 = (int)r.y0;
+    // t / w for t < 32 <= 2^10 and w <= 32 as (t * ceil(2^16 / w)) >> 16:
+    // the rounding adds less than 2^-11 < 1/w to t / w, so the floor is exact.
+    const uint32_t inv_w = (65536u + (uint32_t)w - 1u) / (uint32_t)w;
     uint32_t m = r.mask;
     for (int k = 0; m != 0u && k < count; ++k) {
       const int t = __builtin_ctz(m);
       m &= m - 1u;
-      const int ty = t / w, tx = t - ty * w;
+      const int ty = (int)(((uint32_t)t * inv_w) >> 16), tx = t - ty * w;
       const uint32_t tile = (uint32_t)(F.tile0 + (y0 + ty) * cp.grid_x + (x0 + tx));
       keys[off + k] = ((uint64_t)tile << 32) | (uint64_t)r.depth;
       vals[off + k] = (uint32_t)g;

@@ -148,6 +148,7 @@ inline unsigned max(unsigned a, unsigned b) { return a > b ? a : b; }
 inline int __popcll(unsigned long long m) { return __builtin_popcountll(m); }
 inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
 inline float __builtin_amdgcn_exp2f(float x) { return exp2f(x); }
+inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
 
 inline float __uint_as_float(unsigned u) {
   float f;
