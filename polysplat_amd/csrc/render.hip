@@ -187,10 +187,14 @@ __device__ __forceinline__ float staged_power(const float4& A, const float4& B, 
   return fmaf(u, fmaf(B.x, v, fmaf(A.w, u, A.y)), fmaf(fmaf(B.y, v, A.z), v, A.x));
 }
 
-// Estimated VALU instructions of the two shading paths (per Gaussian for the
-// dense one, per live (Gaussian, quadrant) for the sparse one).
+// Relative costs of the two shading paths (per Gaussian for the dense one,
+// per live (Gaussian, quadrant) for the sparse one), from their VALU counts
+// (about 39 and 12) with the sparse one weighted up: its LDS reads and
+// scalar loop control are not free.  14 was measured best among 10..24
+// (C3 render 0.573 ms at 12, 0.547-0.552 at 14, 0.577 at 17; 30M Gaussians
+// at 3840x2160 2.99 / 2.88-2.96 / 2.96).
 constexpr int kDenseCost = 39;
-constexpr int kSparseCost = 12;
+constexpr int kSparseCost = 14;
 constexpr int kSparseGroupCost = 2;
 
 // Shared per-tile body.  Returns nothing; writes the tile's pixels.
