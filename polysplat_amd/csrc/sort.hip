@@ -338,6 +338,26 @@ __device__ __forceinline__ uint64_t match_digit(int d, bool ok) {
   return peers;
 }
 
+// The same set of lanes through LDS: every valid lane ORs its bit into its
+// digit's 64-bit lane mask, reads the mask back and clears it.  Three LDS
+// operations instead of eight ballots with their per-lane selects; lanes
+// that share a digit contend for one word, which the LDS serialises.  `mt`
+// holds two words per digit (lanes 0..31, 32..63), all zero on entry, and is
+// left zero.
+__device__ __forceinline__ uint64_t match_digit_lds(int d, bool ok, int lane, uint32_t* mt) {
+  if (ok) atomicOr(&mt[2 * d + (lane >> 5)], 1u << (lane & 31));
+  __builtin_amdgcn_wave_barrier();
+  uint64_t peers = 0;
+  if (ok) {
+    const uint2 m = *reinterpret_cast<const uint2*>(&mt[2 * d]);
+    peers = ((uint64_t)m.y << 32) | m.x;
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (ok) *reinterpret_cast<uint2*>(&mt[2 * d]) = make_uint2(0u, 0u);
+  __builtin_amdgcn_wave_barrier();
+  return peers;
+}
+
 // Each wave takes segments seg = wave id, wave id + total waves, ...  A
 // segment longer than kSegCap, or whose keys' high words differ (bits above
 // the sorted ones), goes to the list for the workgroup kernels.  The wave keeps its
@@ -355,11 +375,15 @@ __global__ __launch_bounds__(kSegWaves * kWave) void ps_seg_small(
   __shared__ uint32_t s_lo[kSegWaves][kSegCap];
   __shared__ uint32_t s_v[kSegWaves][kSegCap];
   __shared__ uint32_t cnt[kSegWaves][kRadix];
+  __shared__ uint2 s_match[kSegWaves][kRadix];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t below = lanes_below(lane);
   uint32_t* const slo = s_lo[wave];
   uint32_t* const sv = s_v[wave];
   uint32_t* const c = cnt[wave];
+  uint32_t* const mt = reinterpret_cast<uint32_t*>(s_match[wave]);
+  for (int i = lane; i < 2 * kRadix; i += kWave) mt[i] = 0u;
+  __builtin_amdgcn_wave_barrier();
   for (int seg = blockIdx.x * kSegWaves + wave; seg < segs; seg += gridDim.x * kSegWaves) {
     const int2 rg = ranges[seg];
     const int n = rg.y - rg.x;
@@ -408,7 +432,7 @@ __global__ __launch_bounds__(kSegWaves * kWave) void ps_seg_small(
         if (r < rounds) {
           const bool ok = r * kWave + lane < n;
           const int d = (int)((lo[r] >> shift) & 0xffu);
-          pm[r] = match_digit(d, ok);
+          pm[r] = match_digit_lds(d, ok, lane, mt);
           if (ok && (pm[r] & below) == 0) c[d] += (uint32_t)popc64(pm[r]);
           __builtin_amdgcn_wave_barrier();
         }
