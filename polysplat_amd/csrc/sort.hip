@@ -5,8 +5,9 @@
 //   upsweep   : every block histograms its contiguous share of the keys;
 //   scan      : one workgroup per digit scans that digit's counts over
 //               blocks and records the digit total;
-//   downsweep : every block ranks its keys tile by tile (wave64 ballot
-//               match per digit against per-wave LDS counters), stages them
+//   downsweep : every block ranks its keys tile by tile (per-wave digit
+//               match through LDS lane masks, against per-wave LDS
+//               counters), stages them
 //               in LDS grouped by digit and writes each digit run
 //               contiguously, prefetching the next tile meanwhile.
 // A fixed number of blocks (kSortBlocks) covers any pair count by giving
@@ -141,6 +142,26 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_scan(const uint32_t* __r
   if (tid == 0) totals[d] = (uint32_t)total;
 }
 
+// The same set of lanes through LDS: every valid lane ORs its bit into its
+// digit's 64-bit lane mask, reads the mask back and clears it.  Three LDS
+// operations instead of eight ballots with their per-lane selects; lanes
+// that share a digit contend for one word, which the LDS serialises.  `mt`
+// holds two words per digit (lanes 0..31, 32..63), all zero on entry, and is
+// left zero.
+__device__ __forceinline__ uint64_t match_digit_lds(int d, bool ok, int lane, uint32_t* mt) {
+  if (ok) atomicOr(&mt[2 * d + (lane >> 5)], 1u << (lane & 31));
+  __builtin_amdgcn_wave_barrier();
+  uint64_t peers = 0;
+  if (ok) {
+    const uint2 m = *reinterpret_cast<const uint2*>(&mt[2 * d]);
+    peers = ((uint64_t)m.y << 32) | m.x;
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (ok) *reinterpret_cast<uint2*>(&mt[2 * d]) = make_uint2(0u, 0u);
+  __builtin_amdgcn_wave_barrier();
+  return peers;
+}
+
 __device__ __forceinline__ void load_tile(const uint64_t* __restrict__ keys,
                                           const uint32_t* __restrict__ vals, long long tile0,
                                           int tile_n, int wave, int lane, uint64_t (&k)[kSortItems],
@@ -179,6 +200,7 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
   // tile-local start.  Thread t owns digits t*DPT .. t*DPT + DPT - 1.
   __shared__ uint32_t wcnt[4][D];
   __shared__ uint32_t gbase[D];
+  __shared__ uint2 s_match[4][D];  // per-wave digit lane masks (match_digit_lds)
   __shared__ int scratch[4];
 
   const SortPlan pl = sort_plan(d_n, n_host, max_pairs);
@@ -210,6 +232,9 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
     }
   }
   const uint64_t below = lanes_below(lane);
+  uint32_t* const mt = reinterpret_cast<uint32_t*>(s_match[wave]);
+  for (int i = lane; i < 2 * D; i += kWave) mt[i] = 0u;
+  __builtin_amdgcn_wave_barrier();
 
   uint64_t k[kSortItems];
   uint32_t v[kSortItems];
@@ -225,13 +250,7 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
     for (int r = 0; r < kSortItems; ++r) {
       const int d = (tile_item(wave, r, lane) < tile_n) ? digit_of(k[r], spec) : -1;
       dg[r] = d;
-      uint64_t peers = ballot(d >= 0);
-#pragma unroll
-      for (int bit = 0; bit < DB; ++bit) {
-        const bool set = d >= 0 && ((d >> bit) & 1);
-        const uint64_t bb = ballot(set);
-        peers &= set ? bb : ~bb;
-      }
+      const uint64_t peers = match_digit_lds(d >= 0 ? d : 0, d >= 0, lane, mt);
       const int rank = popc64(peers & below);
       uint32_t base = 0;
       if (d >= 0) base = wcnt[wave][d];
@@ -335,26 +354,6 @@ __device__ __forceinline__ uint64_t match_digit(int d, bool ok) {
     const uint64_t bb = ballot(set);
     peers &= set ? bb : ~bb;
   }
-  return peers;
-}
-
-// The same set of lanes through LDS: every valid lane ORs its bit into its
-// digit's 64-bit lane mask, reads the mask back and clears it.  Three LDS
-// operations instead of eight ballots with their per-lane selects; lanes
-// that share a digit contend for one word, which the LDS serialises.  `mt`
-// holds two words per digit (lanes 0..31, 32..63), all zero on entry, and is
-// left zero.
-__device__ __forceinline__ uint64_t match_digit_lds(int d, bool ok, int lane, uint32_t* mt) {
-  if (ok) atomicOr(&mt[2 * d + (lane >> 5)], 1u << (lane & 31));
-  __builtin_amdgcn_wave_barrier();
-  uint64_t peers = 0;
-  if (ok) {
-    const uint2 m = *reinterpret_cast<const uint2*>(&mt[2 * d]);
-    peers = ((uint64_t)m.y << 32) | m.x;
-  }
-  __builtin_amdgcn_wave_barrier();
-  if (ok) *reinterpret_cast<uint2*>(&mt[2 * d]) = make_uint2(0u, 0u);
-  __builtin_amdgcn_wave_barrier();
   return peers;
 }
 
@@ -530,13 +529,22 @@ __global__ __launch_bounds__(W * kWave, WPS) void ps_seg_group(
   constexpr int kOwners = kRadix / DPT;
   static_assert(kOwners * DPT == kRadix && kOwners <= kThreads, "digit owners");
   constexpr int kCap = kThreads * RD;
+  // Digit match through LDS lane masks where the LDS has room (up to 8
+  // waves), else by ballots.
+  constexpr bool kLdsMatch = W <= 8;
   __shared__ uint32_t s_buf[kCap];
   __shared__ uint32_t wcnt[W][kRadix];
+  __shared__ uint2 s_match[kLdsMatch ? W : 1][kLdsMatch ? kRadix : 1];
   __shared__ int s_wsum[W];
   __shared__ uint32_t s_diff, s_max;
   __shared__ int s_mixed;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t below = lanes_below(lane);
+  uint32_t* const mt = reinterpret_cast<uint32_t*>(s_match[kLdsMatch ? wave : 0]);
+  if (kLdsMatch) {
+    for (int i = lane; i < 2 * kRadix; i += kWave) mt[i] = 0u;
+    __builtin_amdgcn_wave_barrier();
+  }
   const int nseg = min(*big_count, total);
   for (int b = blockIdx.x; b < nseg; b += gridDim.x) {
     const int seg = big_list[b];
@@ -609,7 +617,8 @@ __global__ __launch_bounds__(W * kWave, WPS) void ps_seg_group(
       for (int r = 0; r < RD; ++r) {
         if (r >= rounds) continue;  // uniform
         const int d = (int)((lo[r] >> shift) & 0xffu);
-        const uint64_t peers = match_digit(d, true);
+        const uint64_t peers =
+            kLdsMatch ? match_digit_lds(d, true, lane, mt) : match_digit(d, true);
         const uint32_t rank = (uint32_t)popc64(peers & below);
         const uint32_t base = wcnt[wave][d];
         __builtin_amdgcn_wave_barrier();
