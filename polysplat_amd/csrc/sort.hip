@@ -529,22 +529,17 @@ __global__ __launch_bounds__(W * kWave, WPS) void ps_seg_group(
   constexpr int kOwners = kRadix / DPT;
   static_assert(kOwners * DPT == kRadix && kOwners <= kThreads, "digit owners");
   constexpr int kCap = kThreads * RD;
-  // Digit match through LDS lane masks where the LDS has room (up to 8
-  // waves), else by ballots.
-  constexpr bool kLdsMatch = W <= 8;
+  // The digit match's lane-mask tables (match_digit_lds) live in the
+  // exchange buffer, which holds nothing while a pass ranks its items.
+  static_assert(kCap >= W * 2 * kRadix, "lane-mask tables fit in the exchange buffer");
   __shared__ uint32_t s_buf[kCap];
   __shared__ uint32_t wcnt[W][kRadix];
-  __shared__ uint2 s_match[kLdsMatch ? W : 1][kLdsMatch ? kRadix : 1];
   __shared__ int s_wsum[W];
   __shared__ uint32_t s_diff, s_max;
   __shared__ int s_mixed;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t below = lanes_below(lane);
-  uint32_t* const mt = reinterpret_cast<uint32_t*>(s_match[kLdsMatch ? wave : 0]);
-  if (kLdsMatch) {
-    for (int i = lane; i < 2 * kRadix; i += kWave) mt[i] = 0u;
-    __builtin_amdgcn_wave_barrier();
-  }
+  uint32_t* const mt = s_buf + wave * 2 * kRadix;
   const int nseg = min(*big_count, total);
   for (int b = blockIdx.x; b < nseg; b += gridDim.x) {
     const int seg = big_list[b];
@@ -612,13 +607,13 @@ __global__ __launch_bounds__(W * kWave, WPS) void ps_seg_group(
       const int shift = 8 * p;
       if (((all_diff >> shift) & 0xffu) == 0) continue;  // uniform: constant digit
       for (int i = lane; i < kRadix; i += kWave) wcnt[wave][i] = 0;
+      for (int i = lane; i < 2 * kRadix; i += kWave) mt[i] = 0u;
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int r = 0; r < RD; ++r) {
         if (r >= rounds) continue;  // uniform
         const int d = (int)((lo[r] >> shift) & 0xffu);
-        const uint64_t peers =
-            kLdsMatch ? match_digit_lds(d, true, lane, mt) : match_digit(d, true);
+        const uint64_t peers = match_digit_lds(d, true, lane, mt);
         const uint32_t rank = (uint32_t)popc64(peers & below);
         const uint32_t base = wcnt[wave][d];
         __builtin_amdgcn_wave_barrier();
