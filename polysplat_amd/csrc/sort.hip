@@ -94,14 +94,32 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_upsweep(const uint64_t* 
   const int t_begin = b * pl.tiles_per_block;
   int t_end = t_begin + pl.tiles_per_block;
   if (t_end > pl.ntiles) t_end = pl.ntiles;
+  // The next tile's keys are loaded before this tile's counts are added, so
+  // the loads of consecutive tiles overlap.
+  uint64_t k[kSortItems];
+  int tile_n = 0;
+  if (t_begin < t_end) {
+    const long long tile0 = (long long)t_begin * kSortTile;
+    tile_n = (pl.n - tile0 < kSortTile) ? (int)(pl.n - tile0) : kSortTile;
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+      const int i = tile_item(wave, r, lane);
+      k[r] = (i < tile_n) ? keys[tile0 + i] : 0ull;
+    }
+  }
   for (int t = t_begin; t < t_end; ++t) {
-    const long long tile0 = (long long)t * kSortTile;
-    const int tile_n = (pl.n - tile0 < kSortTile) ? (int)(pl.n - tile0) : kSortTile;
     int d[kSortItems];
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {  // all loads issue before any use
-      const int i = tile_item(wave, r, lane);
-      d[r] = (i < tile_n) ? digit_of(keys[tile0 + i], spec) : -1;
+    for (int r = 0; r < kSortItems; ++r)
+      d[r] = (tile_item(wave, r, lane) < tile_n) ? digit_of(k[r], spec) : -1;
+    if (t + 1 < t_end) {
+      const long long ntile0 = (long long)(t + 1) * kSortTile;
+      tile_n = (pl.n - ntile0 < kSortTile) ? (int)(pl.n - ntile0) : kSortTile;
+#pragma unroll
+      for (int r = 0; r < kSortItems; ++r) {
+        const int i = tile_item(wave, r, lane);
+        k[r] = (i < tile_n) ? keys[ntile0 + i] : 0ull;
+      }
     }
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r)
