@@ -276,7 +276,7 @@ class GpuExecutor:
         curr = torch.empty(1, dtype=torch.int32, device=dev)
         ranges = torch.empty((r.tiles, 2), dtype=torch.int32, device=dev)
         ws = torch.empty(self.ops.forward_workspace_size(sc["P"]), dtype=torch.uint8, device=dev)
-        ws[:256].zero_()  # persistent-queue words start at zero
+        ws[:2048].zero_()  # the 2 KiB tile-queue area (per-XCD counters) starts at zero
         rec = torch.empty(24 * max(sc["P"], 1), dtype=torch.uint8, device=dev)
         out = torch.empty((r.height, r.width, 3), dtype=torch.uint8, device=dev)
         cam = orbit_camera(r.width, r.height, r.azimuth, self.radius)
