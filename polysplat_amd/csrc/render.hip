@@ -190,9 +190,12 @@ __device__ __forceinline__ float staged_power(const float4& A, const float4& B, 
 // Relative costs of the two shading paths (per Gaussian for the dense one,
 // per live (Gaussian, quadrant) for the sparse one), from their VALU counts
 // (about 39 and 12) with the sparse one weighted up: its LDS reads and
-// scalar loop control are not free.  14 was measured best among 10..24
-// (C3 render 0.573 ms at 12, 0.547-0.552 at 14, 0.577 at 17; 30M Gaussians
-// at 3840x2160 2.99 / 2.88-2.96 / 2.96).
+// scalar loop control are not free.  14 was measured best among 10..28
+// (6M Gaussians at 3840x2160: render 0.573 ms at 12, 0.547-0.552 at 14,
+// 0.577 at 17; 30M Gaussians at 3840x2160 2.99 / 2.88-2.96 / 2.96).  A
+// dense-only kernel (66 VGPRs) is 5-8% faster on those two scenes but 33%
+// slower on the 56M-Gaussian city at 9216x6912, whose small splats leave
+// most groups sparse.
 constexpr int kDenseCost = 39;
 constexpr int kSparseCost = 14;
 constexpr int kSparseGroupCost = 2;
@@ -284,8 +287,9 @@ __device__ __forceinline__ void render_tile(
             }
           }
         } else {
-          // Dense: every Gaussian of the group on all four pixels.
-#pragma unroll 2
+          // Dense: every Gaussian of the group on all four pixels (not
+          // unrolled: the persistent kernel fits 72 VGPRs, 7 waves per SIMD).
+#pragma unroll 1
           for (int j = 0; j < CHECK; ++j) {
             const int k = g * CHECK + j;
             const float4 A = sA[cur][k];
@@ -530,8 +534,12 @@ struct TileQueue {
 // quads = 1: the queue's items are (tile, quadrant) pairs, item 4 t + q,
 // each rendered by render_quad; 2: so when the frame averages at least
 // kQuadPairs pairs per tile, else whole tiles; 0: whole tiles.
+// Seven waves per SIMD: measured 2-4% faster than six (6M Gaussians at
+// 3840x2160: render 0.533 vs 0.545 ms; 30M at 7680x4320: 3.59 vs 3.72 ms)
+// and level on the city scene; the few values spilled are reloaded once per
+// tile.
 template <int PREC, bool RANGES, bool GATHERED, int CHECK, bool DT>
-__global__ __launch_bounds__(kWave, 6) void ps_render_persistent(
+__global__ __launch_bounds__(kWave, 7) void ps_render_persistent(
     const uint64_t* __restrict__ keys, const int* __restrict__ d_n, int n_host,
     const int2* __restrict__ ranges, int W, int H, int gx, int tiles, int quads,
     const uint32_t* __restrict__ order, const float2* __restrict__ xy,
@@ -647,7 +655,7 @@ int persistent_grid(K kernel, int tiles) {
 // tiles otherwise.  Measured (render ms, whole tiles -> quadrants): 1280x720
 // with 3M / 30M Gaussians 0.52 -> 0.32 / 1.27 -> 0.78; 1920x1080 with 10M
 // (1532 pairs per tile) 1.14 -> 0.91 but with 1M (153) 0.15 -> 0.20;
-// 3840x2160 at C3 0.59 -> 0.67.  POLYSPLAT_RENDER_QUADS=0 or 1 forces it.
+// 3840x2160 with 6M 0.59 -> 0.67.  POLYSPLAT_RENDER_QUADS=0 or 1 forces it.
 int quadrant_items(int tiles, int waves) {
   const char* e = getenv("POLYSPLAT_RENDER_QUADS");
   if (e && (e[0] == '0' || e[0] == '1')) return e[0] == '1';
