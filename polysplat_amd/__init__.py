@@ -8,8 +8,8 @@ A drop-in for the reference's ``polysplat.ops`` hot-path operators and for
 
 The operators run hand-written HIP kernels for gfx950 through the C ABI in
 ``include/polysplat_amd.h``.  There is no CPU fallback: importing the package
-raises if the native extension has not been built (``python -m
-polysplat_amd._build``).
+raises if the native extension has not been built (``python
+polysplat_amd/_build.py``).
 """
 from __future__ import annotations
 
@@ -25,7 +25,7 @@ try:
 except ImportError as exc:  # pragma: no cover - exercised only on broken installs
     raise ImportError(
         "polysplat_amd: the native extension is not built or cannot be loaded "
-        f"({exc}); run `python -m polysplat_amd._build` first"
+        f"({exc}); run `python polysplat_amd/_build.py` first"
     ) from exc
 
 ops = _C.ops

@@ -7,7 +7,7 @@
   calls the C ABI.
 
 Both land next to this file so they travel with the source tree.  Run
-``python -m polysplat_amd._build`` (or ``__graft_entry__.build()``).
+``python polysplat_amd/_build.py`` (or ``__graft_entry__.build()``).
 """
 from __future__ import annotations
 

@@ -100,16 +100,25 @@ void check_scene(const Tensor& pts, const Tensor& shs, const Tensor& op, const T
 }
 
 // Scratch owned by the module for bindings whose reference signature has no
-// workspace argument.  Grown on demand from torch's caching allocator and
-// zero-filled when created (the persistent queue words must start at 0).
+// workspace argument (preprocess workspaces, the persistent renderer's tile
+// queue, the fused forwards' workspace).  One buffer per (purpose, device,
+// stream): calls on different streams never share a queue or a workspace,
+// and calls on one stream are ordered, so a buffer is never in use by two
+// kernels at once.  Created zero-filled on the calling stream (the queue
+// words must start at 0; every launch leaves them at 0) and grown on demand;
+// a grown buffer replaces one that only earlier work on the same stream
+// used, so torch's caching allocator hands its memory out again only to
+// work ordered after it.  Steady-state calls allocate nothing.
 Tensor cached_workspace(const c10::Device& dev, size_t bytes, const char* tag) {
   static std::mutex mu;
-  static std::map<std::pair<std::string, int>, Tensor> cache;
+  static std::map<std::tuple<std::string, int, uintptr_t>, Tensor> cache;
   std::lock_guard<std::mutex> g(mu);
-  auto key = std::make_pair(std::string(tag), (int)dev.index());
+  const uintptr_t s = reinterpret_cast<uintptr_t>(cur_stream());
+  auto key = std::make_tuple(std::string(tag), (int)dev.index(), s);
   auto it = cache.find(key);
   if (it == cache.end() || (size_t)it->second.numel() < bytes) {
-    Tensor t = at::zeros({(int64_t)bytes}, at::TensorOptions().dtype(at::kByte).device(dev));
+    Tensor t = at::zeros({(int64_t)(bytes > 0 ? bytes : 1)},
+                         at::TensorOptions().dtype(at::kByte).device(dev));
     cache[key] = t;
     return t;
   }
@@ -137,7 +146,7 @@ void preprocess_impl(Tensor& orig_points, Tensor& shs, Tensor& opacities, Tensor
   ps_camera cam = make_camera(width, height, block_x, block_y, position, rotation, focal_x,
                               focal_y, zFar, zNear);
   const size_t wsb = ps_preprocess_workspace_size(P);
-  Tensor ws = at::empty({(int64_t)wsb}, opacities.options().dtype(at::kByte));
+  Tensor ws = cached_workspace(opacities.device(), wsb, "preprocess");
   check(ps_preprocess(P, ptr<float>(orig_points), shs.data_ptr(), sh_prec_of(shs),
                       ptr<float>(opacities), ptr<float>(cov3Ds), &cam, ptr<float>(points_xy),
                       ptr<float>(rgb_depth), ptr<float>(conic_opacity), nullptr, PS_PREC_F16,
@@ -163,7 +172,7 @@ void preprocess_records(Tensor& orig_points, Tensor& shs, Tensor& opacities, Ten
   ps_camera cam = make_camera(width, height, block_x, block_y, position, rotation, focal_x,
                               focal_y, zFar, zNear);
   const size_t wsb = ps_preprocess_workspace_size(P);
-  Tensor ws = at::empty({(int64_t)wsb}, opacities.options().dtype(at::kByte));
+  Tensor ws = cached_workspace(opacities.device(), wsb, "preprocess");
   check(ps_preprocess(P, ptr<float>(orig_points), shs.data_ptr(), sh_prec_of(shs),
                       ptr<float>(opacities), ptr<float>(cov3Ds), &cam, nullptr, nullptr, nullptr,
                       records.data_ptr(), feat_prec, ptr<uint64_t>(keys_unsorted),
@@ -319,12 +328,15 @@ void render_lite_fused(int num_rendered, int width, int height, Tensor& xy, Tens
                     "render_16x16_preranges_smem_persistent_lite_fused");
 }
 
-// Persistent render of 24-byte records (FP16/BF16 features).
+// Persistent render of 24-byte records (FP16/BF16 features); dt skips
+// pixels whose alpha is below 1/255, as the _dt FP32 variant does.
 void render_records(int num_rendered, int width, int height, Tensor& records, int feat_prec,
                     bool gathered, Tensor& values_sorted, Tensor& tile_ranges, Tensor& bg,
-                    Tensor& out) {
+                    Tensor& out, bool dt) {
+  need(feat_prec == PS_PREC_F16 || feat_prec == PS_PREC_BF16,
+       "render_records: feat_prec must be 1 (fp16) or 2 (bf16)");
   persistent_common(num_rendered, width, height, nullptr, nullptr, nullptr, &records, feat_prec,
-                    gathered, nullptr, values_sorted, &tile_ranges, false, bg, out,
+                    gathered, nullptr, values_sorted, &tile_ranges, dt, bg, out,
                     "render_records");
 }
 
@@ -380,13 +392,21 @@ int forward_common(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width,
   int cap = (int)std::min({a.ku->numel(), a.vu->numel(), a.ks->numel(), a.vs->numel()});
   while (cap > 0 && ps_get_sort_buffer_size(cap) > fb.sort_space_bytes) cap /= 2;
   fb.capacity = cap;
+  // The preprocess writes features of every emitting Gaussian, so the
+  // feature buffers must hold all P of them.
   if (feat_prec == PS_PREC_F32) {
     check_feats(*a.xy, *a.rgbd, *a.con);
+    need(a.xy->numel() >= 2LL * P && a.rgbd->numel() >= 4LL * P && a.con->numel() >= 4LL * P,
+         std::string(what) + ": points_xy, rgb_depth and conic_opacity must hold P entries");
     fb.points_xy = ptr<float>(*a.xy);
     fb.rgb_depth = ptr<float>(*a.rgbd);
     fb.conic_opacity = ptr<float>(*a.con);
   } else {
+    need(feat_prec == PS_PREC_F16 || feat_prec == PS_PREC_BF16,
+         std::string(what) + ": unknown feature precision");
     need_dev(*a.records, "records", at::kByte);
+    need(a.records->numel() >= 24LL * P,
+         std::string(what) + ": records must hold 24 bytes per Gaussian");
     fb.records = a.records->data_ptr();
   }
   if (a.ranges) {
@@ -597,7 +617,10 @@ PYBIND11_MODULE(_C, m) {
   ops.def("render_16x16_preranges_smem_persistent_lite", &render_lite);
   ops.def("render_16x16_preranges_smem_persistent_lite_dt", &render_lite_dt);
   ops.def("render_16x16_preranges_smem_persistent_lite_fused", &render_lite_fused);
-  ops.def("render_records", &render_records);
+  ops.def("render_records", &render_records, py::arg("num_rendered"), py::arg("width"),
+          py::arg("height"), py::arg("records"), py::arg("feat_prec"), py::arg("gathered"),
+          py::arg("values_sorted"), py::arg("tile_ranges"), py::arg("bg_color"),
+          py::arg("out_color"), py::arg("dt") = false);
   ops.def("forward_fused", &forward_fused);
   ops.def("forward_fused_e2e", &forward_fused_e2e);
   ops.def("forward_fused_e2e_preranges", &forward_fused_e2e_preranges);

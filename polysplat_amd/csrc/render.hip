@@ -506,15 +506,17 @@ struct TileQueue {
   __device__ TileQueue(unsigned* queue, int ntiles, int l)
       : q(queue), tiles(ntiles), lane(l), home((int)(blockIdx.x % kQueueLines)) {}
   __device__ int lo(int x) const { return (int)(((long long)x * tiles) / kQueueLines); }
-  // The next tile, or -1 when every queue is drained.
+  // The next tile, or -1 when every queue is drained.  The claim is
+  // compared unsigned, so a counter that does not start at zero can only
+  // end the queue early, never yield a tile outside it.
   __device__ int claim() {
     while (probe < kQueueLines) {
       const int x = (home + probe) % kQueueLines;
       const int a = lo(x), len = lo(x + 1) - a;
-      int c = 0;
-      if (lane == 0 && len > 0) c = (int)atomicAdd(&q[x * kQueueStride], 1u);
-      c = bcast(c, 0);
-      if (len > 0 && c < len) return a + c;
+      unsigned c = 0;
+      if (lane == 0 && len > 0) c = atomicAdd(&q[x * kQueueStride], 1u);
+      c = (unsigned)bcast((int)c, 0);
+      if (len > 0 && c < (unsigned)len) return a + (int)c;
       ++probe;  // drained for good: counters only grow
     }
     return -1;
@@ -604,9 +606,10 @@ __global__ __launch_bounds__(kWave, 6) void ps_render_persistent_batch(
                                            order, F.xy, F.rgbd, F.conic, F.rec,
                                            make_float3(F.bg[0], F.bg[1], F.bg[2]), F.out, sA, sB,
                                            sC, lane);
-    int next = 0;
-    if (lane == 0) next = (int)atomicAdd(&queue[0], 1u) + (int)gridDim.x;
-    tile = bcast(next, 0);
+    unsigned next = 0;
+    if (lane == 0) next = atomicAdd(&queue[0], 1u) + gridDim.x;
+    next = (unsigned)bcast((int)next, 0);
+    tile = next < (unsigned)tiles ? (int)next : tiles;  // unsigned: see TileQueue::claim
   }
   if (lane == 0) {
     const unsigned done = atomicAdd(&queue[1], 1u);

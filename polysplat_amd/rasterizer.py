@@ -259,7 +259,7 @@ class Rasterizer:
         elif prec:
             ops.render_records(num_rendered, camera.width, camera.height, self.records, prec,
                                False, self.gaussian_values_sorted, self.tile_ranges_buf,
-                               bg_color, out_color)
+                               bg_color, out_color, render_variant.endswith("_dt"))
         elif render_variant.endswith("_dt"):
             ops.render_16x16_preranges_smem_persistent_lite_dt(
                 num_rendered, camera.width, camera.height, self.points_xy, self.rgb_depth,

@@ -60,7 +60,7 @@ def lib():
     global _lib
     if _lib is None:
         if not LIB.exists():
-            raise RuntimeError(f"{LIB} is not built; run python -m polysplat_amd._build")
+            raise RuntimeError(f"{LIB} is not built; run python polysplat_amd/_build.py")
         _lib = bind(C.CDLL(str(LIB)))
     return _lib
 
