@@ -531,6 +531,17 @@ def summarize(done, rejected, wall_s: float, deadline_ms: float = 1000.0 / 60.0,
     return res
 
 
+def _ops():
+    from . import _C
+
+    return _C.ops
+
+
+def make_executor(scenes_dev, device="cuda:0", **kw):
+    """The executor the benchmark and the GPU tests serve with."""
+    return SequentialExecutor(scenes_dev, device=device, **kw)
+
+
 def make_resident_scenes(sizes=SCENE_SIZES, device="cuda:0", seed0: int = 101, box: float = BOX,
                          mix=None):
     """Scenes generated on the device with the C2 distributions (uniform
