@@ -173,6 +173,26 @@ int ps_render_16x16_persistent(int num_rendered, const int* d_num_rendered, int 
                                const int* tile_ranges, int dt, const float bg[3],
                                uint8_t* out_color, unsigned* queue, ps_stream_t stream);
 
+/* Same as ps_render_16x16_persistent without the `queue` argument: the tile
+ * queue lives in 2 KiB of static device memory the library holds on each
+ * device for the life of the process, so the call allocates nothing and
+ * needs no workspace from the caller.  No counterpart in the reference (whose
+ * renderer keeps a __device__ global counter, render.cu:1537, that callers
+ * cannot reset); it is for callers that cannot allocate device memory:
+ * code under HIP graph capture, tight serving loops.  The queue needs no
+ * more room for larger frames, so every size ps_render_16x16_persistent
+ * accepts is accepted.  The queue is cleared on the stream before each
+ * render (by a kernel, so the call can be captured).  All calls on one
+ * device share that one queue: calls that overlap on a device race and must
+ * be serialised by the caller (one stream, or events between streams). */
+int ps_render_16x16_persistent_noalloc(int num_rendered, const int* d_num_rendered, int width,
+                                       int height, int feat_prec, const float* points_xy,
+                                       const float* rgb_depth, const float* conic_opacity,
+                                       const void* records, int gathered,
+                                       const uint64_t* keys_sorted, const uint32_t* values_sorted,
+                                       const int* tile_ranges, int dt, const float bg[3],
+                                       uint8_t* out_color, ps_stream_t stream);
+
 /* ---------------------------------------------------------------- forward
  * One frame on one stream: preprocess -> sort -> ranges -> render, with the
  * pair count passed between stages in device memory.  Replaces forward_fused

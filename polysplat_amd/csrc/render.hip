@@ -28,6 +28,14 @@
 
 #include <stdlib.h>
 
+#include <mutex>
+
+// Tile-queue words of ps_render_16x16_persistent_noalloc: 2 KiB, the same
+// area a caller hands ps_render_16x16_persistent as `queue`.  Each device
+// holds its own copy for the life of the process.  Host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device
+// (see noalloc_queue below); the array's name is not a device address.
+static __device__ __align__(256) unsigned scratch[512];
 
 namespace ps {
 namespace {
@@ -795,6 +803,57 @@ int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_
   }
 }
 
+// ------------------------------------------------ no-allocation entry point
+namespace {
+
+constexpr int kMaxDevices = 64;
+static_assert(sizeof(scratch) >= 4 * (kQueueDone + 1), "queue words fit the static scratch");
+
+// Clears the queue words before a launch (a kernel rather than a memset, so
+// that the call can be recorded into a HIP graph).
+__global__ __launch_bounds__(256) void ps_queue_clear(unsigned* __restrict__ q, int words) {
+  for (int i = threadIdx.x; i < words; i += 256) q[i] = 0u;
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.
+int noalloc_queue(unsigned** out) {
+  static std::mutex mu;
+  static void* addr[kMaxDevices] = {};
+  int dev = 0;
+  PS_HIP_CHECK(hipGetDevice(&dev));
+  PS_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d the library tracks", dev,
+             kMaxDevices);
+  std::lock_guard<std::mutex> g(mu);
+  if (addr[dev] == nullptr) {
+    void* p = nullptr;
+    PS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PS_REQUIRE(p != nullptr, "no device address for the static tile queue");
+    addr[dev] = p;
+  }
+  *out = static_cast<unsigned*>(addr[dev]);
+  return PS_OK;
+}
+
+}  // namespace
+
+int render_persistent_noalloc(int n_host, const int* d_n, int W, int H, int feat_prec,
+                              const float* xy, const float* rgbd, const float* con,
+                              const void* rec, int gathered, const uint64_t* keys,
+                              const uint32_t* order, const int* ranges, int dt, const float bg[3],
+                              uint8_t* out, hipStream_t stream) {
+  unsigned* q = nullptr;
+  const int st = noalloc_queue(&q);
+  if (st != PS_OK) return st;
+  // The queue needs no more words for larger frames: every size the
+  // allocating entry point accepts is accepted here.
+  const int words = (int)(sizeof(scratch) / sizeof(unsigned));
+  hipLaunchKernelGGL(ps_queue_clear, dim3(1), dim3(256), 0, stream, q, words);
+  PS_LAUNCH_CHECK();
+  return render_persistent_launch(n_host, d_n, W, H, feat_prec, xy, rgbd, con, rec, gathered,
+                                  keys, order, ranges, dt, bg, out, q, stream);
+}
+
 }  // namespace ps
 
 extern "C" int ps_render_16x16(int num_rendered, const int* d_num_rendered, int width, int height,
@@ -819,4 +878,15 @@ extern "C" int ps_render_16x16_persistent(int num_rendered, const int* d_num_ren
                                       points_xy, rgb_depth, conic_opacity, records, gathered,
                                       keys_sorted, values_sorted, tile_ranges, dt, bg, out_color,
                                       queue, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int ps_render_16x16_persistent_noalloc(
+    int num_rendered, const int* d_num_rendered, int width, int height, int feat_prec,
+    const float* points_xy, const float* rgb_depth, const float* conic_opacity,
+    const void* records, int gathered, const uint64_t* keys_sorted, const uint32_t* values_sorted,
+    const int* tile_ranges, int dt, const float bg[3], uint8_t* out_color, ps_stream_t stream) {
+  return ps::render_persistent_noalloc(num_rendered, d_num_rendered, width, height, feat_prec,
+                                       points_xy, rgb_depth, conic_opacity, records, gathered,
+                                       keys_sorted, values_sorted, tile_ranges, dt, bg, out_color,
+                                       reinterpret_cast<hipStream_t>(stream));
 }

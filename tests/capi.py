@@ -87,6 +87,8 @@ def bind(L):
         "ps_render_16x16": ([I, P, I, I, P, P, P, P, P, P, P, P], I),
         "ps_render_16x16_persistent": ([I, P, I, I, I, P, P, P, P, I, P, P, P, I, P, P, P, P],
                                        I),
+        "ps_render_16x16_persistent_noalloc": ([I, P, I, I, I, P, P, P, P, I, P, P, P, I, P, P,
+                                                P], I),
         "ps_forward_workspace_size": ([I], S),
         "ps_forward": ([I, P, P, I, P, P, P, I, I, P, P, P, I, P, P], I),
         "ps_forward_batch_workspace_size": ([I, P], S),

@@ -257,6 +257,13 @@ inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
   *v = 4;  // few compute units: persistent grids stay small
   return hipSuccess;
 }
+// Device symbols are host statics here: their address is the symbol itself.
+#define __align__(n) __attribute__((aligned(n)))
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
+  *p = const_cast<void*>(symbol);
+  return hipSuccess;
+}
 template <typename K>
 inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, K, int, size_t) {
   *n = 2;
