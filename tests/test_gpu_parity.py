@@ -278,3 +278,33 @@ def test_forward_workspace_reused_across_scene_sizes(gpu):
         st, _, img = small.forward(BG, capi.V_LITE, ws=shared)
         capi.check(st)
         np.testing.assert_array_equal(img, fresh)
+
+
+@pytest.mark.parametrize("name", ["c1_small", "mixed_small"])
+def test_golden_fixtures(gpu, name):
+    """The committed fixtures (tests/golden/): keys, order and both range
+    conventions bit-exact, every image within one 8-bit level."""
+    from tests.test_golden import arrays_of, load, spec_of
+
+    d = load(name)
+    M = int(d["num_rendered"])
+    arrays, spec, bg = arrays_of(d), spec_of(d), tuple(d["bg"])
+    g = Gpu(arrays, spec, capacity=M + 64)
+    assert g.preprocess() == M
+    np.testing.assert_array_equal(g.ku[:M].cpu().numpy().view(np.uint64), d["keys"])
+    np.testing.assert_array_equal(g.vu[:M].cpu().numpy().view(np.uint32), d["values"])
+    g.sort()
+    np.testing.assert_array_equal(g.ks[:M].cpu().numpy().view(np.uint64), d["keys_sorted"])
+    np.testing.assert_array_equal(g.vs[:M].cpu().numpy().view(np.uint32), d["values_sorted"])
+    np.testing.assert_array_equal(g.ranges_scan(), d["ranges_scan"])
+    np.testing.assert_array_equal(g.ranges_bsearch(), d["ranges"])
+    for img, ref in ((g.render_plain(bg), "image_check2"),
+                     (g.render_persistent(bg, use_ranges=True), "image_check8")):
+        m = diff_metrics(img, d[ref])
+        assert m["max_abs_diff"] <= 1, (ref, m)
+        assert diff_metrics(img, d["image_fp64"])["max_abs_diff"] <= 1
+    h = Gpu(arrays, spec, sh_type="fp16", feat_prec="fp16", capacity=M + 64)
+    st, n, img = h.forward(bg, capi.V_LITE)
+    capi.check(st)
+    assert n == M
+    assert diff_metrics(img, d["image_fp16_check8"])["max_abs_diff"] <= 1

@@ -332,7 +332,7 @@ def test_tile_ranges_modes():
 
 
 # ------------------------------------------------------------------ render
-def _render_one(feats, W=32, H=32, bg=(0.1, 0.2, 0.3), check=8):
+def _render_one(feats, W=32, H=32, bg=(0.1, 0.2, 0.3), check=8, dt=False):
     xy = np.array([f[0] for f in feats], F32)
     con = np.array([f[1] for f in feats], F32)
     rgb = np.array([list(f[2]) + [1.0] for f in feats], F32)
@@ -345,7 +345,7 @@ def _render_one(feats, W=32, H=32, bg=(0.1, 0.2, 0.3), check=8):
     vals = np.array([i for _, i in keys], np.uint32)
     k = np.array([(t << 32) for t, _ in keys], np.uint64)
     ranges = O.tile_ranges(k, gx * ((H + 15) // 16), 0)
-    return O.render_f32(W, H, ranges, vals, xy, rgb, con, bg, check)
+    return O.render_f32(W, H, ranges, vals, xy, rgb, con, bg, check, dt)
 
 
 def test_render_empty_tiles_write_background():
@@ -386,6 +386,26 @@ def test_render_u8_truncates_and_clamps():
     assert img[0, 0, 0] == 255  # clamp high
     assert img[0, 0, 1] == 63   # 0.25 * 255 = 63.75 -> 63 (truncation)
     assert img[0, 0, 2] == 0    # clamp low
+
+
+def test_render_alpha_skip_threshold_is_log2_of_one_over_255():
+    """The _dt renderers skip a pixel's blend when its power (log2 alpha) is
+    below log2(1/255) (render.cu:352-362: kept when power >= -7.994353375f).
+    A flat Gaussian (zero conic) puts the same power on every pixel, so the
+    output is rgb * 2^w + (1 - 2^w) * bg when kept and bg when skipped."""
+    thr = np.float32(-7.994353375)
+    assert abs(float(thr) - math.log2(1.0 / 255.0)) < 1e-6
+    below = np.nextafter(thr, np.float32(-np.inf))
+    rgb, bg = (0.8, 0.4, 0.2), (0.5, 0.25, 0.125)
+    for w in (thr, below, np.float32(-7.5), np.float32(-9.0)):
+        feats = [((8.0, 8.0), (0.0, 0.0, 0.0, float(w)), rgb)]
+        for dt in (False, True):
+            img, img_f, _, _ = _render_one(feats, bg=bg, dt=dt)
+            a = 2.0 ** float(w)
+            kept = (not dt) or w >= thr
+            want = [c * a + (1.0 - a) * b for c, b in zip(rgb, bg)] if kept else list(bg)
+            np.testing.assert_allclose(img_f[3, 11], want, rtol=2e-6, err_msg=f"w={w} dt={dt}")
+            assert (img_f == img_f[0, 0]).all()  # every pixel the same
 
 
 def test_render_fp32_matches_fp64_c1():
