@@ -45,9 +45,12 @@ def parse():
     ap.add_argument("--resolution", default="",
                     help="override the config's WxH (profiling other sizes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--paths", default="fp16_persistent,fp32_plain",
-                    help="comma-separated subset of fp16_persistent,fp32_plain")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads (default: OMP_NUM_THREADS, else the host's cores)")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="target length of the timed CPU-baseline sample")
+    ap.add_argument("--paths", default="fp16_persistent,bf16_persistent,fp32_plain",
+                    help="comma-separated subset of fp16_persistent,bf16_persistent,fp32_plain")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
     ap.add_argument("--serve-requests", type=int, default=1000,
@@ -167,6 +170,8 @@ def main():
     paths = {}
     if "fp16_persistent" in wanted:
         paths["fp16_persistent"] = (make_rast("fp16"), "preranges_smem_persistent_lite_e2e_preranges")
+    if "bf16_persistent" in wanted:
+        paths["bf16_persistent"] = (make_rast("bf16"), "preranges_smem_persistent_lite_e2e_preranges")
     if "fp32_plain" in wanted:
         paths["fp32_plain"] = (make_rast("fp32"), "default")
     VAR = polysplat.rasterizer.VARIANT_IDS
@@ -175,7 +180,7 @@ def main():
     def frame_fn(rast, variant):
         fp32 = variant == "default"
         prec = 0 if fp32 else PREC[rast.feature_dtype]
-        shs = scene.shs if fp32 else scene.shs_half
+        shs = scene.shs if fp32 else (scene.shs_bf16 if prec == 2 else scene.shs_half)
         v = VAR[variant]
         rng = rast.tile_ranges_buf if v == 1 else rast._empty
         rec = rast.records if prec else rast._empty
@@ -261,32 +266,78 @@ def main():
             stage_ms += [ev[i].elapsed_time(ev[i + 1]) for i in range(4)]
     stage_ms /= reps
 
+    # ---------------------------------------------------------- tail tile
+    # The longest tile's render time: the same launch with every other
+    # tile's range emptied, so one wave renders that tile alone (whole-tile
+    # work items, as the full frame uses when it has more tiles than waves).
+    full_ranges = rast.tile_ranges_buf[:T].clone()
+    t_max = int(np.argmax(lens))
+    one = torch.zeros_like(full_ranges)
+    one[t_max] = full_ranges[t_max]
+    tail = []
+    for r in range(7):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ops.render_records(M, W, H, rast.records, 1, False, rast.gaussian_values_sorted, one, bg,
+                           rast._get_out_color(H, W, dev))
+        e1.record()
+        torch.cuda.synchronize()
+        if r >= 2:
+            tail.append(e0.elapsed_time(e1))
+    rast.tile_ranges_buf[:T].copy_(full_ranges)
+    tail_ms = float(np.median(tail))
+    log(f"tail tile {t_max}: {int(lens[t_max])} pairs, {tail_ms:.4f} ms alone "
+        f"(render {stage_ms[3]:.4f} ms)")
+
     # ---------------------------------------------------------- oracle leg
     cpu = None
     need = None
     quality = {}
     V = None
+    shaded = None
     if not args.no_cpu:
+        if args.cpu_threads > 0:
+            os.environ["OMP_NUM_THREADS"] = str(args.cpu_threads)
         import oracle as O
 
-        os.environ.setdefault("OMP_NUM_THREADS", str(args.cpu_threads))
         pos = arrays.position.cpu().numpy()
         op = arrays.opacity.cpu().numpy()
         cov = arrays.cov3d.cpu().numpy()
         sh16 = arrays.shs.half().cpu().numpy().view(np.uint16)
         ocam = O.camera_from_spec(spec)
+
+        def cpu_frame():
+            return O.frame(pos, sh16, op, cov, ocam, (0, 0, 0), sh_type="fp16",
+                           feat_prec="fp16", check_every=8, fp64=False)
+
+        # CPU baseline: the oracle's whole FP16 frame on the host cores,
+        # warm-up frames first, then the median of the timed ones; as many
+        # (up to 3 + 10) as fit about --cpu-seconds.
         t = time.perf_counter()
-        ref16 = O.frame(pos, sh16, op, cov, ocam, (0, 0, 0), sh_type="fp16", feat_prec="fp16",
-                        check_every=8, fp64=False)
-        cpu_s = time.perf_counter() - t
+        ref16 = cpu_frame()
+        first_s = time.perf_counter() - t
+        n_timed = int(max(1, min(10, args.cpu_seconds // max(first_s, 1e-3))))
+        n_warm = min(2, n_timed - 1) if n_timed > 1 else 0
+        for _ in range(n_warm):
+            cpu_frame()
+        times = []
+        for _ in range(n_timed):
+            t = time.perf_counter()
+            cpu_frame()
+            times.append(time.perf_counter() - t)
+        cpu_s = float(np.median(times))
         need = int(ref16["need"].sum())
+        shaded = int(ref16["stop"].sum())
         V = int((ref16["pre"]["counts"] > 0).sum())
         threads = O.lib().orc_num_threads()
-        cpu = {"value": cpu_s * 1e3, "unit": "ms/frame", "cores": threads, "kind": "port",
-               "sample": f"one full {W}x{H} frame of the same {P}-Gaussian scene (FP16 SH and "
-                         f"features), oracle preprocess + stable sort + ranges + blend"}
-        log(f"oracle frame: {cpu_s:.1f}s on {threads} threads, M={ref16['pre']['M']}, "
-            f"M_need={need}")
+        cpu = {"value": round(cpu_s * 1e3, 2), "unit": "ms/frame", "cores": threads,
+               "kind": "port", "host_cpus": os.cpu_count(),
+               "sample": f"median of {n_timed} full {W}x{H} frames of the same {P}-Gaussian "
+                         f"scene (FP16 SH and features) after {1 + n_warm} warm-up frames: "
+                         "oracle preprocess + stable sort + ranges + blend, OpenMP over "
+                         "Gaussians and tiles"}
+        log(f"oracle frame: {cpu_s:.2f}s (median of {n_timed}) on {threads} threads, "
+            f"M={ref16['pre']['M']}, M_need={need}, shaded={shaded}")
         t = time.perf_counter()
         sh32 = arrays.shs.cpu().numpy()
         ref32 = O.frame(pos, sh32, op, cov, ocam, (0, 0, 0), sh_type="fp32", check_every=2,
@@ -294,6 +345,13 @@ def main():
         log(f"oracle fp32+fp64 frame: {time.perf_counter() - t:.1f}s")
         quality["fp16_persistent"] = diff_metrics(images["fp16_persistent"], ref32["image64"])
         quality["fp16_vs_oracle_fp16"] = diff_metrics(images["fp16_persistent"], ref16["image"])
+        if "bf16_persistent" in images:
+            sh_b = arrays.shs.to(torch.bfloat16).cpu().view(torch.int16).numpy().view(np.uint16)
+            ref_b = O.frame(pos, sh_b, op, cov, ocam, (0, 0, 0), sh_type="bf16",
+                            feat_prec="bf16", check_every=8, fp64=False)
+            quality["bf16_persistent"] = diff_metrics(images["bf16_persistent"], ref32["image64"])
+            quality["bf16_vs_oracle_bf16"] = diff_metrics(images["bf16_persistent"],
+                                                          ref_b["image"])
         if "fp32_plain" in images:
             quality["fp32_plain"] = diff_metrics(images["fp32_plain"], ref32["image64"])
             quality["fp32_vs_oracle_fp32"] = diff_metrics(images["fp32_plain"], ref32["image"])
@@ -338,19 +396,25 @@ def main():
         if "valu_insts_per_launch" in t:
             insts = float(t["valu_insts_per_launch"])
             rate = insts / (rend_ms * 1e-3) / 1e9
-            clk = t["grbm_gui_active_per_launch"] / 8 / (t["profiled_avg_us"] * 1e-6) / 1e9
             roof["valu"] = {"insts_per_launch": int(insts), "achieved_ginst_s": round(rate, 1),
-                            "peak_ginst_s": VALU_PEAK_GINST, "frac": round(rate / VALU_PEAK_GINST, 4),
-                            "profiled_clock_ghz": round(clk, 3),
-                            "frac_at_profiled_clock": round(
-                                insts / (1024 * t["grbm_gui_active_per_launch"] / 8 / 4), 4)}
+                            "peak_ginst_s": VALU_PEAK_GINST,
+                            "frac": round(rate / VALU_PEAK_GINST, 4)}
+            if shaded:
+                # wave-level instructions per (tile, Gaussian) pair the tiles
+                # actually blend (the oracle's stop points at cadence 8)
+                roof["valu"]["insts_per_shaded_pair"] = round(insts / shaded, 2)
     if Mn:
         # The render is bound by VALU issue (about a dozen VALU instructions
         # and one exp2 per pixel and Gaussian), not by HBM.
         roof["pixel_gaussians_per_s"] = Mn * 256 / (rend_ms * 1e-3)
+    roof["pairs_needed"] = need
+    roof["pairs_shaded"] = shaded
+    roof["tail_tile"] = {"tile": t_max, "pairs": int(lens[t_max]), "ms_alone": round(tail_ms, 4),
+                         "frac_of_render": round(tail_ms / rend_ms, 4)}
 
     fp16 = results["fp16_persistent"]["ms_per_frame"]
     fp32 = results.get("fp32_plain", {}).get("ms_per_frame", float("nan"))
+    bf16 = results.get("bf16_persistent", {}).get("ms_per_frame", float("nan"))
     line = {
         "metric": METRIC,
         "value": round(fp16, 4),
@@ -369,8 +433,11 @@ def main():
                    "gaussians": P, "width": W, "height": H, "num_rendered": M,
                    "num_rendered_needed": need},
         "fp32_plain_ms_per_frame": round(fp32, 4),
+        "bf16_persistent_ms_per_frame": round(bf16, 4),
         "speedup_fp16_persistent_vs_fp32_plain": round(fp32 / fp16, 3),
         "frames_per_s": round(1e3 / fp16, 2),
+        # the frame's algorithmic HBM bytes (all four stages) per frame time
+        "frame_hbm_GB_s": round(sum(alg.values()) / (fp16 * 1e-3) / 1e9, 1),
         "stages_fp16": stages,
         "quality_vs_fp64": {k: {"psnr_db": (None if v["psnr_db"] == float("inf")
                                             else round(v["psnr_db"], 2)),
@@ -381,7 +448,7 @@ def main():
         "cpu_baseline": cpu,
     }
     if args.serve_requests > 0:
-        del paths, rast, scene, arrays
+        del paths, rast, scene, arrays, one, full_ranges
         torch.cuda.empty_cache()
         line["serving"] = serve(args)
     print(json.dumps(line), flush=True)
