@@ -48,11 +48,15 @@ namespace {
 constexpr int kPreThreads = 256;
 constexpr int kMassive = 64;
 
-// What the count pass leaves for the emit pass about a Gaussian that emits:
-// rect origin and width (x0w = x0 | width << 24; width 0 marks a rect of more
-// than kMaskTiles tiles, which the emit pass projects and sweeps again), the
-// depth bits of its keys, and the hit mask over the rect (bit t: tile t of
-// the rect in row-major order).
+// What the count pass leaves for the emit pass about a Gaussian that emits,
+// indexing its candidate rect's tiles row-major (local index t):
+// * rect of at most kMaskTiles tiles: x0w = x0 | width << 24, y0, the depth
+//   bits of its keys, and mask = the hit mask (bit t: tile t hit);
+// * larger rect: x0w = x0 (width field 0), y0 = y0 | width << 16, depth, and
+//   mask = the first of its ceil(area / 64) words in the hit pool (word s,
+//   bit l: tile 64 s + l hit), the ballots of the count pass's tile sweep; or
+//   kNoPool when the pool was full, and the emit pass projects and sweeps
+//   that Gaussian again.
 struct alignas(16) EmitRec {
   uint32_t x0w;
   uint32_t y0;
@@ -61,9 +65,23 @@ struct alignas(16) EmitRec {
 };
 // state[0] = offset the frame's keys start at, state[1] = pair count for the
 // later stages (0 after overflow), state[2] = status bits (bit 0: overflow).
-constexpr int kStateBase = 0, kStateCount = 1, kStateFlags = 2, kStateWords = 64;
+// state[4..5]: words of the hit pool taken so far (a 64-bit counter).
+constexpr int kStateBase = 0, kStateCount = 1, kStateFlags = 2, kStatePool = 4, kStateWords = 64;
 
 constexpr int kMaskTiles = 32;
+constexpr uint32_t kNoPool = 0xffffffffu;
+// Hit pool: each count-pass workgroup owns kPoolPerBlock words at
+// vb * kPoolPerBlock (its Gaussians take them by a block-wide prefix over
+// their needs, with no atomics); a workgroup that needs more takes its words
+// from a shared overflow area behind them by one atomic add, and if that is
+// full too its large rects are swept again by the emit pass.  About one
+// Gaussian in a hundred has a rect of more than 32 tiles at the benchmark
+// workloads, most of them one or two words.
+constexpr int kPoolPerBlock = 64;
+inline size_t pool_overflow_words(int nb) { return (size_t)nb * 16 + 16384; }
+inline size_t pool_words_for(int nb) {
+  return (size_t)nb * kPoolPerBlock + pool_overflow_words(nb);
+}
 // The count pass adds each workgroup's pair total into the sum of its chunk
 // of kChunkBlocks workgroups, so an emit workgroup finds its offset from at
 // most chunks + kChunkBlocks sums.
@@ -158,6 +176,20 @@ __device__ __forceinline__ Cand permute_cand(const Cand& v, int src) {
 // (valid where ok).
 struct NullSink {
   __device__ __forceinline__ void step(bool, uint64_t, int, int, int) {}
+  // Massive route: step s of lane i's rect gave ballot m (tiles 64 s ..).
+  __device__ __forceinline__ void ballot(int, int, uint64_t) {}
+};
+
+// Count pass: keeps the massive route's ballots of each Gaussian that has a
+// hit-pool area (base >= 0) as its pool words.
+struct PoolSink {
+  uint64_t* pool;
+  int base, lane;
+  __device__ __forceinline__ void step(bool, uint64_t, int, int, int) {}
+  __device__ __forceinline__ void ballot(int i, int s, uint64_t m) {
+    const int b = bcast(base, i);
+    if (lane == 0 && b >= 0) pool[b + s] = m;
+  }
 };
 
 // Writes keys straight to global memory at off(owner) + k, never past the
@@ -176,6 +208,7 @@ struct GlobalSink {
       vals[o_off + k] = (uint32_t)o_gid;
     }
   }
+  __device__ __forceinline__ void ballot(int, int, uint64_t) {}
 };
 
 // Massive Gaussians: the wave walks each one's rect 64 tiles at a time in
@@ -205,6 +238,7 @@ __device__ int sweep_massive(const Cand& me, bool is_massive, int lane, int bx, 
       }
       const uint64_t m = ballot(ok);
       sink.step(ok, m, (q.ry0 + ty) * grid_x + (q.rx0 + tx), i, run + popc64(m & below));
+      sink.ballot(i, t0 / kWave, m);
       run += popc64(m);
     }
     if (lane == i) mine = run;
@@ -217,7 +251,7 @@ __device__ int sweep_massive(const Cand& me, bool is_massive, int lane, int bx, 
 // whose inclusive prefix of areas exceeds t.  Returns this lane's own count.
 template <class Sink>
 __device__ int sweep_fragmented(const Cand& me, bool is_frag, int lane, int bx, int by,
-                                int grid_x, Sink& sink, uint32_t& mask) {
+                                int grid_x, Sink& sink, uint64_t& mask) {
   const int area = is_frag ? (me.rx1 - me.rx0) * (me.ry1 - me.ry0) : 0;
   const int incl = wave_incl_scan(area, lane);
   const int excl = incl - area;
@@ -263,8 +297,9 @@ __device__ int sweep_fragmented(const Cand& me, bool is_frag, int lane, int bx, 
       if (lo_l < hi_l) {
         const uint64_t seg = m & lane_range(lo_l, hi_l);
         mine += popc64(seg);
+        // rect-local indices stay below the area, itself below kMassive = 64
         const int first_local = t0 + lo_l - excl;
-        if (first_local < kMaskTiles) mask |= (uint32_t)((seg >> lo_l) << first_local);
+        mask |= (seg >> lo_l) << first_local;
       }
     }
   }
@@ -273,15 +308,16 @@ __device__ int sweep_fragmented(const Cand& me, bool is_frag, int lane, int bx, 
 
 // All of a wave's (Gaussian, tile) hits, in the three routes: single-tile
 // rects (tested by their own lane), massive and fragmented.  Returns this
-// lane's count and, for rects of at most kMaskTiles tiles, its hit mask.
-// `single_ok` is this lane's result for a single-tile rect.
+// lane's count and, for rects of fewer than kMassive tiles, its hit mask
+// (bit t: rect-local tile t hit).  `single_ok` is this lane's result for a
+// single-tile rect.
 template <class Sink>
 __device__ __forceinline__ int sweep_all(const Cand& me, int area, bool single_ok, int lane,
                                          int bx, int by, int grid_x, Sink& sink,
-                                         uint32_t& mask) {
+                                         uint64_t& mask) {
   sink.step(single_ok, ballot(single_ok), me.ry0 * grid_x + me.rx0, lane, 0);
   int count = single_ok ? 1 : 0;
-  mask = single_ok ? 1u : 0u;
+  mask = single_ok ? 1ull : 0ull;
   count += sweep_massive(me, area >= kMassive, lane, bx, by, grid_x, sink);
   count += sweep_fragmented(me, area >= 2 && area < kMassive, lane, bx, by, grid_x, sink, mask);
   return count;
@@ -384,7 +420,8 @@ __device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* _
                                                unsigned long long* __restrict__ chunk_sums,
                                                int* __restrict__ state,
                                                const int* __restrict__ curr_offset,
-                                               int reset_offset) {
+                                               int reset_offset, uint64_t* __restrict__ pool,
+                                               long long pool_cap, long long pool_overflow) {
   __shared__ int scratch[4];
   const int tid = threadIdx.x, lane = tid & 63;
   const int g = (vb - F.block0) * kPreThreads + tid;
@@ -393,20 +430,49 @@ __device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* _
   const Proj pj = project_gaussian(F, g, px, py, pz);
   const int area = rect_area(pj);
   const bool single_ok = area == 1 && tile_hits(pj.rmin_x, pj.rmin_y, pj, cp.block_x, cp.block_y);
-  NullSink none;
-  uint32_t mask;
+  // Hit-pool words for rects of more than kMaskTiles tiles (see
+  // kPoolPerBlock): a block-wide prefix over the Gaussians' needs.
+  int pbase = -1;
+  if (pool) {  // uniform
+    __shared__ long long s_over;
+    const int need = area > kMaskTiles ? (area + kWave - 1) / kWave : 0;
+    int btot;
+    const int bexcl = block256_excl_scan(need, scratch, btot);
+    if (btot > 0) {  // uniform across the block
+      const long long own = (long long)vb * kPoolPerBlock;
+      long long start = own;
+      if (btot > kPoolPerBlock) {  // uniform: the shared overflow area
+        if (tid == 0) {
+          const unsigned long long b0 = atomicAdd(
+              reinterpret_cast<unsigned long long*>(state + kStatePool), (unsigned long long)btot);
+          s_over = (long long)b0;
+        }
+        __syncthreads();
+        const long long over0 = pool_cap - (long long)pool_overflow;
+        start = (s_over + btot <= (long long)pool_overflow) ? over0 + s_over : -1;
+      }
+      pbase = (need > 0 && start >= 0) ? (int)(start + bexcl) : -1;
+    }
+  }
+  PoolSink sink{pool, pbase, lane};
+  uint64_t mask;
   const int count = sweep_all(cand_from(pj), area, single_ok, lane, cp.block_x, cp.block_y,
-                              cp.grid_x, none, mask);
+                              cp.grid_x, sink, mask);
+  // Rects of kMaskTiles < area < kMassive: their one pool word is the
+  // fragmented route's mask (the massive route stored its ballots).
+  if (pbase >= 0 && area < kMassive) pool[pbase] = mask;
   counts[(size_t)vb * kPreThreads + tid] = count;
   if (recs && g < F.P) {
     // Written for every Gaussian (zeros when it emits nothing) so that each
     // wave stores whole cache lines.
-    const uint32_t w = area <= kMaskTiles ? (uint32_t)(pj.rmax_x - pj.rmin_x) : 0u;
+    const bool small = area <= kMaskTiles;
+    const uint32_t rw = (uint32_t)(pj.rmax_x - pj.rmin_x);
     EmitRec r;
-    r.x0w = count > 0 ? ((uint32_t)pj.rmin_x | (w << 24)) : 0u;
-    r.y0 = count > 0 ? (uint32_t)pj.rmin_y : 0u;
+    r.x0w = count > 0 ? ((uint32_t)pj.rmin_x | ((small ? rw : 0u) << 24)) : 0u;
+    r.y0 = count > 0 ? ((uint32_t)pj.rmin_y | (small ? 0u : rw << 16)) : 0u;
     r.depth = count > 0 ? f2u(pj.depth) : 0u;
-    r.mask = count > 0 ? mask : 0u;
+    r.mask = count > 0 ? (small ? (uint32_t)mask : (pbase >= 0 ? (uint32_t)pbase : kNoPool))
+                       : 0u;
     recs[(size_t)vb * kPreThreads + tid] = r;
   }
   if (count > 0) write_features<SHP>(F, g, pj, px, py, pz);
@@ -423,9 +489,10 @@ template <int SHP>
 __global__ __launch_bounds__(kPreThreads) void ps_pre_count(
     FrameDev F, int* __restrict__ counts, int* __restrict__ block_sums,
     EmitRec* __restrict__ recs, unsigned long long* __restrict__ chunk_sums,
-    int* __restrict__ state, const int* __restrict__ curr_offset, int reset_offset) {
+    int* __restrict__ state, const int* __restrict__ curr_offset, int reset_offset,
+    uint64_t* __restrict__ pool, long long pool_cap, long long pool_overflow) {
   pre_count_body<SHP>(F, blockIdx.x, counts, block_sums, recs, chunk_sums, state, curr_offset,
-                      reset_offset);
+                      reset_offset, pool, pool_cap, pool_overflow);
 }
 
 __device__ __forceinline__ int frame_of_block(const FrameDev* frames, int nf, int vb) {
@@ -438,7 +505,7 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_count_batch(const FrameDev
                                                                   int* __restrict__ block_sums) {
   const int f = frame_of_block(frames, nf, blockIdx.x);
   pre_count_body<SHP>(frames[f], blockIdx.x, counts, block_sums, nullptr, nullptr, nullptr,
-                      nullptr, 0);
+                      nullptr, 0, nullptr, 0, 0);
 }
 
 
@@ -534,7 +601,7 @@ __device__ __forceinline__ void pre_emit_body(const FrameDev& F, int vb,
   const int area = count > 0 ? rect_area(pj) : 0;
   const Cand me = cand_from(pj);
   GlobalSink sink{off, count, g, F.tile0, pj.depth, keys, vals};
-  uint32_t mask;
+  uint64_t mask;
   sweep_all(me, area, area == 1, lane, cp.block_x, cp.block_y, cp.grid_x, sink, mask);
 }
 
@@ -556,9 +623,11 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_emit_batch(
 
 // ------------------------------------------------- emission from the records
 __global__ __launch_bounds__(256) void ps_pre_chunk_reset(unsigned long long* __restrict__ sums,
-                                                          int n) {
+                                                          int n,
+                                                          unsigned long long* __restrict__ pool_taken) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) sums[i] = 0ull;
+  if (i == 0) *pool_taken = 0ull;
 }
 
 __device__ __forceinline__ long long wave_sum_i64(long long v) {
@@ -582,8 +651,8 @@ __device__ __forceinline__ long long wave_sum_i64(long long v) {
 __global__ __launch_bounds__(kPreThreads) void ps_pre_emit_rec(
     FrameDev F, int nb, const int* __restrict__ counts, const int* __restrict__ block_sums,
     const unsigned long long* __restrict__ chunk_sums, const EmitRec* __restrict__ recs,
-    int* __restrict__ state, int* __restrict__ curr_offset, int capacity,
-    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint64_t* __restrict__ pool, int* __restrict__ state, int* __restrict__ curr_offset,
+    int capacity, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   __shared__ int scratch[4];
   __shared__ long long s_part[kPreThreads / kWave];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -631,13 +700,43 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_emit_rec(
       vals[off + k] = (uint32_t)g;
     }
   }
-  const bool big = count > 0 && w == 0;
+  // Larger rects with pool words: the wave takes them one at a time and
+  // writes each one's hits from the count pass's ballots, 64 rect tiles per
+  // word, in the same row-major order.
+  const uint64_t below = lanes_below(lane);
+  const int tiles = cp.grid_x * cp.grid_y;
+  uint64_t todo = ballot(count > 0 && w == 0 && r.mask != kNoPool);
+  while (todo) {
+    const int i = __builtin_ctzll(todo);
+    todo &= todo - 1;
+    const int base = bcast((int)r.mask, i), cnt = bcast(count, i), o_off = bcast(off, i);
+    const int x0 = bcast((int)(r.x0w & 0xffffffu), i);
+    const int y0 = bcast((int)(r.y0 & 0xffffu), i), rw = bcast((int)(r.y0 >> 16), i);
+    const uint64_t dep = (uint64_t)(uint32_t)bcast((int)r.depth, i);
+    const uint32_t gid = (uint32_t)(g - lane + i);
+    int run = 0;
+    // the words hold exactly cnt hits; the bound only guards a corrupted pool
+    for (int sw = 0; run < cnt && sw * kWave < tiles; ++sw) {
+      const uint64_t m = pool[base + sw];
+      if ((m >> lane) & 1ull) {
+        const int t = sw * kWave + lane;
+        const int ty = t / rw, tx = t - ty * rw;
+        const uint32_t tile = (uint32_t)(F.tile0 + (y0 + ty) * cp.grid_x + (x0 + tx));
+        const int at = o_off + run + popc64(m & below);
+        keys[at] = ((uint64_t)tile << 32) | dep;
+        vals[at] = gid;
+      }
+      run += popc64(m);
+    }
+  }
+  // Rects the pool could not hold: project and sweep again.
+  const bool big = count > 0 && w == 0 && r.mask == kNoPool;
   if (ballot(big) == 0) return;  // uniform across the wave
   float px, py, pz;
   const Proj pj = big ? project_gaussian(F, g, px, py, pz) : no_proj();
   const int area = big ? rect_area(pj) : 0;
   GlobalSink sink{off, count, g, F.tile0, pj.depth, keys, vals};
-  uint32_t mask;
+  uint64_t mask;
   sweep_all(cand_from(pj), area, false, lane, cp.block_x, cp.block_y, cp.grid_x, sink, mask);
 }
 
@@ -648,6 +747,9 @@ struct PreWs {
   int* block_offsets;
   EmitRec* recs;                    // per Gaussian, written by the count pass
   unsigned long long* chunk_sums;   // pairs per chunk of kChunkBlocks workgroups
+  uint64_t* pool;                   // hit-pool words (see EmitRec, kPoolPerBlock)
+  long long pool_cap;               // words in all
+  long long pool_overflow;          // words of the shared overflow area (the last ones)
 };
 
 // Workspace for `nb` preprocess workgroups (a single frame has
@@ -667,6 +769,12 @@ inline size_t pre_layout_blocks(int nb, char* base, PreWs* w) {
   off += align_up(b * kPreThreads * sizeof(EmitRec), 256);
   if (w) w->chunk_sums = reinterpret_cast<unsigned long long*>(base + off);
   off += align_up((size_t)chunks_for((int)b) * sizeof(unsigned long long), 256);
+  if (w) {
+    w->pool = reinterpret_cast<uint64_t*>(base + off);
+    w->pool_cap = (long long)pool_words_for((int)b);
+    w->pool_overflow = (long long)pool_overflow_words((int)b);
+  }
+  off += align_up(pool_words_for((int)b) * sizeof(uint64_t), 256);
   return off;
 }
 
@@ -731,7 +839,8 @@ static int check_frame_args(int P, const void* positions, const void* shs, const
   PS_REQUIRE(cam->block_x > 0 && cam->block_y > 0, "tile size must be positive");
   const long long tiles = (long long)tiles_x(cam) * tiles_y(cam);
   PS_REQUIRE(tiles < (1LL << 31), "too many tiles");
-  PS_REQUIRE(tiles_x(cam) < (1 << 24), "too many tile columns (emit records hold 24 bits)");
+  PS_REQUIRE(tiles_x(cam) < (1 << 16) && tiles_y(cam) < (1 << 16),
+             "too many tile columns or rows (emit records hold 16 bits of each)");
   PS_REQUIRE(P == 0 || (positions && shs && opacities && cov3d), "scene pointer is NULL");
   return PS_OK;
 }
@@ -772,24 +881,28 @@ int preprocess_launch(int P, const float* positions, const void* shs, int sh_pre
   if (record_emission()) {
     const int nc = chunks_for(nb);
     hipLaunchKernelGGL(ps_pre_chunk_reset, dim3((nc + 255) / 256), dim3(256), 0, stream,
-                       w.chunk_sums, nc);
+                       w.chunk_sums, nc,
+                       reinterpret_cast<unsigned long long*>(w.state + kStatePool));
     PS_LAUNCH_CHECK();
     st = launch_count(sh_prec, nb, stream, [&](auto shp) {
       hipLaunchKernelGGL(ps_pre_count<decltype(shp)::value>, dim3(nb), dim3(kPreThreads), 0,
                          stream, F, w.counts, w.block_sums, w.recs, w.chunk_sums, w.state,
-                         curr_offset, reset_offset ? 1 : 0);
+                         curr_offset, reset_offset ? 1 : 0, w.pool, w.pool_cap,
+                         w.pool_overflow);
     });
     if (st != PS_OK) return st;
     hipLaunchKernelGGL(ps_pre_emit_rec, dim3(nb), dim3(kPreThreads), 0, stream, F, nb, w.counts,
                        w.block_sums, (const unsigned long long*)w.chunk_sums,
-                       (const EmitRec*)w.recs, w.state, curr_offset, capacity, keys, values);
+                       (const EmitRec*)w.recs, (const uint64_t*)w.pool, w.state, curr_offset,
+                       capacity, keys, values);
     PS_LAUNCH_CHECK();
     return PS_OK;
   }
   st = launch_count(sh_prec, nb, stream, [&](auto shp) {
     hipLaunchKernelGGL(ps_pre_count<decltype(shp)::value>, dim3(nb), dim3(kPreThreads), 0, stream,
                        F, w.counts, w.block_sums, (EmitRec*)nullptr,
-                       (unsigned long long*)nullptr, w.state, curr_offset, 0);
+                       (unsigned long long*)nullptr, w.state, curr_offset, 0, (uint64_t*)nullptr,
+                       0LL, 0LL);
   });
   if (st != PS_OK) return st;
   hipLaunchKernelGGL(ps_pre_scan, dim3(1), dim3(1024), 0, stream, nb, w.block_sums,

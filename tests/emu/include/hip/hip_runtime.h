@@ -253,8 +253,16 @@ inline hipError_t hipGetDevice(int* d) {
   *d = 0;
   return hipSuccess;
 }
+// The emulated device has few compute units by default, so persistent grids
+// stay small; EMU_CUS and EMU_OCCUPANCY set them to a real device's values
+// (256 and the kernel's waves per CU on MI355X) to run the launch shapes the
+// GPU runs.
+inline int emu_env_int(const char* name, int dflt) {
+  const char* e = getenv(name);
+  return (e && *e) ? atoi(e) : dflt;
+}
 inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
-  *v = 4;  // few compute units: persistent grids stay small
+  *v = emu_env_int("EMU_CUS", 4);
   return hipSuccess;
 }
 // Device symbols are host statics here: their address is the symbol itself.
@@ -266,7 +274,7 @@ inline hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
 }
 template <typename K>
 inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, K, int, size_t) {
-  *n = 2;
+  *n = emu_env_int("EMU_OCCUPANCY", 2);
   return hipSuccess;
 }
 

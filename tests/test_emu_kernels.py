@@ -258,8 +258,10 @@ def _emission_scene():
     return sc, scenes.make_camera(640, 480, 6.0, distance=7.0)
 
 
-def _preprocess(L, sc, spec, capacity, extra=0, base=0, fill=-7):
-    """ps_preprocess into key arrays `extra` entries longer than capacity."""
+def _preprocess(L, sc, spec, capacity, extra=0, base=0, fill=-7, pool_taken=None):
+    """ps_preprocess into key arrays `extra` entries longer than capacity.
+    pool_taken (a list) receives the hit-pool words the count pass asked for
+    (state words 4-5)."""
     P = sc.num_vertex
     pos = np.ascontiguousarray(sc.position.numpy())
     shs = np.ascontiguousarray(sc.shs.numpy())
@@ -279,6 +281,8 @@ def _preprocess(L, sc, spec, capacity, extra=0, base=0, fill=-7):
                          HF.ptr(ku), HF.ptr(vu), capacity, HF.ptr(curr), HF.ptr(ws), nb, None)
     status = int(ws[8:12].view(np.int32)[0])  # state word 2
     count = int(ws[4:8].view(np.int32)[0])  # state word 1
+    if pool_taken is not None:
+        pool_taken.append(int(ws[16:24].view(np.uint64)[0]))
     return st, int(curr[0]), status, count, ku, vu, (xy, rgbd, con)
 
 
@@ -338,3 +342,29 @@ def test_record_emission_overflow_writes_nothing_past_capacity(emu, threads, mon
     monkeypatch.delenv("POLYSPLAT_EMIT")
     st, curr, status, count, ku, vu, _ = _preprocess(emu, sc, spec, M + 10, base=-5)
     assert status & 1 and count == 0 and (ku.view(np.int64) == -7).all()
+
+
+@pytest.mark.parametrize("threads", ["1", "8"])
+def test_record_emission_with_a_full_hit_pool(emu, threads, monkeypatch):
+    """Large rects keep their count-pass ballots in the hit pool; when the
+    pool is full, the rest are projected and swept again.  Half of 2000
+    splats cover all of an 800x600 frame (50 x 38 tiles, 30 pool words
+    each): every workgroup needs more than its own 64 words, and together
+    they ask the shared overflow area (8 * 16 + 16384 words for 8
+    workgroups) for about twice what it holds, so both paths run."""
+    monkeypatch.setenv("EMU_THREADS", threads)
+    sc = scenes.make_scene(2000, 1.0, [(0.5, 0.0, 0.2), (0.3, -2.6, 0.3), (0.2, -4.0, 0.3)],
+                           seed=43)
+    spec = scenes.make_camera(800, 600, 1.0, distance=4.0)
+    cam = O.camera_from_spec(spec)
+    ref = O.preprocess(sc.position.numpy(), sc.shs.numpy(), sc.opacity.numpy(),
+                       sc.cov3d.numpy(), cam)
+    M = ref["M"]
+    taken = []
+    st, curr, status, count, ku, vu, _ = _preprocess(emu, sc, spec, M + 10, extra=5,
+                                                      pool_taken=taken)
+    capi.check_with(emu, st)
+    assert taken[0] > 8 * 16 + 16384, taken  # more words asked for than the overflow holds
+    assert (curr, status, count) == (M, 0, M)
+    np.testing.assert_array_equal(ku[:M], ref["keys"])
+    np.testing.assert_array_equal(vu[:M], ref["vals"])
