@@ -53,8 +53,11 @@ def parse():
                     help="comma-separated subset of fp16_persistent,bf16_persistent,fp32_plain")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
-    ap.add_argument("--serve-requests", type=int, default=1000,
+    ap.add_argument("--serve-requests", type=int, default=10000,
                     help="requests in the 240 req/s serving stream (0 skips it)")
+    ap.add_argument("--serve-tight-requests", type=int, default=2000,
+                    help="requests of a second stream under a tight memory budget (0 skips)")
+    ap.add_argument("--serve-tight-budget-gib", type=float, default=1.5)
     ap.add_argument("--serve-rate", type=float, default=240.0)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving stream")
     return ap.parse_args()
@@ -77,8 +80,11 @@ def copy_probe():
 def serve(args):
     """Serving stream: 4 resident scenes (1M, 3M, 10M, 30M Gaussians, FP16 SH
     and features), Poisson requests at --serve-rate with orbit poses and
-    resolutions drawn from 1280x720 .. 7680x4320, admitted by buffer memory
-    and run in arrival order on one stream (frame after frame)."""
+    resolutions drawn from 1280x720 .. 7680x4320, admitted by buffer memory,
+    run in arrival order on one stream with the next frames enqueued while
+    the host reads back the last ones' pair counts.  A second, shorter
+    stream runs under a memory budget too small for every frame, so that
+    admission rejects and defers."""
     import torch
 
     from polysplat_amd import serving as SV
@@ -87,7 +93,7 @@ def serve(args):
     sc = SV.make_resident_scenes(SV.SCENE_SIZES, device="cuda:0")
     torch.cuda.synchronize()
     log(f"serving: {len(sc)} resident scenes generated in {time.time() - t:.1f}s")
-    ex = SV.SequentialExecutor(sc)
+    ex = SV.make_executor(sc)
     est = SV.Estimates()
     t = time.time()
     SV.calibrate(ex, est, len(sc))
@@ -104,7 +110,11 @@ def serve(args):
     wall = time.perf_counter() - t
     res = SV.summarize(done, sched.rejected, wall, batches=loop.batches)
     res["overflow_retries"] = loop.overflows + ex.grows
-    res["executor"] = "one stream, frames in arrival order, synchronous pair count"
+    res["memory_waits"] = sched.memory_waits
+    res["executor"] = (f"{type(ex).__name__}: one stream, frames in arrival order, enqueued "
+                       "back to back with up to "
+                       f"{ex.slots_n} groups in flight; pair counts read back from pinned "
+                       "slots after each frame (no per-frame host wait)")
     res["rate_req_s"] = args.serve_rate
     res["scenes"] = list(SV.SCENE_SIZES)
     res["resolutions"] = [f"{w}x{h}" for w, h in SV.RESOLUTIONS]
@@ -117,6 +127,22 @@ def serve(args):
             by_res[f"{w}x{h}"] = {"p50_ms": round(float(np.percentile(lat, 50)), 3),
                                   "p99_ms": round(float(np.percentile(lat, 99)), 3)}
     res["by_resolution"] = by_res
+    if args.serve_tight_requests > 0:
+        budget = int(args.serve_tight_budget_gib * (1 << 30))
+        sched2 = SV.Scheduler(SV.SCENE_SIZES, budget, est, max_frames=8, max_batch_ms=6.0)
+        reqs2 = SV.poisson_requests(args.serve_tight_requests, args.serve_rate, len(sc), seed=8)
+        loop2 = SV.ServingLoop(sched2, ex)
+        t = time.perf_counter()
+        done2 = loop2.run(reqs2)
+        torch.cuda.synchronize()
+        tight = SV.summarize(done2, sched2.rejected, time.perf_counter() - t)
+        tight["budget_gib"] = args.serve_tight_budget_gib
+        tight["memory_waits"] = sched2.memory_waits
+        tight["rejected_by_resolution"] = {
+            f"{w}x{h}": sum(1 for r in sched2.rejected if (r.width, r.height) == (w, h))
+            for w, h in SV.RESOLUTIONS}
+        res["tight_budget"] = {k: (round(v, 4) if isinstance(v, float) else v)
+                               for k, v in tight.items()}
     res = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res.items()}
     log(f"serving: {res}")
     return res

@@ -199,9 +199,14 @@ int ps_render_16x16_persistent_noalloc(int num_rendered, const int* d_num_render
  * (pybind.cpp:932-1012), forward_fused_e2e_preranges (pybind.cpp:1016-1080)
  * and forward_fused_e2e (pybind.cpp:1085-1149).  feat_prec F32 writes the
  * f32 feature arrays; F16/BF16 writes 24-byte records into `records`
- * instead.  With sync_count != 0 the call waits for the count and returns
+ * instead.  sync_count 1: the call waits for the count and returns
  * PS_ERR_OVERFLOW when the pairs did not fit; *num_rendered_out receives the
- * count in that case.  Workspace: ps_forward_workspace_size(). */
+ * count in that case.  sync_count 2 ("post"): the call waits for nothing;
+ * the count and overflow status are copied into a pinned slot the library
+ * owns, and *num_rendered_out receives a ticket for ps_forward_wait, so a
+ * caller can enqueue the next frame before reading this one's count (the
+ * reference reads the count mid-frame, pybind.cpp:1122-1125).  Workspace:
+ * ps_forward_workspace_size(). */
 typedef struct ps_frame_buffers {
   int* curr_offset;               /* i32[1] */
   void* sort_space;               /* ps_get_sort_buffer_size(capacity) bytes */
@@ -225,6 +230,18 @@ int ps_forward(int P, const float* positions, const void* shs, int sh_prec,
                const float* opacities, const float* cov3d, const ps_camera* cam,
                int variant, int feat_prec, const ps_frame_buffers* fb, const float bg[3],
                uint8_t* out_color, int sync_count, int* num_rendered_out, ps_stream_t stream);
+
+/* Count of a frame posted with ps_forward(sync_count == 2): waits for the
+ * frame's copy, writes its pair count to *num_rendered_out and returns
+ * PS_ERR_OVERFLOW when its pairs did not fit (nothing was rendered then).
+ * Each ticket is waited for once, on the host thread that posted it; a
+ * thread may have up to 64 posted frames not yet waited for.  No
+ * counterpart in the reference (fetch_num_rendered_async, sort.cu:289-300,
+ * always waits at once). */
+int ps_forward_wait(int ticket, int* num_rendered_out);
+/* 1 when the posted frame's count has arrived (ps_forward_wait will not
+ * block), 0 when not yet, a negative status on error. */
+int ps_forward_query(int ticket);
 
 /* ---------------------------------------------------------------- batch
  * Several frames, each with its own scene and camera, rendered through

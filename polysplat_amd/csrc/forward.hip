@@ -159,6 +159,113 @@ extern "C" int ps_fetch_num_rendered(const int* d_curr_offset, int* out_num_rend
   return ps::read_back(ptrs, 1, out_num_rendered, reinterpret_cast<hipStream_t>(stream));
 }
 
+namespace ps {
+namespace {
+
+// Ring of pinned slots for posted count readbacks (ps_forward with
+// sync_count == 2), one ring per host thread.  Each posted frame's pair
+// count and status word are copied into a slot and an event is recorded
+// after the copy; ps_forward_wait waits for that event.  A slot is reused
+// only after the frame that last used it has been waited for.
+struct PostRing {
+  static constexpr int kSlots = 64;
+  int* host = nullptr;  // 2 ints per slot
+  hipEvent_t ev[kSlots] = {};
+  bool busy[kSlots] = {};
+  unsigned next = 0;
+  int capacity = 0;
+  bool ok = false;
+  PostRing() {
+    if (hipHostMalloc(reinterpret_cast<void**>(&host), 2 * kSlots * sizeof(int),
+                      hipHostMallocDefault) != hipSuccess) {
+      host = nullptr;
+      return;
+    }
+    for (int i = 0; i < kSlots; ++i)
+      if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) return;
+    ok = true;
+  }
+  ~PostRing() {
+    for (int i = 0; i < kSlots; ++i)
+      if (ev[i]) (void)hipEventDestroy(ev[i]);
+    if (host) (void)hipHostFree(host);
+  }
+};
+
+PostRing& post_ring() {
+  static thread_local PostRing r;
+  return r;
+}
+
+// Capacity each posted slot's frame had, for the overflow message.
+thread_local int g_post_capacity[PostRing::kSlots];
+
+// A free slot of the calling thread's ring (checked before a posted frame
+// launches anything, so a refused frame does no work).
+int free_post_slot(int* slot) {
+  PostRing& r = post_ring();
+  PS_REQUIRE(r.ok, "no pinned memory for posted count readbacks");
+  for (int k = 0; k < PostRing::kSlots; ++k) {
+    const int i = (int)((r.next + (unsigned)k) % PostRing::kSlots);
+    if (!r.busy[i]) {
+      *slot = i;
+      return PS_OK;
+    }
+  }
+  set_last_error("more than %d posted frames have not been waited for", PostRing::kSlots);
+  return PS_ERR_INVALID;
+}
+
+int post_count(int slot, const int* d_count, const int* d_status, int capacity,
+               hipStream_t stream) {
+  PostRing& r = post_ring();
+  PS_HIP_CHECK(hipMemcpyAsync(r.host + 2 * slot, d_count, sizeof(int), hipMemcpyDeviceToHost,
+                              stream));
+  PS_HIP_CHECK(hipMemcpyAsync(r.host + 2 * slot + 1, d_status, sizeof(int),
+                              hipMemcpyDeviceToHost, stream));
+  PS_HIP_CHECK(hipEventRecord(r.ev[slot], stream));
+  r.busy[slot] = true;
+  r.next = (unsigned)(slot + 1);
+  g_post_capacity[slot] = capacity;
+  return PS_OK;
+}
+
+}  // namespace
+}  // namespace ps
+
+extern "C" int ps_forward_wait(int ticket, int* num_rendered_out) {
+  using namespace ps;
+  PostRing& r = post_ring();
+  PS_REQUIRE(ticket >= 0 && ticket < PostRing::kSlots && r.busy[ticket],
+             "ticket %d is not a posted frame of this thread", ticket);
+  r.busy[ticket] = false;
+  PS_HIP_CHECK(hipEventSynchronize(r.ev[ticket]));
+  const int n = r.host[2 * ticket], status = r.host[2 * ticket + 1];
+  if (num_rendered_out) *num_rendered_out = n;
+  if (status & 1) {
+    set_last_error("%d pairs do not fit in %d", n, g_post_capacity[ticket]);
+    return PS_ERR_OVERFLOW;
+  }
+  return PS_OK;
+}
+
+extern "C" int ps_forward_query(int ticket) {
+  using namespace ps;
+  PostRing& r = post_ring();
+  if (!(ticket >= 0 && ticket < PostRing::kSlots && r.busy[ticket])) {
+    set_last_error("ticket %d is not a posted frame of this thread", ticket);
+    return -PS_ERR_INVALID;
+  }
+  const hipError_t e = hipEventQuery(r.ev[ticket]);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) {
+    (void)hipGetLastError();  // not an error: clear it
+    return 0;
+  }
+  set_last_error("hipEventQuery failed: %s", hipGetErrorString(e));
+  return -PS_ERR_HIP;
+}
+
 extern "C" size_t ps_forward_workspace_size(int P) {
   return ps::kQueueBytes + ps::align_up(ps_preprocess_workspace_size(P), 256);
 }
@@ -171,6 +278,7 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
   using namespace ps;
   hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
   PS_REQUIRE(fb != nullptr && cam != nullptr && bg != nullptr, "NULL argument");
+  PS_REQUIRE(sync_count >= 0 && sync_count <= 2, "sync_count must be 0, 1 or 2");
   PS_REQUIRE(variant >= PS_VARIANT_DEFAULT && variant <= PS_VARIANT_PERSISTENT_DT,
              "unknown render variant %d", variant);
   PS_REQUIRE(cam->block_x == 16 && cam->block_y == 16,
@@ -187,6 +295,11 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
       variant == PS_VARIANT_PERSISTENT_LITE || variant == PS_VARIANT_PERSISTENT_DT;
   PS_REQUIRE(!need_ranges || fb->tile_ranges != nullptr, "tile_ranges is NULL");
 
+  int slot = -1;
+  if (sync_count == 2) {
+    const int fs = free_post_slot(&slot);
+    if (fs != PS_OK) return fs;
+  }
   // The queue words sit at the start of the workspace, where no other stage
   // writes, so a workspace reused for scenes of other sizes keeps them zero.
   unsigned* queue = static_cast<unsigned*>(fb->workspace);
@@ -223,6 +336,13 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
                                   stream);
   }
   if (st != PS_OK) return st;
+  if (sync_count == 2) {
+    // Posted: the count is read back later through ps_forward_wait.
+    st = post_count(slot, fb->curr_offset, ps_preprocess_status_ptr(ws), fb->capacity, stream);
+    if (st != PS_OK) return st;
+    if (num_rendered_out) *num_rendered_out = slot;
+    return PS_OK;
+  }
   if (sync_count) {
     const int* ptrs[2] = {fb->curr_offset, ps_preprocess_status_ptr(ws)};
     int vals[2] = {0, 0};

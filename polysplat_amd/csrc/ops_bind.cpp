@@ -369,7 +369,7 @@ int forward_common(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width,
                    int block_x, int block_y, Tensor& position, Tensor& rotation, float fx,
                    float fy, float zfar, float znear, int variant, int feat_prec,
                    const FrameArgs& a, Tensor& bg_color, Tensor& out, Tensor* workspace,
-                   bool sync, const char* what) {
+                   int sync, const char* what) {
   check_scene(pts, shs, op, cov);
   need(block_x == 16 && block_y == 16, "only 16x16 tiles are implemented");
   need_dev(*a.curr_offset, "curr_offset", at::kInt);
@@ -431,7 +431,7 @@ int forward_common(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width,
   int n = 0;
   const int st = ps_forward(P, ptr<float>(pts), shs.data_ptr(), sh_prec_of(shs), ptr<float>(op),
                             ptr<float>(cov), &cam, variant, feat_prec, &fb, bg.data(),
-                            ptr<uint8_t>(out), sync ? 1 : 0, &n, cur_stream());
+                            ptr<uint8_t>(out), sync, &n, cur_stream());
   if (st == PS_ERR_OVERFLOW)
     throw std::runtime_error(std::string(what) + ": Too many k-v pairs! (" + ps_last_error() + ")");
   check(st, what);
@@ -448,7 +448,7 @@ int forward_fused(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width, 
   FrameArgs a{&curr_offset, &sort_space, &ku, &vu, &ks, &vs, &xy, &rgbd, &con, nullptr, nullptr};
   return forward_common(pts, shs, op, cov, width, height, block_x, block_y, position, rotation,
                         fx, fy, zfar, znear, PS_VARIANT_DEFAULT, PS_PREC_F32, a, bg, out, nullptr,
-                        true, "forward_fused");
+                        1, "forward_fused");
 }
 
 int forward_fused_e2e(Tensor& pts, Tensor& shs_half, Tensor& op, Tensor& cov, int width,
@@ -459,7 +459,7 @@ int forward_fused_e2e(Tensor& pts, Tensor& shs_half, Tensor& op, Tensor& cov, in
   FrameArgs a{&curr_offset, &sort_space, &ku, &vu, &ks, &vs, &xy, &rgbd, &con, nullptr, nullptr};
   return forward_common(pts, shs_half, op, cov, width, height, block_x, block_y, position,
                         rotation, fx, fy, zfar, znear, PS_VARIANT_PERSISTENT_FUSED, PS_PREC_F32, a,
-                        bg, out, nullptr, true, "forward_fused_e2e");
+                        bg, out, nullptr, 1, "forward_fused_e2e");
 }
 
 int forward_fused_e2e_preranges(Tensor& pts, Tensor& shs_half, Tensor& op, Tensor& cov, int width,
@@ -472,7 +472,7 @@ int forward_fused_e2e_preranges(Tensor& pts, Tensor& shs_half, Tensor& op, Tenso
               &tile_ranges};
   return forward_common(pts, shs_half, op, cov, width, height, block_x, block_y, position,
                         rotation, fx, fy, zfar, znear, PS_VARIANT_PERSISTENT_LITE, PS_PREC_F32, a,
-                        bg, out, nullptr, true, "forward_fused_e2e_preranges");
+                        bg, out, nullptr, 1, "forward_fused_e2e_preranges");
 }
 
 // Generic frame: any variant, any feature precision, optional caller
@@ -486,8 +486,36 @@ int forward_frame(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width, 
   FrameArgs a{&curr_offset, &sort_space, &ku, &vu, &ks, &vs, &xy, &rgbd, &con, &records,
               tile_ranges.numel() > 0 ? &tile_ranges : nullptr};
   return forward_common(pts, shs, op, cov, width, height, 16, 16, position, rotation, fx, fy,
-                        zfar, znear, variant, feat_prec, a, bg, out, &workspace, sync,
+                        zfar, znear, variant, feat_prec, a, bg, out, &workspace, sync ? 1 : 0,
                         "forward_frame");
+}
+
+// forward_frame without waiting for the pair count: returns a ticket for
+// forward_wait, which returns the count (or raises on overflow) once the
+// frame has run, so frames can be enqueued back to back.
+int forward_frame_post(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width, int height,
+                       Tensor& position, Tensor& rotation, float fx, float fy, float zfar,
+                       float znear, int variant, int feat_prec, Tensor& curr_offset,
+                       Tensor& sort_space, Tensor& ku, Tensor& vu, Tensor& ks, Tensor& vs,
+                       Tensor& xy, Tensor& rgbd, Tensor& con, Tensor& records,
+                       Tensor& tile_ranges, Tensor& bg, Tensor& out, Tensor& workspace) {
+  FrameArgs a{&curr_offset, &sort_space, &ku, &vu, &ks, &vs, &xy, &rgbd, &con, &records,
+              tile_ranges.numel() > 0 ? &tile_ranges : nullptr};
+  return forward_common(pts, shs, op, cov, width, height, 16, 16, position, rotation, fx, fy,
+                        zfar, znear, variant, feat_prec, a, bg, out, &workspace, 2,
+                        "forward_frame_post");
+}
+
+// (pair count, whether the pairs overflowed the key buffers: nothing was
+// rendered then and the frame has to be run again with more room).
+py::tuple forward_wait(int ticket) {
+  int n = 0, st = 0;
+  {
+    py::gil_scoped_release nogil;
+    st = ps_forward_wait(ticket, &n);
+  }
+  if (st != PS_ERR_OVERFLOW) check(st, "forward_wait");
+  return py::make_tuple(n, st == PS_ERR_OVERFLOW);
 }
 
 // Batch of frames through shared launches.  Each entry of `frames` is a
@@ -625,6 +653,13 @@ PYBIND11_MODULE(_C, m) {
   ops.def("forward_fused_e2e", &forward_fused_e2e);
   ops.def("forward_fused_e2e_preranges", &forward_fused_e2e_preranges);
   ops.def("forward_frame", &forward_frame);
+  ops.def("forward_frame_post", &forward_frame_post);
+  ops.def("forward_wait", &forward_wait);
+  ops.def("forward_query", [](int ticket) {
+    const int r = ps_forward_query(ticket);
+    if (r < 0) check(-r, "forward_query");
+    return r == 1;
+  });
   ops.def("camera_matrices", &camera_matrices);
   ops.def("forward_batch", &forward_batch);
   ops.def("forward_batch_workspace_size", [](std::vector<int> P) {

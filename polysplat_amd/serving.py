@@ -160,6 +160,7 @@ class Scheduler:
         self.waiting: collections.deque = collections.deque()
         self.in_flight_bytes = 0
         self.rejected: list = []
+        self.memory_waits = 0  # batches ended early because the budget was full
 
     def request_bytes(self, r: Request) -> int:
         return frame_bytes(self.P[r.scene], r.width, r.height, self.est.capacity(r.key),
@@ -186,6 +187,7 @@ class Scheduler:
             if batch and ms + c > self.max_batch_ms:
                 break
             if self.in_flight_bytes + nbytes + b > self.budget:
+                self.memory_waits += 1
                 break
             self.waiting.popleft()
             batch.append(r)
@@ -217,7 +219,7 @@ class InFlight:
 class GpuExecutor:
     """(Not used by the benchmark or the tests: a GPU run of this executor
     faulted the device once and the cause has not been found; the
-    benchmark serves with SequentialExecutor.)
+    benchmark serves with PipelinedExecutor.)
 
     Runs a group of admitted requests on one stream slot: each frame is
     one ``ops.forward_frame`` call (preprocess, sort with tile ranges,
@@ -448,6 +450,92 @@ class SequentialExecutor:
         return self.busy[0] is None
 
 
+class PipelinedExecutor(SequentialExecutor):
+    """Frames on one stream, enqueued back to back: each frame is one
+    ``ops.forward_frame_post`` call, whose pair count is read back later
+    (``ops.forward_wait``) from a pinned slot the library owns, so the GPU
+    runs the next admitted frame while the host is still handling the last
+    one, instead of idling between a frame's end and the next frame's
+    launches.  Up to ``slots`` groups are in flight.  Buffers as in
+    SequentialExecutor: frames on one stream run in order, so consecutive
+    frames can share the key, sort and range buffers and a scene's records
+    and workspace; only each frame's image is its own.  Key buffers grow
+    before a frame whose estimate exceeds them; a frame whose pairs still
+    did not fit is reported with its count for a retry."""
+
+    def __init__(self, scenes_dev, device="cuda:0", feat_prec: str = "fp16",
+                 radius: float = ORBIT_RADIUS, margin: float = 1.05, slots: int = 3):
+        super().__init__(scenes_dev, device=device, feat_prec=feat_prec, radius=radius,
+                         margin=margin)
+        torch = self.torch
+        self.slots_n = slots
+        self.inflight: collections.deque = collections.deque()
+        self.stream = torch.cuda.current_stream(self.dev)
+        self.start = torch.cuda.Event(enable_timing=True)
+
+    def begin(self):
+        torch = self.torch
+        torch.cuda.synchronize(self.dev)
+        self.start.record(self.stream)
+        torch.cuda.synchronize(self.dev)
+        self.t0 = time.perf_counter()
+
+    def free_slot(self):
+        return len(self.inflight) if len(self.inflight) < self.slots_n else None
+
+    def _post(self, r, cap: int):
+        torch = self.torch
+        self._ensure_keys(max(int(cap), 1))
+        sc = self.scenes[r.scene]
+        rec, ws = self._scene_bufs(r.scene)
+        ranges = self._ranges(r)
+        cam = orbit_camera(r.width, r.height, r.azimuth, self.radius)
+        bg = torch.tensor(r.bg, dtype=torch.float32)
+        e = self._empty
+        ku, vu, ks, vs, space, curr = self.keys
+        out = torch.empty((r.height, r.width, 3), dtype=torch.uint8, device=self.dev)
+        t = self.ops.forward_frame_post(sc["pos"], sc["shs"], sc["opac"], sc["cov"], r.width,
+                                        r.height, cam.position, cam.rotation, cam.focal_x,
+                                        cam.focal_y, cam.zFar, cam.zNear, 1, self.feat_prec, curr,
+                                        space, ku, vu, ks, vs, e, e, e, rec, ranges, bg, out, ws)
+        return t, out
+
+    def launch(self, reqs, nbytes, capacities, slot) -> InFlight:
+        torch = self.torch
+        fl = InFlight(self._bid, reqs, nbytes, list(capacities), slot)
+        self._bid += 1
+        tickets = []
+        for i, r in enumerate(reqs):
+            t, out = self._post(r, capacities[i])
+            tickets.append(t)
+            fl.outputs.append(out)
+            fl.capacities[i] = self.cap  # the room the frame had
+        fl.host_counts = tickets
+        fl.done = torch.cuda.Event(enable_timing=True)
+        fl.done.record(self.stream)
+        self.inflight.append(fl)
+        return fl
+
+    def poll(self):
+        """Completed groups, oldest first: (group, done_ms, counts).  A frame
+        that overflowed reports its count, above its capacity."""
+        out = []
+        while self.inflight and self.inflight[0].done.query():
+            fl = self.inflight.popleft()
+            counts = []
+            for t, cap in zip(fl.host_counts, fl.capacities):
+                n, overflowed = self.ops.forward_wait(t)
+                counts.append(n if not overflowed else max(n, cap + 1))
+                if overflowed:
+                    self.grows += 1
+                    self._ensure_keys(int(n * self.margin) + 4096)
+            out.append((fl, self.start.elapsed_time(fl.done), counts))
+        return out
+
+    def idle(self) -> bool:
+        return not self.inflight
+
+
 class ServingLoop:
     """Drives requests through a Scheduler and an executor in real time."""
 
@@ -539,7 +627,7 @@ def _ops():
 
 def make_executor(scenes_dev, device="cuda:0", **kw):
     """The executor the benchmark and the GPU tests serve with."""
-    return SequentialExecutor(scenes_dev, device=device, **kw)
+    return PipelinedExecutor(scenes_dev, device=device, **kw)
 
 
 def make_resident_scenes(sizes=SCENE_SIZES, device="cuda:0", seed0: int = 101, box: float = BOX,
@@ -562,7 +650,7 @@ def calibrate(executor, est: Estimates, n_scenes: int, resolutions=RESOLUTIONS,
               azimuths=(0.0, math.pi / 4), repeats: int = 3):
     """Renders every (scene, resolution) alone at a few orbit poses: warms the
     kernels and seeds the pair and time estimates."""
-    if isinstance(executor, SequentialExecutor):
+    if isinstance(executor, SequentialExecutor):  # and PipelinedExecutor
         return _calibrate_sequential(executor, est, n_scenes, resolutions, azimuths, repeats)
     torch = executor.torch
     executor.begin()

@@ -91,6 +91,8 @@ def bind(L):
                                                 P], I),
         "ps_forward_workspace_size": ([I], S),
         "ps_forward": ([I, P, P, I, P, P, P, I, I, P, P, P, I, P, P], I),
+        "ps_forward_wait": ([I, P], I),
+        "ps_forward_query": ([I], I),
         "ps_forward_batch_workspace_size": ([I, P], S),
         "ps_forward_batch": ([I, P, I, I, P, I, P, P], I),
         "ps_forward_batch_counts_ptr": ([P, I, P], P),

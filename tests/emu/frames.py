@@ -111,7 +111,7 @@ class HostFrame:
         st = self.L.ps_forward(self.P, ptr(self.pos), ptr(self.shs), PREC[self.sh_type],
                                ptr(self.opac), ptr(self.cov), C.byref(self.cam), variant,
                                PREC[self.feat_prec], C.byref(fb), capi.floats3(bg), ptr(self.out),
-                               1 if sync else 0, C.byref(n), None)
+                               int(sync), C.byref(n), None)
         return st, n.value, self.out.copy()
 
 

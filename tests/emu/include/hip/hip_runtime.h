@@ -209,7 +209,7 @@ inline void __builtin_amdgcn_sched_barrier(int) {}
 // ---------------------------------------------------------------- runtime
 typedef struct ihipStream_t* hipStream_t;
 typedef struct ihipEvent_t* hipEvent_t;
-typedef enum { hipSuccess = 0, hipErrorInvalidValue = 1 } hipError_t;
+typedef enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorNotReady = 600 } hipError_t;
 typedef enum {
   hipMemcpyHostToHost = 0,
   hipMemcpyHostToDevice = 1,
@@ -246,6 +246,7 @@ inline hipError_t hipEventDestroy(hipEvent_t e) {
 }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }  // launches run to completion
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline const char* hipGetErrorString(hipError_t) { return "emulated"; }

@@ -145,7 +145,7 @@ class Gpu:
         st = capi.lib().ps_forward(self.P, capi.ptr(self.pos), capi.ptr(self.shs),
                                    PREC[self.sh_type], capi.ptr(self.opac), capi.ptr(self.cov),
                                    C.byref(self.cam), variant, PREC[self.feat_prec], C.byref(fb),
-                                   capi.floats3(bg), capi.ptr(self.out), 1 if sync else 0,
+                                   capi.floats3(bg), capi.ptr(self.out), int(sync),
                                    C.byref(n), capi.stream())
         torch.cuda.synchronize()
         return st, n.value, self.out.cpu().numpy().copy()

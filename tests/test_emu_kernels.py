@@ -368,3 +368,29 @@ def test_record_emission_with_a_full_hit_pool(emu, threads, monkeypatch):
     assert (curr, status, count) == (M, 0, M)
     np.testing.assert_array_equal(ku[:M], ref["keys"])
     np.testing.assert_array_equal(vu[:M], ref["vals"])
+
+
+def test_posted_count_readback(emu):
+    """ps_forward(sync_count=2) posts the pair count for ps_forward_wait: the
+    same count and image as a synchronous frame, overflow reported at the
+    wait, every ticket waited for once (the limit of 64 outstanding tickets
+    is checked on the GPU, where 65 frames take milliseconds)."""
+    sc = _scene()
+    spec = scenes.make_camera(200, 120, 10.0, distance=12.0)
+    ref = oracle_frame(sc, spec, sh_type="fp16", feat_prec="fp16", bg=(0.1, 0.2, 0.3),
+                       check_every=8, fp64=False)
+    M = ref["pre"]["M"]
+    f = HF.HostFrame(emu, sc, spec, capacity=M + 100)
+    st, ticket, img = f.forward((0.1, 0.2, 0.3), sync=2)
+    capi.check_with(emu, st)
+    assert 0 <= ticket < 64 and emu.ps_forward_query(ticket) == 1
+    n = capi.C.c_int(-1)
+    capi.check_with(emu, emu.ps_forward_wait(ticket, capi.C.byref(n)))
+    assert n.value == M
+    assert np.abs(img.astype(int) - ref["image"].astype(int)).max() <= 1
+    assert emu.ps_forward_wait(ticket, capi.C.byref(n)) == capi.PS_ERR_INVALID  # waited already
+    assert emu.ps_forward_query(ticket) < 0
+    small = HF.HostFrame(emu, sc, spec, capacity=M - 1)
+    st, t2, _ = small.forward((0.1, 0.2, 0.3), sync=2)
+    capi.check_with(emu, st)
+    assert emu.ps_forward_wait(t2, capi.C.byref(n)) == capi.PS_ERR_OVERFLOW and n.value == M

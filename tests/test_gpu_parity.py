@@ -308,3 +308,41 @@ def test_golden_fixtures(gpu, name):
     capi.check(st)
     assert n == M
     assert diff_metrics(img, d["image_fp16_check8"])["max_abs_diff"] <= 1
+
+
+def test_posted_frames_and_their_counts(gpu):
+    """ps_forward(sync_count=2): frames enqueued back to back, counts waited
+    for afterwards; same images and counts as synchronous frames, overflow
+    at the wait, at most 64 outstanding tickets per thread."""
+    import ctypes as C
+
+    scene = scenes.make_config_scene("c1", seed=6, P=20_000)
+    spec = scenes.config_camera("c1")
+    ref = oracle_frame(scene, spec, fp64=False)
+    M = ref["pre"]["M"]
+    g = Gpu(scene, spec, sh_type="fp16", feat_prec="fp16", capacity=M + 64)
+    st, n, want = g.forward(BG, capi.V_LITE)
+    capi.check(st)
+    L = capi.lib()
+    tickets = []
+    for _ in range(64):
+        st, t, img = g.forward(BG, capi.V_LITE, sync=2)
+        capi.check(st)
+        tickets.append(t)
+        np.testing.assert_array_equal(img, want)
+    assert sorted(tickets) == list(range(64))
+    # a 65th outstanding ticket is refused before anything runs (the helper
+    # cleared the image first, and it stays cleared)
+    st, _, img = g.forward(BG, capi.V_LITE, sync=2)
+    assert st == capi.PS_ERR_INVALID and not img.any()
+    torch.cuda.synchronize()
+    out = C.c_int(-1)
+    for t in tickets:
+        assert L.ps_forward_query(t) == 1
+        capi.check(L.ps_forward_wait(t, C.byref(out)))
+        assert out.value == M
+    g.cap = M - 1
+    st, t, _ = g.forward(BG, capi.V_LITE, sync=2)
+    capi.check(st)
+    assert L.ps_forward_wait(t, C.byref(out)) == capi.PS_ERR_OVERFLOW and out.value == M
+    assert L.ps_forward_wait(t, C.byref(out)) == capi.PS_ERR_INVALID

@@ -1,8 +1,9 @@
-"""Serving loop on the GPU with the single-stream executor: every request of
-a mixed stream (three resident scenes, three resolutions, orbit poses,
-requests grouped by the scheduler, key buffers grown on overflow) is
-byte-identical to the same request rendered alone through the single-frame
-entry point with buffers of its own."""
+"""Serving loop on the GPU with the single-stream executors (frames waited
+for one by one, and frames enqueued back to back with their pair counts
+read back later): every request of a mixed stream (three resident scenes,
+three resolutions, orbit poses, requests grouped by the scheduler, key
+buffers grown on overflow) is byte-identical to the same request rendered
+alone through the single-frame entry point with buffers of its own."""
 import math
 
 import numpy as np
@@ -42,11 +43,12 @@ def _alone(sc, r, radius):
     return n, out.cpu().numpy()
 
 
-def test_sequential_serving_equals_frames_rendered_alone(gpu):
+@pytest.mark.parametrize("executor", ["SequentialExecutor", "PipelinedExecutor"])
+def test_serving_equals_frames_rendered_alone(gpu, executor):
     radius = 1.5 * BOX
     scenes = SV.make_resident_scenes(SIZES, device="cuda:0", box=BOX,
                                      mix=[(0.9, -3.5, 0.5), (0.1, -2.0, 0.4)])
-    ex = SV.SequentialExecutor(scenes, radius=radius)
+    ex = getattr(SV, executor)(scenes, radius=radius)
     est = SV.Estimates()
     say("sequential serving calibrate")
     SV.calibrate(ex, est, len(SIZES), resolutions=RES, azimuths=(0.0,), repeats=1)
