@@ -45,6 +45,18 @@ enum {
   PS_VARIANT_PERSISTENT_DT = 3      /* "preranges_smem_persistent_lite_dt": skip alpha < 1/255 */
 };
 
+/* OR'ed into ps_forward's variant: the depth-first ("early sorting") key
+ * pipeline of preprocess_es_pass_a/_b, es_depth_sort_and_scan and
+ * es_tile_sort (preprocess.cu:673-1248, sort.cu:151-285, selected in
+ * example.py:246-291): the Gaussians are sorted by depth first, their
+ * (tile, Gaussian) pairs emitted in that order as 32-bit tile ids and sorted
+ * stably by tile id, so no per-tile depth sort is needed.  It gives the same
+ * sorted keys and values, tile ranges and image as the default pipeline;
+ * keys_unsorted then holds 32-bit tile ids (emitted and sorted) instead of
+ * 64-bit keys.  It pays off when tiles hold long lists (many pairs per
+ * tile), where the per-tile depth sort is slow. */
+#define PS_FORWARD_DEPTH_FIRST 0x100
+
 /* Camera exactly as the reference bindings receive it (pybind.cpp:122-150):
  * rotation holds the 9 values of cameras.json "rotation" row by row. */
 typedef struct ps_camera {

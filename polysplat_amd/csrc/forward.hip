@@ -21,6 +21,14 @@ int preprocess_launch(int P, const float* positions, const void* shs, int sh_pre
                       int* curr_offset, void* workspace, size_t workspace_bytes,
                       hipStream_t stream, bool reset_offset);
 int sort_end_bit(int width, int height, int block_x, int block_y);
+int preprocess_es_launch(int P, const float* positions, const void* shs, int sh_prec,
+                         const float* opacities, const float* cov3d, const ps_camera* cam,
+                         float* points_xy, float* rgb_depth, float* conic_opacity,
+                         void* feat_records, int feat_prec, uint32_t* tiles, uint32_t* vals,
+                         int capacity, int* curr_offset, void* workspace, size_t workspace_bytes,
+                         void* sort_space, size_t sort_space_bytes, int sort_max_pairs,
+                         hipStream_t stream);
+const uint32_t* preprocess_depth_words(int P, void* workspace);
 int render_plain_launch(int n_host, const int* d_n, int W, int H, const float* xy,
                         const float* rgbd, const float* con, const uint64_t* keys,
                         const uint32_t* order, const float bg[3], uint8_t* out,
@@ -279,6 +287,8 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
   hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
   PS_REQUIRE(fb != nullptr && cam != nullptr && bg != nullptr, "NULL argument");
   PS_REQUIRE(sync_count >= 0 && sync_count <= 2, "sync_count must be 0, 1 or 2");
+  const bool depth_first = (variant & PS_FORWARD_DEPTH_FIRST) != 0;
+  variant &= ~PS_FORWARD_DEPTH_FIRST;
   PS_REQUIRE(variant >= PS_VARIANT_DEFAULT && variant <= PS_VARIANT_PERSISTENT_DT,
              "unknown render variant %d", variant);
   PS_REQUIRE(cam->block_x == 16 && cam->block_y == 16,
@@ -306,23 +316,48 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
   char* ws = static_cast<char*>(fb->workspace) + kQueueBytes;
   const size_t pre_bytes = align_up(ps_preprocess_workspace_size(P), 256);
   int st;
-  // The frame's keys start at offset 0 (the reference clears curr_offset
-  // first, pybind.cpp:946); the scan kernel takes 0 as the base directly.
-  st = preprocess_launch(P, positions, shs, sh_prec, opacities, cov3d, cam,
-                         f32 ? fb->points_xy : nullptr, f32 ? fb->rgb_depth : nullptr,
-                         f32 ? fb->conic_opacity : nullptr, f32 ? nullptr : fb->records,
-                         feat_prec, fb->keys_unsorted, fb->values_unsorted, fb->capacity,
-                         fb->curr_offset, ws, pre_bytes, stream, true);
-  if (st != PS_OK) return st;
-  const int* d_n = ps_preprocess_count_ptr(ws);
-  // The sort leaves the tile ranges in tile_ranges when the renderer needs
-  // them (its segment pass computes them anyway).
   const int tiles = ((cam->width + 15) / 16) * ((cam->height + 15) / 16);
-  st = sort_launch(0, d_n, fb->capacity, sort_end_bit(cam->width, cam->height, 16, 16),
-                   fb->sort_space, fb->sort_space_bytes, fb->keys_unsorted, fb->values_unsorted,
-                   fb->keys_sorted, fb->values_sorted, stream,
-                   need_ranges ? fb->tile_ranges : nullptr, tiles);
-  if (st != PS_OK) return st;
+  const int* d_n = ps_preprocess_count_ptr(ws);
+  if (depth_first) {
+    // Depth-first pipeline: Gaussians sorted by depth, their (tile id,
+    // Gaussian) pairs emitted in that order as 32-bit tile ids (first half
+    // of keys_unsorted's memory), stably sorted by tile id (the sorted ids
+    // in its second half); tile ranges from that sort and the 64-bit sorted
+    // keys rebuilt from the tile ids and the depth words.
+    PS_REQUIRE(fb->tile_ranges != nullptr || !need_ranges, "tile_ranges is NULL");
+    uint32_t* tk = reinterpret_cast<uint32_t*>(fb->keys_unsorted);
+    uint32_t* tk_sorted = tk + fb->capacity;
+    st = preprocess_es_launch(P, positions, shs, sh_prec, opacities, cov3d, cam,
+                              f32 ? fb->points_xy : nullptr, f32 ? fb->rgb_depth : nullptr,
+                              f32 ? fb->conic_opacity : nullptr, f32 ? nullptr : fb->records,
+                              feat_prec, tk, fb->values_unsorted, fb->capacity, fb->curr_offset,
+                              ws, pre_bytes, fb->sort_space, fb->sort_space_bytes, fb->capacity,
+                              stream);
+    if (st != PS_OK) return st;
+    st = tile_sort_launch(d_n, fb->capacity, tiles, fb->sort_space, fb->sort_space_bytes, tk,
+                          fb->values_unsorted, tk_sorted, fb->values_sorted,
+                          need_ranges ? fb->tile_ranges : nullptr, stream);
+    if (st != PS_OK) return st;
+    st = rebuild_keys_launch(d_n, fb->capacity, tk_sorted, fb->values_sorted,
+                             preprocess_depth_words(P, ws), fb->keys_sorted, stream);
+    if (st != PS_OK) return st;
+  } else {
+    // The frame's keys start at offset 0 (the reference clears curr_offset
+    // first, pybind.cpp:946); the scan kernel takes 0 as the base directly.
+    st = preprocess_launch(P, positions, shs, sh_prec, opacities, cov3d, cam,
+                           f32 ? fb->points_xy : nullptr, f32 ? fb->rgb_depth : nullptr,
+                           f32 ? fb->conic_opacity : nullptr, f32 ? nullptr : fb->records,
+                           feat_prec, fb->keys_unsorted, fb->values_unsorted, fb->capacity,
+                           fb->curr_offset, ws, pre_bytes, stream, true);
+    if (st != PS_OK) return st;
+    // The sort leaves the tile ranges in tile_ranges when the renderer needs
+    // them (its segment pass computes them anyway).
+    st = sort_launch(0, d_n, fb->capacity, sort_end_bit(cam->width, cam->height, 16, 16),
+                     fb->sort_space, fb->sort_space_bytes, fb->keys_unsorted, fb->values_unsorted,
+                     fb->keys_sorted, fb->values_sorted, stream,
+                     need_ranges ? fb->tile_ranges : nullptr, tiles);
+    if (st != PS_OK) return st;
+  }
   if (variant == PS_VARIANT_DEFAULT) {
     st = render_plain_launch(0, d_n, cam->width, cam->height, fb->points_xy, fb->rgb_depth,
                              fb->conic_opacity, fb->keys_sorted, fb->values_sorted, bg, out_color,

@@ -421,7 +421,9 @@ __device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* _
                                                int* __restrict__ state,
                                                const int* __restrict__ curr_offset,
                                                int reset_offset, uint64_t* __restrict__ pool,
-                                               long long pool_cap, long long pool_overflow) {
+                                               long long pool_cap, long long pool_overflow,
+                                               uint32_t* __restrict__ dkey,
+                                               uint32_t* __restrict__ plan) {
   __shared__ int scratch[4];
   const int tid = threadIdx.x, lane = tid & 63;
   const int g = (vb - F.block0) * kPreThreads + tid;
@@ -476,6 +478,30 @@ __device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* _
     recs[(size_t)vb * kPreThreads + tid] = r;
   }
   if (count > 0) write_features<SHP>(F, g, pj, px, py, pz);
+  if (dkey) {  // uniform: the depth-first pipeline's depth words and plan
+    if (g < F.P) dkey[g] = count > 0 ? f2u(pj.depth) : 0u;
+    __shared__ uint32_t s_or[kPreThreads / kWave], s_and[kPreThreads / kWave];
+    uint32_t o = count > 0 ? f2u(pj.depth) : 0u, a = count > 0 ? f2u(pj.depth) : ~0u;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      o |= (uint32_t)__shfl_xor((int)o, d);
+      a &= (uint32_t)__shfl_xor((int)a, d);
+    }
+    if (lane == 0) {
+      s_or[tid >> 6] = o;
+      s_and[tid >> 6] = a;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      o = s_or[0] | s_or[1] | s_or[2] | s_or[3];
+      a = s_and[0] & s_and[1] & s_and[2] & s_and[3];
+      if (o != 0u) {  // the block has an emitting Gaussian
+        const int slot = vb % kPlanSlots;
+        atomicOr(&plan[slot * kPlanStride], o);
+        atomicAnd(&plan[(kPlanSlots + slot) * kPlanStride], a);
+      }
+    }
+  }
   int total;
   block256_excl_scan(count, scratch, total);
   if (tid == 0) {
@@ -490,9 +516,10 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_count(
     FrameDev F, int* __restrict__ counts, int* __restrict__ block_sums,
     EmitRec* __restrict__ recs, unsigned long long* __restrict__ chunk_sums,
     int* __restrict__ state, const int* __restrict__ curr_offset, int reset_offset,
-    uint64_t* __restrict__ pool, long long pool_cap, long long pool_overflow) {
+    uint64_t* __restrict__ pool, long long pool_cap, long long pool_overflow,
+    uint32_t* __restrict__ dkey, uint32_t* __restrict__ plan) {
   pre_count_body<SHP>(F, blockIdx.x, counts, block_sums, recs, chunk_sums, state, curr_offset,
-                      reset_offset, pool, pool_cap, pool_overflow);
+                      reset_offset, pool, pool_cap, pool_overflow, dkey, plan);
 }
 
 __device__ __forceinline__ int frame_of_block(const FrameDev* frames, int nf, int vb) {
@@ -505,7 +532,7 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_count_batch(const FrameDev
                                                                   int* __restrict__ block_sums) {
   const int f = frame_of_block(frames, nf, blockIdx.x);
   pre_count_body<SHP>(frames[f], blockIdx.x, counts, block_sums, nullptr, nullptr, nullptr,
-                      nullptr, 0, nullptr, 0, 0);
+                      nullptr, 0, nullptr, 0, 0, nullptr, nullptr);
 }
 
 
@@ -624,10 +651,15 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_emit_batch(
 // ------------------------------------------------- emission from the records
 __global__ __launch_bounds__(256) void ps_pre_chunk_reset(unsigned long long* __restrict__ sums,
                                                           int n,
-                                                          unsigned long long* __restrict__ pool_taken) {
+                                                          unsigned long long* __restrict__ pool_taken,
+                                                          uint32_t* __restrict__ plan) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) sums[i] = 0ull;
   if (i == 0) *pool_taken = 0ull;
+  if (plan && i < kPlanSlots) {
+    plan[i * kPlanStride] = 0u;                  // OR of emitting depth words
+    plan[(kPlanSlots + i) * kPlanStride] = ~0u;  // AND
+  }
 }
 
 __device__ __forceinline__ long long wave_sum_i64(long long v) {
@@ -640,34 +672,68 @@ __device__ __forceinline__ long long wave_sum_i64(long long v) {
   return v;
 }
 
-// Emit pass over the count pass's records.  Workgroup vb starts at the
-// frame's offset plus the chunk sums before its chunk plus the block sums of
-// its chunk before it; keys go to their canonical slots (excl + k).  A
-// Gaussian with a hit mask writes its keys from the mask; one whose rect was
-// too large for a mask is projected and swept again, as the three-launch
-// emit does.  Pairs are written only by workgroups whose whole range fits
-// below `capacity`.  The last workgroup publishes the total, the count for
-// the later stages and the overflow bit.
-__global__ __launch_bounds__(kPreThreads) void ps_pre_emit_rec(
-    FrameDev F, int nb, const int* __restrict__ counts, const int* __restrict__ block_sums,
-    const unsigned long long* __restrict__ chunk_sums, const EmitRec* __restrict__ recs,
+// Where the emit passes put a (tile, Gaussian) pair: the 64-bit key
+// (tile << 32 | depth word) and the id, or, in the depth-first pipeline,
+// the 32-bit tile id and the id.
+struct Out64 {
+  uint64_t* keys;
+  uint32_t* vals;
+  __device__ __forceinline__ void put(int at, uint32_t tile, uint32_t depth, uint32_t gid) const {
+    keys[at] = ((uint64_t)tile << 32) | (uint64_t)depth;
+    vals[at] = gid;
+  }
+};
+struct Out32 {
+  uint32_t* tiles;
+  uint32_t* vals;
+  __device__ __forceinline__ void put(int at, uint32_t tile, uint32_t, uint32_t gid) const {
+    tiles[at] = tile;
+    vals[at] = gid;
+  }
+};
+
+// Sweep sink writing through an Out (the re-projecting emit).
+template <class Out>
+struct OutSink {
+  int off, cnt, gid, tile0;
+  float depth;
+  Out out;
+  __device__ __forceinline__ void step(bool ok, uint64_t, int tile, int owner, int k) {
+    const int o_off = permute(off, owner), o_cnt = permute(cnt, owner);
+    const int o_gid = permute(gid, owner);
+    const float o_depth = permute(depth, owner);
+    if (ok && k < o_cnt) out.put(o_off + k, (uint32_t)(tile0 + tile), f2u(o_depth), (uint32_t)o_gid);
+  }
+  __device__ __forceinline__ void ballot(int, int, uint64_t) {}
+};
+
+// One workgroup of an emit pass over the count pass's records.  Thread tid
+// handles Gaussian g (g >= F.P: none) with `count` pairs and record r; the
+// workgroup's pairs start at the frame's offset plus the chunk sums before
+// its chunk plus the block sums of its chunk before it, and each Gaussian's
+// at the block's exclusive prefix of the counts, in thread order.  A
+// Gaussian with a hit mask writes its pairs from the mask, one with pool
+// words from those (the wave takes them one at a time, 64 rect tiles per
+// word), and one whose rect found no room in the pool is projected and swept
+// again.  Pairs are written only by workgroups whose whole range fits below
+// `capacity`.  Workgroup nb - 1 publishes the total, the count for the later
+// stages and the overflow bit.
+template <class Out>
+__device__ __forceinline__ void emit_records_block(
+    const FrameDev& F, int vb, int nb, int g, int count, const EmitRec& r,
+    const int* __restrict__ block_sums, const unsigned long long* __restrict__ chunk_sums,
     const uint64_t* __restrict__ pool, int* __restrict__ state, int* __restrict__ curr_offset,
-    int capacity, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    int capacity, const Out& out) {
   __shared__ int scratch[4];
   __shared__ long long s_part[kPreThreads / kWave];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int vb = blockIdx.x, chunk = vb / kChunkBlocks;
+  const int chunk = vb / kChunkBlocks;
   long long part = 0;
   for (int i = tid; i < chunk; i += kPreThreads) part += (long long)chunk_sums[i];
   const int b = chunk * kChunkBlocks + tid;
   if (b < vb) part += block_sums[b];
   part = wave_sum_i64(part);
   if (lane == 0) s_part[wave] = part;
-  const int g = vb * kPreThreads + tid;
-  const int count = g < F.P ? counts[g] : 0;
-  // The record is loaded before the scan (a lane that emits nothing ignores
-  // whatever its slot holds), so its latency overlaps the scan's.
-  const EmitRec r = g < F.P ? recs[g] : EmitRec{0u, 0u, 0u, 0u};
   int total;
   const int excl = block256_excl_scan(count, scratch, total);  // its barriers publish s_part
   const long long base = state[kStateBase];
@@ -695,49 +761,120 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_emit_rec(
       const int t = __builtin_ctz(m);
       m &= m - 1u;
       const int ty = (int)(((uint32_t)t * inv_w) >> 16), tx = t - ty * w;
-      const uint32_t tile = (uint32_t)(F.tile0 + (y0 + ty) * cp.grid_x + (x0 + tx));
-      keys[off + k] = ((uint64_t)tile << 32) | (uint64_t)r.depth;
-      vals[off + k] = (uint32_t)g;
+      out.put(off + k, (uint32_t)(F.tile0 + (y0 + ty) * cp.grid_x + (x0 + tx)), r.depth,
+              (uint32_t)g);
     }
   }
-  // Larger rects with pool words: the wave takes them one at a time and
-  // writes each one's hits from the count pass's ballots, 64 rect tiles per
-  // word, in the same row-major order.
   const uint64_t below = lanes_below(lane);
   const int tiles = cp.grid_x * cp.grid_y;
   uint64_t todo = ballot(count > 0 && w == 0 && r.mask != kNoPool);
   while (todo) {
     const int i = __builtin_ctzll(todo);
     todo &= todo - 1;
-    const int base = bcast((int)r.mask, i), cnt = bcast(count, i), o_off = bcast(off, i);
+    const int pbase = bcast((int)r.mask, i), cnt = bcast(count, i), o_off = bcast(off, i);
     const int x0 = bcast((int)(r.x0w & 0xffffffu), i);
     const int y0 = bcast((int)(r.y0 & 0xffffu), i), rw = bcast((int)(r.y0 >> 16), i);
-    const uint64_t dep = (uint64_t)(uint32_t)bcast((int)r.depth, i);
-    const uint32_t gid = (uint32_t)(g - lane + i);
+    const uint32_t dep = (uint32_t)bcast((int)r.depth, i);
+    const uint32_t gid = (uint32_t)bcast(g, i);
     int run = 0;
     // the words hold exactly cnt hits; the bound only guards a corrupted pool
     for (int sw = 0; run < cnt && sw * kWave < tiles; ++sw) {
-      const uint64_t m = pool[base + sw];
+      const uint64_t m = pool[pbase + sw];
       if ((m >> lane) & 1ull) {
         const int t = sw * kWave + lane;
         const int ty = t / rw, tx = t - ty * rw;
-        const uint32_t tile = (uint32_t)(F.tile0 + (y0 + ty) * cp.grid_x + (x0 + tx));
-        const int at = o_off + run + popc64(m & below);
-        keys[at] = ((uint64_t)tile << 32) | dep;
-        vals[at] = gid;
+        out.put(o_off + run + popc64(m & below),
+                (uint32_t)(F.tile0 + (y0 + ty) * cp.grid_x + (x0 + tx)), dep, gid);
       }
       run += popc64(m);
     }
   }
-  // Rects the pool could not hold: project and sweep again.
   const bool big = count > 0 && w == 0 && r.mask == kNoPool;
   if (ballot(big) == 0) return;  // uniform across the wave
   float px, py, pz;
   const Proj pj = big ? project_gaussian(F, g, px, py, pz) : no_proj();
   const int area = big ? rect_area(pj) : 0;
-  GlobalSink sink{off, count, g, F.tile0, pj.depth, keys, vals};
+  OutSink<Out> sink{off, count, g, F.tile0, pj.depth, out};
   uint64_t mask;
   sweep_all(cand_from(pj), area, false, lane, cp.block_x, cp.block_y, cp.grid_x, sink, mask);
+}
+
+// Emit pass in Gaussian order: keys go to their canonical slots.
+__global__ __launch_bounds__(kPreThreads) void ps_pre_emit_rec(
+    FrameDev F, int nb, const int* __restrict__ counts, const int* __restrict__ block_sums,
+    const unsigned long long* __restrict__ chunk_sums, const EmitRec* __restrict__ recs,
+    const uint64_t* __restrict__ pool, int* __restrict__ state, int* __restrict__ curr_offset,
+    int capacity, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int g = blockIdx.x * kPreThreads + threadIdx.x;
+  const int count = g < F.P ? counts[g] : 0;
+  // The record is loaded before the scan (a lane that emits nothing ignores
+  // whatever its slot holds), so its latency overlaps the scan's.
+  const EmitRec r = g < F.P ? recs[g] : EmitRec{0u, 0u, 0u, 0u};
+  emit_records_block(F, blockIdx.x, nb, g, count, r, block_sums, chunk_sums, pool, state,
+                     curr_offset, capacity, Out64{keys, vals});
+}
+
+// ---------------------------------------------------- depth-first emission
+// Slot s of the depth order holds Gaussian depth_order(s): the depth sort's
+// values, or s itself when no digit of the sort ran.
+struct DepthOrder {
+  const uint32_t* ids;  // NULL: the identity
+  __device__ __forceinline__ int at(int s) const { return ids ? (int)ids[s] : s; }
+};
+
+__device__ __forceinline__ DepthOrder depth_order(const uint32_t* __restrict__ plan,
+                                                  uint32_t* const vb0, uint32_t* const vb1,
+                                                  uint32_t* s_k) {
+  const int tid = threadIdx.x;
+  if (tid < kWave) {
+    uint32_t o = plan[tid * kPlanStride], a = plan[(kPlanSlots + tid) * kPlanStride];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      o |= (uint32_t)__shfl_xor((int)o, d);
+      a &= (uint32_t)__shfl_xor((int)a, d);
+    }
+    if (tid == 0) *s_k = (uint32_t)depth_digits_running(o == 0u ? 0u : (o ^ a));
+  }
+  __syncthreads();
+  const int k = (int)*s_k;
+  return DepthOrder{k == 0 ? nullptr : ((k - 1) & 1 ? vb1 : vb0)};
+}
+
+// Counts in depth order, their block sums and the chunk sums of those.
+__global__ __launch_bounds__(kPreThreads) void ps_es_count(
+    int P, const uint32_t* __restrict__ plan, uint32_t* vb0, uint32_t* vb1,
+    const int* __restrict__ counts, int* __restrict__ counts_sorted, int* __restrict__ block_sums,
+    unsigned long long* __restrict__ chunk_sums) {
+  __shared__ int scratch[4];
+  __shared__ uint32_t s_k;
+  const DepthOrder ord = depth_order(plan, vb0, vb1, &s_k);
+  const int s = blockIdx.x * kPreThreads + threadIdx.x;
+  const int c = s < P ? counts[ord.at(s)] : 0;
+  if (s < P) counts_sorted[s] = c;
+  int total;
+  block256_excl_scan(c, scratch, total);
+  if (threadIdx.x == 0) {
+    block_sums[blockIdx.x] = total;
+    atomicAdd(&chunk_sums[blockIdx.x / kChunkBlocks], (unsigned long long)total);
+  }
+}
+
+// Emit pass in depth order: (tile id, Gaussian) pairs, each Gaussian's in
+// row-major tile order, Gaussians in (depth, id) order.
+__global__ __launch_bounds__(kPreThreads) void ps_es_emit(
+    FrameDev F, int nb, const uint32_t* __restrict__ plan, uint32_t* vb0, uint32_t* vb1,
+    const int* __restrict__ counts_sorted, const int* __restrict__ block_sums,
+    const unsigned long long* __restrict__ chunk_sums, const EmitRec* __restrict__ recs,
+    const uint64_t* __restrict__ pool, int* __restrict__ state, int* __restrict__ curr_offset,
+    int capacity, uint32_t* __restrict__ tiles, uint32_t* __restrict__ vals) {
+  __shared__ uint32_t s_k;
+  const DepthOrder ord = depth_order(plan, vb0, vb1, &s_k);
+  const int s = blockIdx.x * kPreThreads + threadIdx.x;
+  const int count = s < F.P ? counts_sorted[s] : 0;
+  const int g = s < F.P ? ord.at(s) : F.P;
+  const EmitRec r = count > 0 ? recs[g] : EmitRec{0u, 0u, 0u, 0u};
+  emit_records_block(F, blockIdx.x, nb, g, count, r, block_sums, chunk_sums, pool, state,
+                     curr_offset, capacity, Out32{tiles, vals});
 }
 
 struct PreWs {
@@ -750,6 +887,13 @@ struct PreWs {
   uint64_t* pool;                   // hit-pool words (see EmitRec, kPoolPerBlock)
   long long pool_cap;               // words in all
   long long pool_overflow;          // words of the shared overflow area (the last ones)
+  // depth-first pipeline: depth words, sort plan, depth-sort ping-pong
+  // buffers, counts in depth order
+  uint32_t* dkey;
+  uint32_t* plan;
+  uint32_t* kb[2];
+  uint32_t* vb[2];
+  int* counts_sorted;
 };
 
 // Workspace for `nb` preprocess workgroups (a single frame has
@@ -775,6 +919,19 @@ inline size_t pre_layout_blocks(int nb, char* base, PreWs* w) {
     w->pool_overflow = (long long)pool_overflow_words((int)b);
   }
   off += align_up(pool_words_for((int)b) * sizeof(uint64_t), 256);
+  const size_t g = b * kPreThreads * sizeof(uint32_t);  // one word per Gaussian slot
+  if (w) w->dkey = reinterpret_cast<uint32_t*>(base + off);
+  off += align_up(g, 256);
+  if (w) w->plan = reinterpret_cast<uint32_t*>(base + off);
+  off += align_up((size_t)2 * kPlanSlots * kPlanStride * sizeof(uint32_t), 256);
+  for (int i = 0; i < 2; ++i) {
+    if (w) w->kb[i] = reinterpret_cast<uint32_t*>(base + off);
+    off += align_up(g, 256);
+    if (w) w->vb[i] = reinterpret_cast<uint32_t*>(base + off);
+    off += align_up(g, 256);
+  }
+  if (w) w->counts_sorted = reinterpret_cast<int*>(base + off);
+  off += align_up(g, 256);
   return off;
 }
 
@@ -882,13 +1039,14 @@ int preprocess_launch(int P, const float* positions, const void* shs, int sh_pre
     const int nc = chunks_for(nb);
     hipLaunchKernelGGL(ps_pre_chunk_reset, dim3((nc + 255) / 256), dim3(256), 0, stream,
                        w.chunk_sums, nc,
-                       reinterpret_cast<unsigned long long*>(w.state + kStatePool));
+                       reinterpret_cast<unsigned long long*>(w.state + kStatePool),
+                       (uint32_t*)nullptr);
     PS_LAUNCH_CHECK();
     st = launch_count(sh_prec, nb, stream, [&](auto shp) {
       hipLaunchKernelGGL(ps_pre_count<decltype(shp)::value>, dim3(nb), dim3(kPreThreads), 0,
                          stream, F, w.counts, w.block_sums, w.recs, w.chunk_sums, w.state,
                          curr_offset, reset_offset ? 1 : 0, w.pool, w.pool_cap,
-                         w.pool_overflow);
+                         w.pool_overflow, (uint32_t*)nullptr, (uint32_t*)nullptr);
     });
     if (st != PS_OK) return st;
     hipLaunchKernelGGL(ps_pre_emit_rec, dim3(nb), dim3(kPreThreads), 0, stream, F, nb, w.counts,
@@ -902,7 +1060,7 @@ int preprocess_launch(int P, const float* positions, const void* shs, int sh_pre
     hipLaunchKernelGGL(ps_pre_count<decltype(shp)::value>, dim3(nb), dim3(kPreThreads), 0, stream,
                        F, w.counts, w.block_sums, (EmitRec*)nullptr,
                        (unsigned long long*)nullptr, w.state, curr_offset, 0, (uint64_t*)nullptr,
-                       0LL, 0LL);
+                       0LL, 0LL, (uint32_t*)nullptr, (uint32_t*)nullptr);
   });
   if (st != PS_OK) return st;
   hipLaunchKernelGGL(ps_pre_scan, dim3(1), dim3(1024), 0, stream, nb, w.block_sums,
@@ -913,6 +1071,80 @@ int preprocess_launch(int P, const float* positions, const void* shs, int sh_pre
                      w.block_offsets, w.state, keys, values);
   PS_LAUNCH_CHECK();
   return PS_OK;
+}
+
+// Depth-first preprocess: count pass (features, records, hit pool, depth
+// words, sort plan), the stable depth sort of the Gaussians, then the
+// (tile id, Gaussian) pairs emitted in (depth, id) order into tiles/vals
+// from offset 0.  The sort space supplies the depth sort's digit counts.
+// The same state words as preprocess_launch are published (count, overflow).
+int preprocess_es_launch(int P, const float* positions, const void* shs, int sh_prec,
+                         const float* opacities, const float* cov3d, const ps_camera* cam,
+                         float* points_xy, float* rgb_depth, float* conic_opacity,
+                         void* feat_records, int feat_prec, uint32_t* tiles, uint32_t* vals,
+                         int capacity, int* curr_offset, void* workspace, size_t workspace_bytes,
+                         void* sort_space, size_t sort_space_bytes, int sort_max_pairs,
+                         hipStream_t stream) {
+  int st = check_frame_args(P, positions, shs, opacities, cov3d, cam);
+  if (st != PS_OK) return st;
+  PS_REQUIRE(sh_prec == PS_PREC_F32 || sh_prec == PS_PREC_F16 || sh_prec == PS_PREC_BF16,
+             "unknown SH precision %d", sh_prec);
+  PS_REQUIRE(capacity >= 0 && curr_offset != nullptr, "bad key buffers");
+  PS_REQUIRE(capacity == 0 || (tiles && vals), "key/value buffers are NULL");
+  PS_REQUIRE(workspace != nullptr && workspace_bytes >= ps_preprocess_workspace_size(P),
+             "preprocess workspace too small");
+  PreWs w;
+  pre_layout(P, static_cast<char*>(workspace), &w);
+  const FrameDev F = make_frame_dev(P, positions, shs, opacities, cov3d, cam, points_xy, rgb_depth,
+                                    conic_opacity, feat_records, feat_prec, nullptr, nullptr);
+  const int nb = blocks_for(P);
+  if (nb == 0) {
+    hipLaunchKernelGGL(ps_pre_scan, dim3(1), dim3(1024), 0, stream, 0, w.block_sums,
+                       w.block_offsets, curr_offset, capacity, 1, w.state,
+                       (const FrameDev*)nullptr, 0, (int*)nullptr);
+    PS_LAUNCH_CHECK();
+    return PS_OK;
+  }
+  const int nc = chunks_for(nb);
+  hipLaunchKernelGGL(ps_pre_chunk_reset, dim3((nc + 255) / 256), dim3(256), 0, stream,
+                     w.chunk_sums, nc, reinterpret_cast<unsigned long long*>(w.state + kStatePool),
+                     w.plan);
+  PS_LAUNCH_CHECK();
+  st = launch_count(sh_prec, nb, stream, [&](auto shp) {
+    hipLaunchKernelGGL(ps_pre_count<decltype(shp)::value>, dim3(nb), dim3(kPreThreads), 0, stream,
+                       F, w.counts, w.block_sums, w.recs, (unsigned long long*)nullptr, w.state,
+                       curr_offset, 1, w.pool, w.pool_cap, w.pool_overflow, w.dkey, w.plan);
+  });
+  if (st != PS_OK) return st;
+  DepthSortArgs A;
+  A.plan = w.plan;
+  A.words = w.dkey;
+  A.kb[0] = w.kb[0];
+  A.kb[1] = w.kb[1];
+  A.vb[0] = w.vb[0];
+  A.vb[1] = w.vb[1];
+  A.n = P;
+  st = depth_sort_launch(A, sort_max_pairs, sort_space, sort_space_bytes, stream);
+  if (st != PS_OK) return st;
+  hipLaunchKernelGGL(ps_es_count, dim3(nb), dim3(kPreThreads), 0, stream, P,
+                     (const uint32_t*)w.plan, w.vb[0], w.vb[1], (const int*)w.counts,
+                     w.counts_sorted, w.block_sums, w.chunk_sums);
+  PS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(ps_es_emit, dim3(nb), dim3(kPreThreads), 0, stream, F, nb,
+                     (const uint32_t*)w.plan, w.vb[0], w.vb[1], (const int*)w.counts_sorted,
+                     (const int*)w.block_sums, (const unsigned long long*)w.chunk_sums,
+                     (const EmitRec*)w.recs, (const uint64_t*)w.pool, w.state, curr_offset,
+                     capacity, tiles, vals);
+  PS_LAUNCH_CHECK();
+  return PS_OK;
+}
+
+// Depth word per Gaussian the depth-first preprocess leaves in its workspace
+// (0 for a Gaussian that emits nothing), for rebuilding 64-bit keys.
+const uint32_t* preprocess_depth_words(int P, void* workspace) {
+  PreWs w;
+  pre_layout(P, static_cast<char*>(workspace), &w);
+  return w.dkey;
 }
 
 // Pinned staging ring for batch descriptors, one per host thread.  A copy

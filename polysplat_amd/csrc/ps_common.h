@@ -329,6 +329,33 @@ static_assert(sizeof(FeatRec16) == 24, "record layout");
 
 namespace ps {
 
+// Depth-sort plan of the depth-first pipeline: the count pass ORs and ANDs
+// the depth words of emitting Gaussians into kPlanSlots slot pairs (slot
+// words kPlanStride apart, so that the atomics of different slots fall on
+// different cache lines); the sort kernels combine the slots.
+constexpr int kPlanSlots = 64;
+constexpr int kPlanStride = 32;
+
+// Buffers of the depth sort (sort.hip) and of the emission that reads its
+// order (preprocess.hip).
+struct DepthSortArgs {
+  const uint32_t* plan;   // kPlanSlots OR words, then kPlanSlots AND words (kPlanStride apart)
+  const uint32_t* words;  // depth word per Gaussian, 0 when it emits nothing
+  uint32_t* kb[2];        // keys between digits
+  uint32_t* vb[2];        // values (Gaussian ids) between digits
+  int n;                  // Gaussians
+};
+
+// Plan of the depth sort: varying = OR ^ AND of the emitting depth words (0
+// when nothing emits); digit q (8 bits) runs when its byte of varying is
+// non-zero.  The sorted ids end in vb[(running digits - 1) % 2], or are the
+// identity when no digit runs.
+__host__ __device__ inline int depth_digits_running(uint32_t varying) {
+  int k = 0;
+  for (int q = 0; q < 4; ++q) k += ((varying >> (8 * q)) & 0xffu) != 0u ? 1 : 0;
+  return k;
+}
+
 // Everything the preprocess and render kernels need about one frame.  A
 // single-frame launch passes it by value; a batched launch reads an array of
 // them from device memory and maps each workgroup / tile to its frame.

@@ -126,6 +126,21 @@ int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles
                             const uint64_t* keys, int* ranges, hipStream_t stream,
                             uint32_t mask = 0xffffffffu, int* zero_counter = nullptr,
                             int zero_words = 1);
+// Depth-first pipeline: the depth sort of the Gaussians and the tile sort of
+// their (tile id, Gaussian) pairs (sort.hip).
+int depth_sort_launch(const DepthSortArgs& A, int max_pairs, void* space, size_t space_bytes,
+                      hipStream_t stream);
+int tile_sort_launch(const int* d_n, int max_pairs, int tiles, void* space, size_t space_bytes,
+                     const uint32_t* tkeys_in, const uint32_t* vals_in, uint32_t* tkeys_out,
+                     uint32_t* vals_out, int* tile_ranges, hipStream_t stream);
+// Boundary-scan tile ranges over 32-bit tile ids (depth-first pipeline).
+int tile_ranges_scan32_launch(const int* d_n, int max_pairs, int tiles, const uint32_t* tkeys,
+                              int* ranges, hipStream_t stream);
+// keys[i] = tile id << 32 | depth word of the Gaussian values[i] (sorted
+// 64-bit keys rebuilt after the depth-first pipeline's 32-bit tile sort).
+int rebuild_keys_launch(const int* d_n, int max_pairs, const uint32_t* tkeys,
+                        const uint32_t* values, const uint32_t* depth_words, uint64_t* keys,
+                        hipStream_t stream);
 // Stable radix sort of (key, value) pairs over key bits [0, end_bit).  When
 // `tile_ranges` is given, the keys' high words are tile ids below `tiles`
 // and the sort also leaves each tile's [start, end) there.

@@ -82,6 +82,39 @@ __global__ __launch_bounds__(kThreads) void ps_ranges_scan(const uint64_t* __res
   }
 }
 
+// The same over 32-bit tile ids (their ids need no masking).
+__global__ __launch_bounds__(kThreads) void ps_ranges_scan32(const uint32_t* __restrict__ keys,
+                                                             const int* __restrict__ d_n,
+                                                             int max_pairs, int tiles,
+                                                             int2* __restrict__ ranges) {
+  const int n = count_of(d_n, 0, max_pairs);
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const uint32_t cur = keys[i];
+    const bool cur_ok = cur < (uint32_t)tiles;
+    if (i == 0) {
+      if (cur_ok) ranges[cur].x = 0;
+    } else {
+      const uint32_t prev = keys[i - 1];
+      if (cur != prev) {
+        if (prev < (uint32_t)tiles) ranges[prev].y = i;
+        if (cur_ok) ranges[cur].x = i;
+      }
+    }
+    if (i == n - 1 && cur_ok) ranges[cur].y = n;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void ps_rebuild_keys(const int* __restrict__ d_n,
+                                                            int max_pairs,
+                                                            const uint32_t* __restrict__ tkeys,
+                                                            const uint32_t* __restrict__ values,
+                                                            const uint32_t* __restrict__ words,
+                                                            uint64_t* __restrict__ keys) {
+  const int n = count_of(d_n, 0, max_pairs);
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+    keys[i] = ((uint64_t)tkeys[i] << 32) | (uint64_t)words[values[i]];
+}
+
 __global__ __launch_bounds__(kThreads) void ps_gather_f32(
     const int* __restrict__ d_n, int n_host, int max_pairs, const uint32_t* __restrict__ order,
     const float2* __restrict__ sxy, const float4* __restrict__ srgbd,
@@ -127,6 +160,29 @@ int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles
                        n_host, max_pairs, tiles, mask, reinterpret_cast<int2*>(ranges));
     PS_LAUNCH_CHECK();
   }
+  return PS_OK;
+}
+
+int tile_ranges_scan32_launch(const int* d_n, int max_pairs, int tiles, const uint32_t* tkeys,
+                              int* ranges, hipStream_t stream) {
+  hipLaunchKernelGGL(ps_fill_int2, dim3(grid_for(tiles)), dim3(kThreads), 0, stream,
+                     reinterpret_cast<int2*>(ranges), tiles, (int*)nullptr, 0);
+  PS_LAUNCH_CHECK();
+  if (max_pairs > 0) {
+    hipLaunchKernelGGL(ps_ranges_scan32, dim3(grid_for(max_pairs)), dim3(kThreads), 0, stream,
+                       tkeys, d_n, max_pairs, tiles, reinterpret_cast<int2*>(ranges));
+    PS_LAUNCH_CHECK();
+  }
+  return PS_OK;
+}
+
+int rebuild_keys_launch(const int* d_n, int max_pairs, const uint32_t* tkeys,
+                        const uint32_t* values, const uint32_t* depth_words, uint64_t* keys,
+                        hipStream_t stream) {
+  if (max_pairs <= 0) return PS_OK;
+  hipLaunchKernelGGL(ps_rebuild_keys, dim3(grid_for(max_pairs)), dim3(kThreads), 0, stream, d_n,
+                     max_pairs, tkeys, values, depth_words, keys);
+  PS_LAUNCH_CHECK();
   return PS_OK;
 }
 

@@ -70,18 +70,36 @@ __device__ __forceinline__ int tile_item(int wave, int r, int lane) {
   return wave * kWaveItems + r * kWave + lane;
 }
 
+// Where a digit pass reads its pairs: arrays of keys (64-bit sort keys, or
+// 32-bit tile ids / depth words) and values, or, in the first pass of the
+// depth sort of the depth-first pipeline, the count pass's depth words with
+// each Gaussian's id as its value.
+template <typename KeyT>
+struct ArraySrc {
+  const KeyT* k;
+  const uint32_t* v;
+  __device__ __forceinline__ KeyT key(long long i) const { return k[i]; }
+  __device__ __forceinline__ uint32_t val(long long i) const { return v[i]; }
+};
+struct DepthSrc {
+  const uint32_t* k;
+  uint32_t subst;  // key for a Gaussian that emits nothing (depth word 0)
+  __device__ __forceinline__ uint32_t key(long long i) const {
+    const uint32_t x = k[i];
+    return x ? x : subst;
+  }
+  __device__ __forceinline__ uint32_t val(long long i) const { return (uint32_t)i; }
+};
+
 // counts layout: [digit][block].  Every block histograms the same contiguous
 // run of tiles the downsweep will give it.  DB: digit bits of the kernel
 // (8 or 9); a digit of fewer bits uses the low buckets.
-template <int DB>
-__global__ __launch_bounds__(kSortThreads) void ps_sort_upsweep(const uint64_t* __restrict__ keys,
-                                                                const int* __restrict__ d_n,
-                                                                int n_host, int max_pairs,
-                                                                int spec,
-                                                                uint32_t* __restrict__ counts) {
+template <int DB, typename Src>
+__device__ __forceinline__ void upsweep_body(const Src& src, const SortPlan& pl, int spec,
+                                             uint32_t* __restrict__ counts,
+                                             uint32_t (*hist)[1 << DB]) {
+  using KeyT = decltype(src.key(0));
   constexpr int D = 1 << DB, DPT = D / kSortThreads;
-  __shared__ uint32_t hist[4][D];
-  const SortPlan pl = sort_plan(d_n, n_host, max_pairs);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, b = blockIdx.x;
 #pragma unroll
   for (int q = 0; q < DPT; ++q) {
@@ -96,7 +114,7 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_upsweep(const uint64_t* 
   if (t_end > pl.ntiles) t_end = pl.ntiles;
   // The next tile's keys are loaded before this tile's counts are added, so
   // the loads of consecutive tiles overlap.
-  uint64_t k[kSortItems];
+  KeyT k[kSortItems];
   int tile_n = 0;
   if (t_begin < t_end) {
     const long long tile0 = (long long)t_begin * kSortTile;
@@ -104,21 +122,21 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_upsweep(const uint64_t* 
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
       const int i = tile_item(wave, r, lane);
-      k[r] = (i < tile_n) ? keys[tile0 + i] : 0ull;
+      k[r] = (i < tile_n) ? src.key(tile0 + i) : (KeyT)0;
     }
   }
   for (int t = t_begin; t < t_end; ++t) {
     int d[kSortItems];
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r)
-      d[r] = (tile_item(wave, r, lane) < tile_n) ? digit_of(k[r], spec) : -1;
+      d[r] = (tile_item(wave, r, lane) < tile_n) ? digit_of((uint64_t)k[r], spec) : -1;
     if (t + 1 < t_end) {
       const long long ntile0 = (long long)(t + 1) * kSortTile;
       tile_n = (pl.n - ntile0 < kSortTile) ? (int)(pl.n - ntile0) : kSortTile;
 #pragma unroll
       for (int r = 0; r < kSortItems; ++r) {
         const int i = tile_item(wave, r, lane);
-        k[r] = (i < tile_n) ? keys[ntile0 + i] : 0ull;
+        k[r] = (i < tile_n) ? src.key(ntile0 + i) : (KeyT)0;
       }
     }
 #pragma unroll
@@ -131,6 +149,17 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_upsweep(const uint64_t* 
     const int dd = tid + q * kSortThreads;
     counts[(size_t)dd * kSortBlocks + b] = hist[0][dd] + hist[1][dd] + hist[2][dd] + hist[3][dd];
   }
+}
+
+template <int DB, typename KeyT>
+__global__ __launch_bounds__(kSortThreads) void ps_sort_upsweep(const KeyT* __restrict__ keys,
+                                                                const int* __restrict__ d_n,
+                                                                int n_host, int max_pairs,
+                                                                int spec,
+                                                                uint32_t* __restrict__ counts) {
+  __shared__ uint32_t hist[4][1 << DB];
+  upsweep_body<DB>(ArraySrc<KeyT>{keys, nullptr}, sort_plan(d_n, n_host, max_pairs), spec, counts,
+                   hist);
 }
 
 // One workgroup per digit: exclusive scan of counts[d][0..kSortBlocks) into
@@ -180,16 +209,16 @@ __device__ __forceinline__ uint64_t match_digit_lds(int d, bool ok, int lane, ui
   return peers;
 }
 
-__device__ __forceinline__ void load_tile(const uint64_t* __restrict__ keys,
-                                          const uint32_t* __restrict__ vals, long long tile0,
-                                          int tile_n, int wave, int lane, uint64_t (&k)[kSortItems],
+template <typename Src, typename KeyT>
+__device__ __forceinline__ void load_tile(const Src& src, long long tile0, int tile_n, int wave,
+                                          int lane, KeyT (&k)[kSortItems],
                                           uint32_t (&v)[kSortItems]) {
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const int i = tile_item(wave, r, lane);
     if (i < tile_n) {
-      k[r] = keys[tile0 + i];
-      v[r] = vals[tile0 + i];
+      k[r] = src.key(tile0 + i);
+      v[r] = src.val(tile0 + i);
     } else {
       k[r] = 0;
       v[r] = 0;
@@ -197,31 +226,34 @@ __device__ __forceinline__ void load_tile(const uint64_t* __restrict__ keys,
   }
 }
 
+// LDS of one downsweep workgroup, declared by the kernel.
+template <int DB, typename KeyT>
+struct DownLds {
+  KeyT skey[kSortTile];
+  uint32_t sval[kSortTile];
+  // per-wave digit counters while ranking, then (same slots) the tile-local
+  // start of each (wave, digit) run; wcnt[0][d] is digit d's tile-local
+  // start.  Thread t owns digits t*DPT .. t*DPT + DPT - 1.
+  uint32_t wcnt[4][1 << DB];
+  uint32_t gbase[1 << DB];
+  uint2 s_match[4][1 << DB];  // per-wave digit lane masks (match_digit_lds)
+  int scratch[4];
+};
+
 // Rank, stage and scatter.  Per tile: each wave ranks its 960 items against
 // its own LDS digit counters (a wave's LDS operations execute in order, so no
 // barrier is needed between rounds); one block scan turns the per-wave counts
 // into tile-local starts; items are staged in LDS grouped by digit and each
 // digit run is written contiguously.  The next tile's keys are loaded into
 // registers while the current one is scanned, staged and written (about 210
-// VGPRs: two workgroups per CU, no spills).
-template <int DB>
-__global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
-    const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
-    uint64_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, const int* __restrict__ d_n,
-    int n_host, int max_pairs, int spec, const uint32_t* __restrict__ offsets,
-    const uint32_t* __restrict__ totals) {
+// VGPRs with 64-bit keys: two workgroups per CU, no spills).
+template <int DB, typename Src, typename KeyT>
+__device__ __forceinline__ void downsweep_body(const Src& src, KeyT* __restrict__ keys_out,
+                                               uint32_t* __restrict__ vals_out, const SortPlan& pl,
+                                               int spec, const uint32_t* __restrict__ offsets,
+                                               const uint32_t* __restrict__ totals,
+                                               DownLds<DB, KeyT>& L) {
   constexpr int D = 1 << DB, DPT = D / kSortThreads;
-  __shared__ uint64_t skey[kSortTile];
-  __shared__ uint32_t sval[kSortTile];
-  // wcnt: per-wave digit counters while ranking, then (same slots) the
-  // tile-local start of each (wave, digit) run; wcnt[0][d] is digit d's
-  // tile-local start.  Thread t owns digits t*DPT .. t*DPT + DPT - 1.
-  __shared__ uint32_t wcnt[4][D];
-  __shared__ uint32_t gbase[D];
-  __shared__ uint2 s_match[4][D];  // per-wave digit lane masks (match_digit_lds)
-  __shared__ int scratch[4];
-
-  const SortPlan pl = sort_plan(d_n, n_host, max_pairs);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, b = blockIdx.x;
   const int t_begin = b * pl.tiles_per_block;
   int t_end = t_begin + pl.tiles_per_block;
@@ -237,28 +269,28 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
       local += (int)tv[q];
     }
     int dtotal;
-    uint32_t run = (uint32_t)block256_excl_scan(local, scratch, dtotal);
+    uint32_t run = (uint32_t)block256_excl_scan(local, L.scratch, dtotal);
 #pragma unroll
     for (int q = 0; q < DPT; ++q) {
       const int dd = tid * DPT + q;
-      gbase[dd] = run + offsets[(size_t)dd * kSortBlocks + b];
+      L.gbase[dd] = run + offsets[(size_t)dd * kSortBlocks + b];
       run += tv[q];
-      wcnt[0][dd] = 0;
-      wcnt[1][dd] = 0;
-      wcnt[2][dd] = 0;
-      wcnt[3][dd] = 0;
+      L.wcnt[0][dd] = 0;
+      L.wcnt[1][dd] = 0;
+      L.wcnt[2][dd] = 0;
+      L.wcnt[3][dd] = 0;
     }
   }
   const uint64_t below = lanes_below(lane);
-  uint32_t* const mt = reinterpret_cast<uint32_t*>(s_match[wave]);
+  uint32_t* const mt = reinterpret_cast<uint32_t*>(L.s_match[wave]);
   for (int i = lane; i < 2 * D; i += kWave) mt[i] = 0u;
   __builtin_amdgcn_wave_barrier();
 
-  uint64_t k[kSortItems];
+  KeyT k[kSortItems];
   uint32_t v[kSortItems];
   long long tile0 = (long long)t_begin * kSortTile;
   int tile_n = (pl.n - tile0 < kSortTile) ? (int)(pl.n - tile0) : kSortTile;
-  load_tile(keys_in, vals_in, tile0, tile_n, wave, lane, k, v);
+  load_tile(src, tile0, tile_n, wave, lane, k, v);
   __syncthreads();
 
   for (int t = t_begin; t < t_end; ++t) {
@@ -266,76 +298,76 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
     uint32_t pos[kSortItems];
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
-      const int d = (tile_item(wave, r, lane) < tile_n) ? digit_of(k[r], spec) : -1;
+      const int d = (tile_item(wave, r, lane) < tile_n) ? digit_of((uint64_t)k[r], spec) : -1;
       dg[r] = d;
       const uint64_t peers = match_digit_lds(d >= 0 ? d : 0, d >= 0, lane, mt);
       const int rank = popc64(peers & below);
       uint32_t base = 0;
-      if (d >= 0) base = wcnt[wave][d];
+      if (d >= 0) base = L.wcnt[wave][d];
       __builtin_amdgcn_wave_barrier();
-      if (d >= 0 && rank == 0) wcnt[wave][d] = base + (uint32_t)popc64(peers);
+      if (d >= 0 && rank == 0) L.wcnt[wave][d] = base + (uint32_t)popc64(peers);
       __builtin_amdgcn_wave_barrier();
       pos[r] = base + (uint32_t)rank;
     }
     // Prefetch the next tile while this one is scanned, staged and written.
-    uint64_t kn[kSortItems];
+    KeyT kn[kSortItems];
     uint32_t vn[kSortItems];
     const long long ntile0 = tile0 + kSortTile;
     const int ntile_n =
         (t + 1 < t_end) ? ((pl.n - ntile0 < kSortTile) ? (int)(pl.n - ntile0) : kSortTile) : 0;
-    if (ntile_n > 0) load_tile(keys_in, vals_in, ntile0, ntile_n, wave, lane, kn, vn);
+    if (ntile_n > 0) load_tile(src, ntile0, ntile_n, wave, lane, kn, vn);
     __syncthreads();
     uint32_t c0[DPT], c1[DPT], c2[DPT], c3[DPT], tot[DPT];
     int local = 0;
 #pragma unroll
     for (int q = 0; q < DPT; ++q) {
       const int dd = tid * DPT + q;
-      c0[q] = wcnt[0][dd];
-      c1[q] = wcnt[1][dd];
-      c2[q] = wcnt[2][dd];
-      c3[q] = wcnt[3][dd];
+      c0[q] = L.wcnt[0][dd];
+      c1[q] = L.wcnt[1][dd];
+      c2[q] = L.wcnt[2][dd];
+      c3[q] = L.wcnt[3][dd];
       tot[q] = c0[q] + c1[q] + c2[q] + c3[q];
       local += (int)tot[q];
     }
     int tile_total;
-    uint32_t ls = (uint32_t)block256_excl_scan(local, scratch, tile_total);
+    uint32_t ls = (uint32_t)block256_excl_scan(local, L.scratch, tile_total);
 #pragma unroll
     for (int q = 0; q < DPT; ++q) {
       const int dd = tid * DPT + q;
-      wcnt[0][dd] = ls;
-      wcnt[1][dd] = ls + c0[q];
-      wcnt[2][dd] = ls + c0[q] + c1[q];
-      wcnt[3][dd] = ls + c0[q] + c1[q] + c2[q];
+      L.wcnt[0][dd] = ls;
+      L.wcnt[1][dd] = ls + c0[q];
+      L.wcnt[2][dd] = ls + c0[q] + c1[q];
+      L.wcnt[3][dd] = ls + c0[q] + c1[q] + c2[q];
       ls += tot[q];
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
       if (dg[r] >= 0) {
-        const uint32_t p = wcnt[wave][dg[r]] + pos[r];
-        skey[p] = k[r];
-        sval[p] = v[r];
+        const uint32_t p = L.wcnt[wave][dg[r]] + pos[r];
+        L.skey[p] = k[r];
+        L.sval[p] = v[r];
       }
     }
     __syncthreads();
     for (int i = tid; i < tile_n; i += kSortThreads) {
-      const uint64_t key = skey[i];
-      const int d = digit_of(key, spec);
-      const uint32_t dst = gbase[d] + ((uint32_t)i - wcnt[0][d]);
+      const KeyT key = L.skey[i];
+      const int d = digit_of((uint64_t)key, spec);
+      const uint32_t dst = L.gbase[d] + ((uint32_t)i - L.wcnt[0][d]);
       if (dst < (uint32_t)pl.n) {
         keys_out[dst] = key;
-        vals_out[dst] = sval[i];
+        vals_out[dst] = L.sval[i];
       }
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < DPT; ++q) {
       const int dd = tid * DPT + q;
-      gbase[dd] += tot[q];
-      wcnt[0][dd] = 0;
-      wcnt[1][dd] = 0;
-      wcnt[2][dd] = 0;
-      wcnt[3][dd] = 0;
+      L.gbase[dd] += tot[q];
+      L.wcnt[0][dd] = 0;
+      L.wcnt[1][dd] = 0;
+      L.wcnt[2][dd] = 0;
+      L.wcnt[3][dd] = 0;
     }
     tile0 = ntile0;
     tile_n = ntile_n;
@@ -348,6 +380,94 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
   }
 }
 
+template <int DB, typename KeyT>
+__global__ __launch_bounds__(kSortThreads) void ps_sort_downsweep(
+    const KeyT* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    KeyT* __restrict__ keys_out, uint32_t* __restrict__ vals_out, const int* __restrict__ d_n,
+    int n_host, int max_pairs, int spec, const uint32_t* __restrict__ offsets,
+    const uint32_t* __restrict__ totals) {
+  __shared__ DownLds<DB, KeyT> L;
+  downsweep_body<DB>(ArraySrc<KeyT>{keys_in, vals_in}, keys_out, vals_out,
+                     sort_plan(d_n, n_host, max_pairs), spec, offsets, totals, L);
+}
+
+// ------------------------------------------------------------ depth sort
+// Stable sort of the Gaussians by the 32-bit words of their depths (the
+// depth-first pipeline): 8-bit digit passes, of which only the digits that
+// differ among the emitting Gaussians run.  The count pass leaves each
+// Gaussian's depth word (0 when it emits nothing) and, in kPlanSlots slots,
+// the OR and the AND of the emitting depths; every kernel of the sort
+// derives the same plan from them: varying = OR ^ AND, digit q runs when its
+// byte of `varying` is non-zero, reading what the previous running digit
+// wrote (or, the first, the depth words with ids as values, a Gaussian that
+// emits nothing taking the AND as its key) into buffer (index among running
+// digits) % 2.  No digit runs when no Gaussian emits or all emitting
+// depths are equal: the sorted order is then the identity.
+
+// Reads the plan (every thread gets it through LDS).
+__device__ __forceinline__ void read_depth_plan(const uint32_t* __restrict__ plan,
+                                                uint32_t* s_plan) {
+  const int tid = threadIdx.x;
+  if (tid < kWave) {
+    uint32_t o = plan[tid * kPlanStride], a = plan[(kPlanSlots + tid) * kPlanStride];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      o |= (uint32_t)__shfl_xor((int)o, d);
+      a &= (uint32_t)__shfl_xor((int)a, d);
+    }
+    if (tid == 0) {
+      s_plan[0] = o == 0u ? 0u : (o ^ a);  // varying bits (none without emitters)
+      s_plan[1] = a;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ bool digit_runs(uint32_t varying, int q) {
+  return ((varying >> (8 * q)) & 0xffu) != 0u;
+}
+
+__device__ __forceinline__ int digits_before(uint32_t varying, int q) {
+  int k = 0;
+  for (int i = 0; i < q; ++i) k += digit_runs(varying, i) ? 1 : 0;
+  return k;
+}
+
+__global__ __launch_bounds__(kSortThreads) void ps_dsort_upsweep(DepthSortArgs A, int q,
+                                                                 uint32_t* __restrict__ counts) {
+  __shared__ uint32_t hist[4][256];
+  __shared__ uint32_t s_plan[2];
+  read_depth_plan(A.plan, s_plan);
+  const uint32_t varying = s_plan[0];
+  if (!digit_runs(varying, q)) return;  // uniform
+  const int k = digits_before(varying, q);
+  const SortPlan pl = sort_plan(nullptr, A.n, A.n);
+  const int spec = digit_spec(8 * q, 8);
+  if (k == 0)
+    upsweep_body<8>(DepthSrc{A.words, s_plan[1]}, pl, spec, counts, hist);
+  else
+    upsweep_body<8>(ArraySrc<uint32_t>{A.kb[(k - 1) & 1], A.vb[(k - 1) & 1]}, pl, spec, counts,
+                    hist);
+}
+
+__global__ __launch_bounds__(kSortThreads) void ps_dsort_downsweep(
+    DepthSortArgs A, int q, const uint32_t* __restrict__ offsets,
+    const uint32_t* __restrict__ totals) {
+  __shared__ DownLds<8, uint32_t> L;
+  __shared__ uint32_t s_plan[2];
+  read_depth_plan(A.plan, s_plan);
+  const uint32_t varying = s_plan[0];
+  if (!digit_runs(varying, q)) return;  // uniform
+  const int k = digits_before(varying, q);
+  const SortPlan pl = sort_plan(nullptr, A.n, A.n);
+  const int spec = digit_spec(8 * q, 8);
+  if (k == 0)
+    downsweep_body<8>(DepthSrc{A.words, s_plan[1]}, A.kb[0], A.vb[0], pl, spec, offsets, totals,
+                      L);
+  else
+    downsweep_body<8>(ArraySrc<uint32_t>{A.kb[(k - 1) & 1], A.vb[(k - 1) & 1]}, A.kb[k & 1],
+                      A.vb[k & 1], pl, spec, offsets, totals, L);
+}
 
 // ------------------------------------------------------------ segment pass
 // After the digit passes over the tile bits [32, end_bit), pairs are grouped
@@ -892,40 +1012,44 @@ inline bool use_segmented() {
   return !(e && e[0] == 'l');
 }
 
-template <int DB>
-void launch_digit_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout,
+template <int DB, typename KeyT>
+void launch_digit_pass(const KeyT* kin, const uint32_t* vin, KeyT* kout, uint32_t* vout,
                        int n_host, const int* d_n, int max_pairs, int spec, const SortWs& w,
                        hipStream_t stream) {
-  hipLaunchKernelGGL(ps_sort_upsweep<DB>, dim3(kSortBlocks), dim3(kSortThreads), 0, stream, kin,
-                     d_n, n_host, max_pairs, spec, w.counts);
+  hipLaunchKernelGGL((ps_sort_upsweep<DB, KeyT>), dim3(kSortBlocks), dim3(kSortThreads), 0,
+                     stream, kin, d_n, n_host, max_pairs, spec, w.counts);
   hipLaunchKernelGGL(ps_sort_scan, dim3(1 << DB), dim3(kSortThreads), 0, stream, w.counts,
                      w.offsets, w.totals);
-  hipLaunchKernelGGL(ps_sort_downsweep<DB>, dim3(kSortBlocks), dim3(kSortThreads), 0, stream,
-                     kin, vin, kout, vout, d_n, n_host, max_pairs, spec, w.offsets, w.totals);
+  hipLaunchKernelGGL((ps_sort_downsweep<DB, KeyT>), dim3(kSortBlocks), dim3(kSortThreads), 0,
+                     stream, kin, vin, kout, vout, d_n, n_host, max_pairs, spec, w.offsets,
+                     w.totals);
 }
 
 // Stable LSD passes over key bits [bit0, bit1): 8-bit digits, or 9-bit ones
 // when that needs fewer passes; the bits are spread evenly over the passes.
+// `alt_keys`/`alt_vals` hold the pairs between passes.
+template <typename KeyT>
 int digit_passes(int n_host, const int* d_n, int max_pairs, int bit0, int bit1, const SortWs& w,
-                 const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
+                 const KeyT* keys_in, const uint32_t* vals_in, KeyT* keys_out,
                  uint32_t* vals_out, hipStream_t stream) {
   const int bits = bit1 - bit0;
   const int p8 = (bits + 7) / 8, p9 = (bits + kMaxDigitBits - 1) / kMaxDigitBits;
   const int passes = p9 < p8 ? p9 : p8;
-  const uint64_t* kin = keys_in;
+  const KeyT* kin = keys_in;
   const uint32_t* vin = vals_in;
+  KeyT* const alt_keys = reinterpret_cast<KeyT*>(w.alt_keys);
   int lo = bit0;
   for (int p = 0; p < passes; ++p) {
     // Ping-pong so that the last pass lands in the caller's output.
     const bool to_out = ((passes - 1 - p) % 2) == 0;
-    uint64_t* kout = to_out ? keys_out : w.alt_keys;
+    KeyT* kout = to_out ? keys_out : alt_keys;
     uint32_t* vout = to_out ? vals_out : w.alt_vals;
     const int width = (bit1 - lo + (passes - p) - 1) / (passes - p);  // <= 9
     const int spec = digit_spec(lo, width);
     if (width > 8)
-      launch_digit_pass<9>(kin, vin, kout, vout, n_host, d_n, max_pairs, spec, w, stream);
+      launch_digit_pass<9, KeyT>(kin, vin, kout, vout, n_host, d_n, max_pairs, spec, w, stream);
     else
-      launch_digit_pass<8>(kin, vin, kout, vout, n_host, d_n, max_pairs, spec, w, stream);
+      launch_digit_pass<8, KeyT>(kin, vin, kout, vout, n_host, d_n, max_pairs, spec, w, stream);
     PS_LAUNCH_CHECK();
     lo += width;
     kin = kout;
@@ -1018,6 +1142,47 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
                      w.alt_keys, w.alt_vals, rg, w.big_list, w.big_count + 3, big_cap);
   PS_LAUNCH_CHECK();
   return PS_OK;
+}
+
+// Depth sort of the depth-first pipeline (see DepthSortArgs): four 8-bit
+// digit launches each; a digit that does not differ among the emitting
+// depths returns at once.  The sort space supplies the digit counts only.
+int depth_sort_launch(const DepthSortArgs& A, int max_pairs, void* space, size_t space_bytes,
+                      hipStream_t stream) {
+  PS_REQUIRE(space != nullptr && space_bytes >= sort_layout(max_pairs, nullptr, nullptr),
+             "sort space too small");
+  if (A.n <= 0) return PS_OK;
+  SortWs w;
+  sort_layout(max_pairs, static_cast<char*>(space), &w);
+  for (int q = 0; q < 4; ++q) {
+    hipLaunchKernelGGL(ps_dsort_upsweep, dim3(kSortBlocks), dim3(kSortThreads), 0, stream, A, q,
+                       w.counts);
+    hipLaunchKernelGGL(ps_sort_scan, dim3(256), dim3(kSortThreads), 0, stream, w.counts,
+                       w.offsets, w.totals);
+    hipLaunchKernelGGL(ps_dsort_downsweep, dim3(kSortBlocks), dim3(kSortThreads), 0, stream, A,
+                       q, (const uint32_t*)w.offsets, (const uint32_t*)w.totals);
+    PS_LAUNCH_CHECK();
+  }
+  return PS_OK;
+}
+
+// Tile sort of the depth-first pipeline: stable sort of the (tile id, value)
+// pairs, emitted in depth order, by their 32-bit tile ids (bits
+// [0, bit_length(tiles))), then the tile ranges by a boundary scan.  The
+// pair count comes from device memory.
+int tile_sort_launch(const int* d_n, int max_pairs, int tiles, void* space, size_t space_bytes,
+                     const uint32_t* tkeys_in, const uint32_t* vals_in, uint32_t* tkeys_out,
+                     uint32_t* vals_out, int* tile_ranges, hipStream_t stream) {
+  PS_REQUIRE(space != nullptr && space_bytes >= sort_layout(max_pairs, nullptr, nullptr),
+             "sort space too small");
+  PS_REQUIRE(tiles > 0, "tile ranges need a tile count");
+  PS_REQUIRE(tkeys_in != tkeys_out && vals_in != vals_out, "sort outputs must not alias inputs");
+  SortWs w;
+  sort_layout(max_pairs, static_cast<char*>(space), &w);
+  const int st = digit_passes<uint32_t>(0, d_n, max_pairs, 0, bit_length((unsigned)tiles), w,
+                                        tkeys_in, vals_in, tkeys_out, vals_out, stream);
+  if (st != PS_OK || tile_ranges == nullptr) return st;
+  return tile_ranges_scan32_launch(d_n, max_pairs, tiles, tkeys_out, tile_ranges, stream);
 }
 
 int sort_end_bit(int width, int height, int block_x, int block_y) {

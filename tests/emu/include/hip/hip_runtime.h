@@ -184,6 +184,10 @@ inline T atomicOr(T* p, T v) {
   return std::atomic_ref<T>(*p).fetch_or(v);
 }
 template <typename T>
+inline T atomicAnd(T* p, T v) {
+  return std::atomic_ref<T>(*p).fetch_and(v);
+}
+template <typename T>
 inline T atomicMax(T* p, T v) {
   std::atomic_ref<T> a(*p);
   T cur = a.load();

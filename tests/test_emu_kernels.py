@@ -394,3 +394,66 @@ def test_posted_count_readback(emu):
     st, t2, _ = small.forward((0.1, 0.2, 0.3), sync=2)
     capi.check_with(emu, st)
     assert emu.ps_forward_wait(t2, capi.C.byref(n)) == capi.PS_ERR_OVERFLOW and n.value == M
+
+
+# ------------------------------------------------------------------ depth-first pipeline
+DEPTH_FIRST = 0x100
+
+
+def _flat_scene(P=600, seed=47):
+    """Every Gaussian at the same depth: no digit of the depth sort runs."""
+    sc = scenes.make_scene(P, 4.0, [(0.8, -3.0, 0.4), (0.2, -1.5, 0.3)], seed=seed)
+    sc.position[:, 2] = 0.0
+    return sc
+
+
+@pytest.mark.parametrize("case,variant", [
+    ("mixed", capi.V_LITE), ("mixed", capi.V_FUSED), ("mixed", capi.V_DEFAULT),
+    ("pool_overflow", capi.V_LITE), ("flat", capi.V_LITE), ("empty", capi.V_LITE),
+    ("routes", capi.V_LITE)])
+def test_depth_first_pipeline_equals_default(emu, case, variant):
+    if case == "mixed":
+        sc, spec = _scene(), scenes.make_camera(200, 120, 10.0, distance=12.0)
+    elif case == "pool_overflow":
+        sc = scenes.make_scene(2000, 1.0, [(0.5, 0.0, 0.2), (0.3, -2.6, 0.3), (0.2, -4.0, 0.3)],
+                               seed=43)
+        spec = scenes.make_camera(300, 200, 1.0, distance=4.0)
+    elif case == "flat":
+        sc, spec = _flat_scene(), scenes.make_camera(160, 96, 4.0, distance=6.0)
+    elif case == "empty":
+        sc = scenes.make_scene(300, 2.0, [(1.0, -3.0, 0.3)], seed=5)
+        sc.position[:, 2] -= 50.0  # behind the camera
+        spec = scenes.make_camera(64, 48, 2.0)
+    else:
+        sc, spec = _emission_scene()
+        spec = scenes.make_camera(320, 240, 6.0, distance=7.0)
+    prec = "fp32" if variant == capi.V_DEFAULT else "fp16"
+    bg = (0.3, 0.1, 0.6)
+    ref = oracle_frame(sc, spec, sh_type=prec, feat_prec=prec, bg=bg,
+                       check_every={capi.V_LITE: 8, capi.V_FUSED: 16, capi.V_DEFAULT: 2}[variant],
+                       fp64=False)
+    M = ref["pre"]["M"]
+    std = HF.HostFrame(emu, sc, spec, sh_type=prec, feat_prec=prec, capacity=M + 40)
+    st, n, img_std = std.forward(bg, variant)
+    capi.check_with(emu, st)
+    es = HF.HostFrame(emu, sc, spec, sh_type=prec, feat_prec=prec, capacity=M + 40,
+                      garbage=True, seed=9)
+    st, n2, img_es = es.forward(bg, variant | DEPTH_FIRST)
+    capi.check_with(emu, st)
+    assert n == n2 == M
+    np.testing.assert_array_equal(es.ks[:M], ref["keys_sorted"])
+    np.testing.assert_array_equal(es.vs[:M], ref["vals_sorted"])
+    np.testing.assert_array_equal(img_es, img_std)
+    if variant == capi.V_LITE:
+        np.testing.assert_array_equal(es.ranges, std.ranges)
+    assert np.abs(img_es.astype(int) - ref["image"].astype(int)).max() <= 1
+
+
+def test_depth_first_overflow_is_reported(emu):
+    sc, spec = _scene(), scenes.make_camera(200, 120, 10.0, distance=12.0)
+    ref = oracle_frame(sc, spec, sh_type="fp16", feat_prec="fp16", fp64=False)
+    M = ref["pre"]["M"]
+    f = HF.HostFrame(emu, sc, spec, capacity=M - 1)
+    st, n, img = f.forward((0.5, 0.5, 0.5), capi.V_LITE | DEPTH_FIRST)
+    assert st == capi.PS_ERR_OVERFLOW and n == M
+    assert (img == 127).all()  # nothing sorted: every tile shows the background
