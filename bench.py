@@ -27,6 +27,7 @@ sys.path.insert(0, str(ROOT))
 
 VALU_PEAK_GINST = 1024 * 2.4 / 4  # wave64 FP32 VALU instructions, G/s
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+DEPTH_FIRST = 0x100  # PS_FORWARD_DEPTH_FIRST: the depth-first key pipeline
 METRIC = "ms/frame + HBM GB/s @4K/6M Gaussians FP16 vs FP32; p99 latency @60Hz stream"
 
 
@@ -45,6 +46,8 @@ def parse():
     ap.add_argument("--resolution", default="",
                     help="override the config's WxH (profiling other sizes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
+    ap.add_argument("--depth-first", choices=("auto", "on", "off"), default="auto",
+                    help="key pipeline of the timed frames: depth-first (ES) or default")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle threads (default: OMP_NUM_THREADS, else the host's cores)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
@@ -203,12 +206,12 @@ def main():
     VAR = polysplat.rasterizer.VARIANT_IDS
     PREC = {"fp16": 1, "fp32": 0, "bf16": 2}
 
-    def frame_fn(rast, variant):
+    def frame_fn(rast, variant, depth_first=args.depth_first == "on"):
         fp32 = variant == "default"
         prec = 0 if fp32 else PREC[rast.feature_dtype]
         shs = scene.shs if fp32 else (scene.shs_bf16 if prec == 2 else scene.shs_half)
-        v = VAR[variant]
-        rng = rast.tile_ranges_buf if v == 1 else rast._empty
+        v = VAR[variant] | (DEPTH_FIRST if depth_first else 0)
+        rng = rast.tile_ranges_buf if VAR[variant] == 1 else rast._empty
         rec = rast.records if prec else rast._empty
 
         def run(sync=False):
@@ -242,6 +245,25 @@ def main():
     if "fp16_persistent" not in paths:
         print(json.dumps({k: v for k, v in results.items()}), flush=True)
         return
+    # the other key pipeline on the FP16 persistent frame, for comparison
+    rast, variant = paths["fp16_persistent"]
+    other = frame_fn(rast, variant, depth_first=args.depth_first != "on")
+    other(sync=True)
+    for _ in range(args.warmup):
+        other()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        other()
+    torch.cuda.synchronize()
+    other_ms = (time.perf_counter() - t) * 1e3 / args.steps
+    other_img = rast._get_out_color(H, W, dev).cpu().numpy()
+    pipelines = {"timed": "depth_first" if args.depth_first == "on" else "default",
+                 "other_ms_per_frame": round(other_ms, 4),
+                 "other_image_identical": bool((other_img == images["fp16_persistent"]).all())}
+    log(f"fp16_persistent, other key pipeline: {other_ms:.3f} ms/frame, "
+        f"identical image: {pipelines['other_image_identical']}")
+    frame_fn(rast, variant)(sync=True)  # leave the timed pipeline's buffers behind
     # ---------------------------------------------------------- per-stage timing
     rast, _ = paths["fp16_persistent"]
     M = results["fp16_persistent"]["num_rendered"]
@@ -471,6 +493,7 @@ def main():
                             for k, v in quality.items()},
         "roofline": roof,
         "tile_lists": tile_lists,
+        "key_pipelines": pipelines,
         "cpu_baseline": cpu,
     }
     if args.serve_requests > 0:

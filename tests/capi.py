@@ -17,6 +17,7 @@ HEADER = ROOT / "include" / "polysplat_amd.h"
 PS_OK, PS_ERR_INVALID, PS_ERR_HIP, PS_ERR_OVERFLOW, PS_ERR_UNSUPPORTED = range(5)
 F32, F16, BF16 = 0, 1, 2
 V_DEFAULT, V_LITE, V_FUSED, V_DT = 0, 1, 2, 3
+DEPTH_FIRST = 0x100  # PS_FORWARD_DEPTH_FIRST, OR-ed into a variant
 MAX_BATCH_FRAMES = 64
 
 

@@ -35,6 +35,7 @@ pytestmark = pytest.mark.gpu
 
 DEV = torch.device("cuda:0")
 BG = (0.0, 0.0, 0.0)
+DEPTH_FIRST = capi.DEPTH_FIRST
 
 
 def _host_view(a):
@@ -95,30 +96,36 @@ def test_c3_keys_order_ranges_features_bit_exact(c3):
     np.testing.assert_array_equal(g.ranges_scan(), O.tile_ranges(ref["keys_sorted"], g.tiles, 1))
 
 
-def test_c3_fp32_plain_path(c3):
+@pytest.mark.parametrize("depth_first", [False, True])
+def test_c3_fp32_plain_path(c3, depth_first):
     arrays, host, spec, ref = c3
     M = ref["pre"]["M"]
     g = Gpu(arrays, spec, capacity=M + 4096)
-    st, n, img = g.forward(BG, capi.V_DEFAULT)
+    st, n, img = g.forward(BG, capi.V_DEFAULT | (DEPTH_FIRST if depth_first else 0))
     capi.check(st)
     assert n == M
+    np.testing.assert_array_equal(_keys(g.ks, M), ref["keys_sorted"])
+    np.testing.assert_array_equal(_vals(g.vs, M), ref["vals_sorted"])
     r2 = oracle_frame(host, spec, bg=BG, check_every=2, fp64=False)
     m64 = _quality(img, r2, ref["image64"], "c3 fp32 plain")
     assert m64["max_abs_diff"] <= 1
 
 
-@pytest.mark.parametrize("prec", ["fp16", "bf16"])
-def test_c3_reduced_precision_persistent_path(c3, prec):
+@pytest.mark.parametrize("prec,depth_first", [("fp16", False), ("bf16", False),
+                                               ("fp16", True)])
+def test_c3_reduced_precision_persistent_path(c3, prec, depth_first):
     arrays, host, spec, ref = c3
     M = ref["pre"]["M"]
     g = Gpu(arrays, spec, sh_type=prec, feat_prec=prec, capacity=M + 4096)
-    st, n, img = g.forward(BG, capi.V_LITE)
+    st, n, img = g.forward(BG, capi.V_LITE | (DEPTH_FIRST if depth_first else 0))
     capi.check(st)
     assert n == M
     # the SH and feature precision change colours only: keys and order are
     # those of the FP32 scene
     np.testing.assert_array_equal(_keys(g.ks, M), ref["keys_sorted"])
     np.testing.assert_array_equal(_vals(g.vs, M), ref["vals_sorted"])
+    np.testing.assert_array_equal(g.ranges.cpu().numpy(),
+                                  O.tile_ranges(ref["keys_sorted"], g.tiles, 1))
     rq = oracle_frame(host, spec, sh_type=prec, feat_prec=prec, bg=BG, check_every=8, fp64=False)
     m64 = _quality(img, rq, ref["image64"], f"c3 {prec} persistent")
     print(f"c3 {prec} features: PSNR {m64['psnr_db']:.2f} dB vs FP64, max-abs "
@@ -139,15 +146,17 @@ def c4(gpu):
     return arrays, host, spec, ref
 
 
-def test_c4_city_bit_exact_and_image(c4):
+@pytest.mark.parametrize("depth_first", [False, True])
+def test_c4_city_bit_exact_and_image(c4, depth_first):
     arrays, host, spec, ref = c4
     M = ref["pre"]["M"]
     g = Gpu(arrays, spec, sh_type="fp16", feat_prec="fp16", capacity=M + 4096)
-    say("c4 forward")
-    st, n, img = g.forward(BG, capi.V_LITE)
+    say("c4 forward", "depth first" if depth_first else "default")
+    st, n, img = g.forward(BG, capi.V_LITE | (DEPTH_FIRST if depth_first else 0))
     capi.check(st)
     assert n == M
-    np.testing.assert_array_equal(_keys(g.ku, M), ref["pre"]["keys"])
+    if not depth_first:  # the depth-first pipeline keeps 32-bit tile ids there
+        np.testing.assert_array_equal(_keys(g.ku, M), ref["pre"]["keys"])
     np.testing.assert_array_equal(_keys(g.ks, M), ref["keys_sorted"])
     np.testing.assert_array_equal(_vals(g.vs, M), ref["vals_sorted"])
     rng = g.ranges.cpu().numpy()  # the sort's boundary scan: empty tiles (0, 0)

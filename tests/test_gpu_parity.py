@@ -163,6 +163,45 @@ def test_forward_pipeline_matches_stages(case):
     np.testing.assert_array_equal(img1, img2)
 
 
+@pytest.mark.parametrize("variant", [capi.V_DEFAULT, capi.V_LITE, capi.V_FUSED])
+def test_depth_first_pipeline(case, variant):
+    """Depth-first key pipeline (Gaussians sorted by depth, pairs emitted in
+    depth order as 32-bit tile ids, stable tile sort, rebuilt keys): the same
+    sorted keys, values and ranges as the oracle, and the same image bytes as
+    the default pipeline of the variant."""
+    name, scene, spec, ref = case
+    g = Gpu(scene, spec, capacity=_cap(ref))
+    say("depth first", name, variant)
+    st, n, img0 = g.forward(BG, variant)
+    capi.check(st)
+    st, n, img = g.forward(BG, variant | capi.DEPTH_FIRST)
+    capi.check(st)
+    M = ref["pre"]["M"]
+    assert n == M
+    np.testing.assert_array_equal(g.ks[:M].cpu().numpy().view(np.uint64), ref["keys_sorted"])
+    np.testing.assert_array_equal(g.vs[:M].cpu().numpy().view(np.uint32), ref["vals_sorted"])
+    if variant == capi.V_LITE:
+        np.testing.assert_array_equal(g.ranges.cpu().numpy(),
+                                      O.tile_ranges(ref["keys_sorted"], g.tiles, 1))
+    np.testing.assert_array_equal(img, img0)
+
+
+def test_depth_first_overflow(gpu):
+    scene = scenes.make_config_scene("c1", seed=2, P=20_000)
+    spec = scenes.config_camera("c1")
+    ref = oracle_frame(scene, spec, fp64=False)
+    M = ref["pre"]["M"]
+    g = Gpu(scene, spec, capacity=M - 1)
+    say("depth first overflow", M)
+    st, n, _ = g.forward(BG, capi.V_LITE | capi.DEPTH_FIRST)
+    assert st == capi.PS_ERR_OVERFLOW and n == M
+    g = Gpu(scene, spec, capacity=M)
+    st, n, img = g.forward(BG, capi.V_LITE | capi.DEPTH_FIRST)
+    capi.check(st)
+    assert n == M
+    np.testing.assert_array_equal(g.ks[:M].cpu().numpy().view(np.uint64), ref["keys_sorted"])
+
+
 def test_overflow_is_reported_and_nothing_is_written_past_capacity(gpu):
     scene = scenes.make_config_scene("c1", seed=2, P=20_000)
     spec = scenes.config_camera("c1")
@@ -207,16 +246,17 @@ def test_preprocess_appends_at_curr_offset(gpu):
     assert torch.equal(g.ku[M:2 * M], first) and torch.equal(g.ku[:M], first)
 
 
+@pytest.mark.parametrize("depth_first", [False, True])
 @pytest.mark.parametrize("P", [0, 1, 2])
-def test_tiny_and_empty_scenes(gpu, P):
+def test_tiny_and_empty_scenes(gpu, P, depth_first):
     scene = scenes.make_scene(max(P, 1), 2.0, [(1.0, -2.0, 0.3)], seed=11)
     if P == 0:
         scene = scenes.SceneArrays(scene.position[:0], scene.shs[:0], scene.opacity[:0],
                                    scene.cov3d[:0])
     spec = scenes.make_camera(50, 30, 2.0)
     g = Gpu(scene, spec)
-    say("tiny", P)
-    st, n, img = g.forward((0.2, 0.4, 0.6), capi.V_LITE)
+    say("tiny", P, depth_first)
+    st, n, img = g.forward((0.2, 0.4, 0.6), capi.V_LITE | (capi.DEPTH_FIRST if depth_first else 0))
     capi.check(st)
     ref = oracle_frame(scene, spec, bg=(0.2, 0.4, 0.6), fp64=False)
     assert n == ref["pre"]["M"]
