@@ -125,6 +125,44 @@ int ps_sort_gaussian(int num_rendered, const int* d_num_rendered, int max_pairs,
                      const uint64_t* keys_unsorted, const uint32_t* values_unsorted,
                      uint64_t* keys_sorted, uint32_t* values_sorted, ps_stream_t stream);
 
+/* ---------------------------------------------------------------- depth first
+ * The depth-first ("ES", early sorting) key pipeline in two calls, for
+ * callers that time or drive the stages themselves (ps_forward runs the same
+ * kernels under PS_FORWARD_DEPTH_FIRST).
+ *
+ * ps_preprocess_depth_first replaces preprocess_es_pass_a[_half_sh]
+ * (preprocess.cu:673-1040), es_depth_sort_and_scan (sort.cu:217-253) and
+ * preprocess_es_pass_b (preprocess.cu:1042-1248); bound at
+ * pybind.cpp:206-304.  Count pass (features and records as ps_preprocess
+ * writes them), a stable sort of the Gaussians by depth, then every
+ * (tile id, Gaussian) pair in (depth, Gaussian id) order, each Gaussian's
+ * tiles row-major: tile ids u32[capacity] and values u32[capacity] from
+ * offset 0.  *curr_offset receives the pair count; on overflow the same
+ * rules as ps_preprocess hold.  workspace: ps_preprocess_workspace_size(P);
+ * sort_space: ps_get_sort_buffer_size(capacity) bytes (the depth sort's
+ * digit counts).
+ *
+ * ps_es_tile_sort replaces es_tile_sort (sort.cu:256-285); bound at
+ * pybind.cpp:306-327.  Stable sort of those pairs by tile id, then
+ * keys_sorted u64 = tile id << 32 (the low word zero, as the reference
+ * rebuilds them) and, when tile_ranges is not NULL, the tile ranges (empty
+ * tile -> (0, 0)).  The values come out in the canonical (tile, depth,
+ * Gaussian id) order.  The pair count is num_rendered, or *d_num_rendered
+ * (capped at max_pairs) when that is not NULL. */
+int ps_preprocess_depth_first(int P, const float* positions, const void* shs, int sh_prec,
+                              const float* opacities, const float* cov3d, const ps_camera* cam,
+                              float* points_xy, float* rgb_depth, float* conic_opacity,
+                              void* feat_records, int feat_prec, uint32_t* tile_ids,
+                              uint32_t* values, int capacity, int* curr_offset, void* workspace,
+                              size_t workspace_bytes, void* sort_space, size_t sort_space_bytes,
+                              ps_stream_t stream);
+int ps_es_tile_sort(int num_rendered, const int* d_num_rendered, int max_pairs, int width,
+                    int height, int block_x, int block_y, void* sort_space,
+                    size_t sort_space_bytes, const uint32_t* tile_ids_unsorted,
+                    const uint32_t* values_unsorted, uint32_t* tile_ids_sorted,
+                    uint32_t* values_sorted, uint64_t* keys_sorted, int* tile_ranges,
+                    ps_stream_t stream);
+
 /* Copies *d_curr_offset to the host through pinned memory and waits for it.
  * Replaces fetch_num_rendered_async (sort.cu:289-300, ops.h:103). */
 int ps_fetch_num_rendered(const int* d_curr_offset, int* out_num_rendered, ps_stream_t stream);

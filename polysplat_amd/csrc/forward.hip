@@ -28,7 +28,6 @@ int preprocess_es_launch(int P, const float* positions, const void* shs, int sh_
                          int capacity, int* curr_offset, void* workspace, size_t workspace_bytes,
                          void* sort_space, size_t sort_space_bytes, int sort_max_pairs,
                          hipStream_t stream);
-const uint32_t* preprocess_depth_words(int P, void* workspace);
 int render_plain_launch(int n_host, const int* d_n, int W, int H, const float* xy,
                         const float* rgbd, const float* con, const uint64_t* keys,
                         const uint32_t* order, const float bg[3], uint8_t* out,
@@ -274,6 +273,38 @@ extern "C" int ps_forward_query(int ticket) {
   return -PS_ERR_HIP;
 }
 
+extern "C" int ps_preprocess_depth_first(int P, const float* positions, const void* shs,
+                                         int sh_prec, const float* opacities, const float* cov3d,
+                                         const ps_camera* cam, float* points_xy, float* rgb_depth,
+                                         float* conic_opacity, void* feat_records, int feat_prec,
+                                         uint32_t* tile_ids, uint32_t* values, int capacity,
+                                         int* curr_offset, void* workspace,
+                                         size_t workspace_bytes, void* sort_space,
+                                         size_t sort_space_bytes, ps_stream_t stream) {
+  PS_REQUIRE(cam != nullptr, "NULL camera");
+  return ps::preprocess_es_launch(P, positions, shs, sh_prec, opacities, cov3d, cam, points_xy,
+                                  rgb_depth, conic_opacity, feat_records, feat_prec, tile_ids,
+                                  values, capacity, curr_offset, workspace, workspace_bytes,
+                                  sort_space, sort_space_bytes, capacity,
+                                  reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int ps_es_tile_sort(int num_rendered, const int* d_num_rendered, int max_pairs,
+                               int width, int height, int block_x, int block_y, void* sort_space,
+                               size_t sort_space_bytes, const uint32_t* tile_ids_unsorted,
+                               const uint32_t* values_unsorted, uint32_t* tile_ids_sorted,
+                               uint32_t* values_sorted, uint64_t* keys_sorted, int* tile_ranges,
+                               ps_stream_t stream) {
+  PS_REQUIRE(width > 0 && height > 0 && block_x > 0 && block_y > 0, "bad image/tile size");
+  PS_REQUIRE(tile_ids_unsorted && values_unsorted && tile_ids_sorted && values_sorted,
+             "NULL pair buffer");
+  const int tiles = ((width + block_x - 1) / block_x) * ((height + block_y - 1) / block_y);
+  return ps::tile_sort_launch(num_rendered, d_num_rendered, max_pairs, tiles, sort_space,
+                              sort_space_bytes, tile_ids_unsorted, values_unsorted,
+                              tile_ids_sorted, values_sorted, keys_sorted, tile_ranges,
+                              reinterpret_cast<hipStream_t>(stream));
+}
+
 extern "C" size_t ps_forward_workspace_size(int P) {
   return ps::kQueueBytes + ps::align_up(ps_preprocess_workspace_size(P), 256);
 }
@@ -322,8 +353,8 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
     // Depth-first pipeline: Gaussians sorted by depth, their (tile id,
     // Gaussian) pairs emitted in that order as 32-bit tile ids (first half
     // of keys_unsorted's memory), stably sorted by tile id (the sorted ids
-    // in its second half); tile ranges from that sort and the 64-bit sorted
-    // keys rebuilt from the tile ids and the depth words.
+    // in its second half); the 64-bit sorted keys (tile id << 32) and the
+    // tile ranges from that sort.
     PS_REQUIRE(fb->tile_ranges != nullptr || !need_ranges, "tile_ranges is NULL");
     uint32_t* tk = reinterpret_cast<uint32_t*>(fb->keys_unsorted);
     uint32_t* tk_sorted = tk + fb->capacity;
@@ -334,12 +365,9 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
                               ws, pre_bytes, fb->sort_space, fb->sort_space_bytes, fb->capacity,
                               stream);
     if (st != PS_OK) return st;
-    st = tile_sort_launch(d_n, fb->capacity, tiles, fb->sort_space, fb->sort_space_bytes, tk,
-                          fb->values_unsorted, tk_sorted, fb->values_sorted,
+    st = tile_sort_launch(0, d_n, fb->capacity, tiles, fb->sort_space, fb->sort_space_bytes, tk,
+                          fb->values_unsorted, tk_sorted, fb->values_sorted, fb->keys_sorted,
                           need_ranges ? fb->tile_ranges : nullptr, stream);
-    if (st != PS_OK) return st;
-    st = rebuild_keys_launch(d_n, fb->capacity, tk_sorted, fb->values_sorted,
-                             preprocess_depth_words(P, ws), fb->keys_sorted, stream);
     if (st != PS_OK) return st;
   } else {
     // The frame's keys start at offset 0 (the reference clears curr_offset

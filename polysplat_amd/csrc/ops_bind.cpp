@@ -181,6 +181,92 @@ void preprocess_records(Tensor& orig_points, Tensor& shs, Tensor& opacities, Ten
         "preprocess_records");
 }
 
+// ------------------------------------------------------------------ depth first
+// Depth-first preprocess: count pass, depth sort of the Gaussians and the
+// (tile id, Gaussian) pairs in depth order (one call for the reference's
+// preprocess_es_pass_a[_half_sh], es_depth_sort_and_scan and
+// preprocess_es_pass_b, pybind.cpp:206-304).  feat_prec 0 writes the FP32
+// feature arrays, 1/2 the FP16/BF16 records.  The tile ids start at 0;
+// curr_offset receives the pair count.
+void preprocess_es(Tensor& orig_points, Tensor& shs, Tensor& opacities, Tensor& cov3Ds, int width,
+                   int height, int block_x, int block_y, Tensor& position, Tensor& rotation,
+                   float focal_x, float focal_y, float zFar, float zNear, Tensor& points_xy,
+                   Tensor& rgb_depth, Tensor& conic_opacity, Tensor& records, int feat_prec,
+                   Tensor& tile_keys_unsorted, Tensor& values_unsorted, Tensor& curr_offset,
+                   Tensor& list_sorting_space) {
+  check_scene(orig_points, shs, opacities, cov3Ds);
+  const int P = (int)opacities.size(0);
+  need(feat_prec == PS_PREC_F32 || feat_prec == PS_PREC_F16 || feat_prec == PS_PREC_BF16,
+       "feat_prec must be 0 (FP32), 1 (FP16) or 2 (BF16)");
+  const bool f32 = feat_prec == PS_PREC_F32;
+  if (f32) {
+    need_dev(points_xy, "points_xy", at::kFloat);
+    need_dev(rgb_depth, "rgb_depth", at::kFloat);
+    need_dev(conic_opacity, "conic_opacity", at::kFloat);
+    need(points_xy.numel() >= 2LL * P && rgb_depth.numel() >= 4LL * P &&
+             conic_opacity.numel() >= 4LL * P,
+         "feature buffers must hold P entries");
+  } else {
+    need_dev(records, "records", at::kByte);
+    need(records.numel() >= 24LL * P, "records must hold 24 bytes per Gaussian");
+  }
+  need_dev(tile_keys_unsorted, "tile_keys_unsorted", at::kInt);
+  need_dev(values_unsorted, "gaussian_values_unsorted", at::kInt);
+  need_dev(curr_offset, "curr_offset", at::kInt);
+  need_dev(list_sorting_space, "list_sorting_space", at::kChar);
+  const int cap = (int)std::min<int64_t>(tile_keys_unsorted.numel(), values_unsorted.numel());
+  ps_camera cam = make_camera(width, height, block_x, block_y, position, rotation, focal_x,
+                              focal_y, zFar, zNear);
+  const size_t wsb = ps_preprocess_workspace_size(P);
+  Tensor ws = cached_workspace(opacities.device(), wsb, "preprocess");
+  check(ps_preprocess_depth_first(
+            P, ptr<float>(orig_points), shs.data_ptr(), sh_prec_of(shs), ptr<float>(opacities),
+            ptr<float>(cov3Ds), &cam, f32 ? ptr<float>(points_xy) : nullptr,
+            f32 ? ptr<float>(rgb_depth) : nullptr, f32 ? ptr<float>(conic_opacity) : nullptr,
+            f32 ? nullptr : records.data_ptr(), feat_prec, ptr<uint32_t>(tile_keys_unsorted),
+            ptr<uint32_t>(values_unsorted), cap, ptr<int>(curr_offset), ws.data_ptr(), wsb,
+            list_sorting_space.data_ptr(), (size_t)list_sorting_space.numel(), cur_stream()),
+        "preprocess_es");
+}
+
+// es_tile_sort(num_rendered, width, height, block_x, block_y, tile_keys_unsorted,
+// gauss_values_unsorted, tile_keys_sorted, gauss_values_sorted,
+// gaussian_keys_sorted_out, cub_sort_scratch) as bound at pybind.cpp:306-327;
+// tile_ranges (optional) also receives the tile ranges.
+void es_tile_sort(int num_rendered, int width, int height, int block_x, int block_y,
+                  Tensor& tile_keys_unsorted, Tensor& values_unsorted, Tensor& tile_keys_sorted,
+                  Tensor& values_sorted, Tensor& keys_sorted_out, Tensor& list_sorting_space,
+                  c10::optional<Tensor> tile_ranges) {
+  need_dev(tile_keys_unsorted, "tile_keys_unsorted", at::kInt);
+  need_dev(values_unsorted, "gauss_values_unsorted", at::kInt);
+  need_dev(tile_keys_sorted, "tile_keys_sorted", at::kInt);
+  need_dev(values_sorted, "gauss_values_sorted", at::kInt);
+  need_dev(keys_sorted_out, "gaussian_keys_sorted_out", at::kLong);
+  need_dev(list_sorting_space, "list_sorting_space", at::kChar);
+  const int cap = (int)std::min({tile_keys_unsorted.numel(), values_unsorted.numel(),
+                                 tile_keys_sorted.numel(), values_sorted.numel(),
+                                 keys_sorted_out.numel()});
+  int max_pairs = cap;
+  const size_t have = (size_t)list_sorting_space.numel();
+  while (max_pairs > 0 && ps_get_sort_buffer_size(max_pairs) > have) max_pairs /= 2;
+  need(num_rendered >= 0 && num_rendered <= max_pairs,
+       "list_sorting_space or the pair buffers too small for num_rendered");
+  int* rng = nullptr;
+  if (tile_ranges.has_value()) {
+    need_dev(*tile_ranges, "tile_ranges", at::kInt);
+    const int64_t tiles = (int64_t)((width + block_x - 1) / block_x) *
+                          ((height + block_y - 1) / block_y);
+    need(tile_ranges->numel() >= 2 * tiles, "tile_ranges must hold [tiles, 2]");
+    rng = ptr<int>(*tile_ranges);
+  }
+  check(ps_es_tile_sort(num_rendered, nullptr, max_pairs, width, height, block_x, block_y,
+                        list_sorting_space.data_ptr(), have, ptr<uint32_t>(tile_keys_unsorted),
+                        ptr<uint32_t>(values_unsorted), ptr<uint32_t>(tile_keys_sorted),
+                        ptr<uint32_t>(values_sorted), ptr<uint64_t>(keys_sorted_out), rng,
+                        cur_stream()),
+        "es_tile_sort");
+}
+
 // ------------------------------------------------------------------ sort
 size_t get_sort_buffer_size(int64_t num_rendered) {
   need(num_rendered >= 0 && num_rendered < (1LL << 31), "num_rendered out of range");
@@ -635,6 +721,13 @@ PYBIND11_MODULE(_C, m) {
   ops.def("preprocess", &preprocess_impl);
   ops.def("preprocess_half_sh", &preprocess_impl);
   ops.def("preprocess_records", &preprocess_records);
+  ops.def("preprocess_es", &preprocess_es);
+  ops.def("es_tile_sort", &es_tile_sort, py::arg("num_rendered"), py::arg("width"),
+          py::arg("height"), py::arg("block_x"), py::arg("block_y"),
+          py::arg("tile_keys_unsorted"), py::arg("gauss_values_unsorted"),
+          py::arg("tile_keys_sorted"), py::arg("gauss_values_sorted"),
+          py::arg("gaussian_keys_sorted_out"), py::arg("cub_sort_scratch"),
+          py::arg("tile_ranges") = py::none());
   ops.def("get_sort_buffer_size", &get_sort_buffer_size);
   ops.def("sort_gaussian", &sort_gaussian);
   ops.def("gather_features", &gather_features);

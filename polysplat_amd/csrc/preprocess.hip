@@ -1139,14 +1139,6 @@ int preprocess_es_launch(int P, const float* positions, const void* shs, int sh_
   return PS_OK;
 }
 
-// Depth word per Gaussian the depth-first preprocess leaves in its workspace
-// (0 for a Gaussian that emits nothing), for rebuilding 64-bit keys.
-const uint32_t* preprocess_depth_words(int P, void* workspace) {
-  PreWs w;
-  pre_layout(P, static_cast<char*>(workspace), &w);
-  return w.dkey;
-}
-
 // Pinned staging ring for batch descriptors, one per host thread.  A copy
 // from pageable memory would block the host until the stream reaches it;
 // from a pinned slot it is queued like a kernel.  A slot is rewritten only

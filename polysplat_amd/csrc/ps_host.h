@@ -130,17 +130,18 @@ int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles
 // their (tile id, Gaussian) pairs (sort.hip).
 int depth_sort_launch(const DepthSortArgs& A, int max_pairs, void* space, size_t space_bytes,
                       hipStream_t stream);
-int tile_sort_launch(const int* d_n, int max_pairs, int tiles, void* space, size_t space_bytes,
-                     const uint32_t* tkeys_in, const uint32_t* vals_in, uint32_t* tkeys_out,
-                     uint32_t* vals_out, int* tile_ranges, hipStream_t stream);
-// Boundary-scan tile ranges over 32-bit tile ids (depth-first pipeline).
-int tile_ranges_scan32_launch(const int* d_n, int max_pairs, int tiles, const uint32_t* tkeys,
-                              int* ranges, hipStream_t stream);
-// keys[i] = tile id << 32 | depth word of the Gaussian values[i] (sorted
-// 64-bit keys rebuilt after the depth-first pipeline's 32-bit tile sort).
-int rebuild_keys_launch(const int* d_n, int max_pairs, const uint32_t* tkeys,
-                        const uint32_t* values, const uint32_t* depth_words, uint64_t* keys,
-                        hipStream_t stream);
+// Stable sort of the depth-first pipeline's (tile id, value) pairs by tile
+// id, then the 64-bit keys (tile id << 32) and, when `tile_ranges` is given,
+// the tile ranges.  The pair count is n_host, or *d_n when d_n is given.
+int tile_sort_launch(int n_host, const int* d_n, int max_pairs, int tiles, void* space,
+                     size_t space_bytes, const uint32_t* tkeys_in, const uint32_t* vals_in,
+                     uint32_t* tkeys_out, uint32_t* vals_out, uint64_t* keys_out,
+                     int* tile_ranges, hipStream_t stream);
+// keys[i] = tkeys[i] << 32 and, when `ranges` is given, the boundary-scan
+// tile ranges of the sorted tile ids (empty tiles (0, 0)).
+int tile_ids_finish_launch(int n_host, const int* d_n, int max_pairs, int tiles,
+                           const uint32_t* tkeys, uint64_t* keys, int* ranges,
+                           hipStream_t stream);
 // Stable radix sort of (key, value) pairs over key bits [0, end_bit).  When
 // `tile_ranges` is given, the keys' high words are tile ids below `tiles`
 // and the sort also leaves each tile's [start, end) there.

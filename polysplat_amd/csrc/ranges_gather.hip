@@ -82,19 +82,27 @@ __global__ __launch_bounds__(kThreads) void ps_ranges_scan(const uint64_t* __res
   }
 }
 
-// The same over 32-bit tile ids (their ids need no masking).
-__global__ __launch_bounds__(kThreads) void ps_ranges_scan32(const uint32_t* __restrict__ keys,
-                                                             const int* __restrict__ d_n,
-                                                             int max_pairs, int tiles,
-                                                             int2* __restrict__ ranges) {
-  const int n = count_of(d_n, 0, max_pairs);
+// Sorted 32-bit tile ids to the 64-bit keys the renderers read (tile id in
+// the high word, low word zero, as the reference's depth-first pipeline
+// rebuilds them: sort.cu:205-215) and, when `ranges` is given, the tile
+// ranges by the same boundary scan as ps_ranges_scan (tile ids need no
+// masking).
+__global__ __launch_bounds__(kThreads) void ps_tile_ids_finish(const uint32_t* __restrict__ tk,
+                                                               const int* __restrict__ d_n,
+                                                               int n_host, int max_pairs,
+                                                               int tiles,
+                                                               uint64_t* __restrict__ keys,
+                                                               int2* __restrict__ ranges) {
+  const int n = count_of(d_n, n_host, max_pairs);
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-    const uint32_t cur = keys[i];
+    const uint32_t cur = tk[i];
+    keys[i] = (uint64_t)cur << 32;
+    if (ranges == nullptr) continue;
     const bool cur_ok = cur < (uint32_t)tiles;
     if (i == 0) {
       if (cur_ok) ranges[cur].x = 0;
     } else {
-      const uint32_t prev = keys[i - 1];
+      const uint32_t prev = tk[i - 1];
       if (cur != prev) {
         if (prev < (uint32_t)tiles) ranges[prev].y = i;
         if (cur_ok) ranges[cur].x = i;
@@ -102,17 +110,6 @@ __global__ __launch_bounds__(kThreads) void ps_ranges_scan32(const uint32_t* __r
     }
     if (i == n - 1 && cur_ok) ranges[cur].y = n;
   }
-}
-
-__global__ __launch_bounds__(kThreads) void ps_rebuild_keys(const int* __restrict__ d_n,
-                                                            int max_pairs,
-                                                            const uint32_t* __restrict__ tkeys,
-                                                            const uint32_t* __restrict__ values,
-                                                            const uint32_t* __restrict__ words,
-                                                            uint64_t* __restrict__ keys) {
-  const int n = count_of(d_n, 0, max_pairs);
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
-    keys[i] = ((uint64_t)tkeys[i] << 32) | (uint64_t)words[values[i]];
 }
 
 __global__ __launch_bounds__(kThreads) void ps_gather_f32(
@@ -163,26 +160,20 @@ int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles
   return PS_OK;
 }
 
-int tile_ranges_scan32_launch(const int* d_n, int max_pairs, int tiles, const uint32_t* tkeys,
-                              int* ranges, hipStream_t stream) {
-  hipLaunchKernelGGL(ps_fill_int2, dim3(grid_for(tiles)), dim3(kThreads), 0, stream,
-                     reinterpret_cast<int2*>(ranges), tiles, (int*)nullptr, 0);
-  PS_LAUNCH_CHECK();
-  if (max_pairs > 0) {
-    hipLaunchKernelGGL(ps_ranges_scan32, dim3(grid_for(max_pairs)), dim3(kThreads), 0, stream,
-                       tkeys, d_n, max_pairs, tiles, reinterpret_cast<int2*>(ranges));
+int tile_ids_finish_launch(int n_host, const int* d_n, int max_pairs, int tiles,
+                           const uint32_t* tkeys, uint64_t* keys, int* ranges,
+                           hipStream_t stream) {
+  if (ranges != nullptr) {
+    hipLaunchKernelGGL(ps_fill_int2, dim3(grid_for(tiles)), dim3(kThreads), 0, stream,
+                       reinterpret_cast<int2*>(ranges), tiles, (int*)nullptr, 0);
     PS_LAUNCH_CHECK();
   }
-  return PS_OK;
-}
-
-int rebuild_keys_launch(const int* d_n, int max_pairs, const uint32_t* tkeys,
-                        const uint32_t* values, const uint32_t* depth_words, uint64_t* keys,
-                        hipStream_t stream) {
-  if (max_pairs <= 0) return PS_OK;
-  hipLaunchKernelGGL(ps_rebuild_keys, dim3(grid_for(max_pairs)), dim3(kThreads), 0, stream, d_n,
-                     max_pairs, tkeys, values, depth_words, keys);
-  PS_LAUNCH_CHECK();
+  const int n = d_n ? max_pairs : n_host;
+  if (n > 0) {
+    hipLaunchKernelGGL(ps_tile_ids_finish, dim3(grid_for(n)), dim3(kThreads), 0, stream, tkeys,
+                       d_n, n_host, max_pairs, tiles, keys, reinterpret_cast<int2*>(ranges));
+    PS_LAUNCH_CHECK();
+  }
   return PS_OK;
 }
 

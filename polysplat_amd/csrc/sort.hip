@@ -1168,21 +1168,26 @@ int depth_sort_launch(const DepthSortArgs& A, int max_pairs, void* space, size_t
 
 // Tile sort of the depth-first pipeline: stable sort of the (tile id, value)
 // pairs, emitted in depth order, by their 32-bit tile ids (bits
-// [0, bit_length(tiles))), then the tile ranges by a boundary scan.  The
-// pair count comes from device memory.
-int tile_sort_launch(const int* d_n, int max_pairs, int tiles, void* space, size_t space_bytes,
-                     const uint32_t* tkeys_in, const uint32_t* vals_in, uint32_t* tkeys_out,
-                     uint32_t* vals_out, int* tile_ranges, hipStream_t stream) {
+// [0, bit_length(tiles))), then the 64-bit keys and the tile ranges.
+int tile_sort_launch(int n_host, const int* d_n, int max_pairs, int tiles, void* space,
+                     size_t space_bytes, const uint32_t* tkeys_in, const uint32_t* vals_in,
+                     uint32_t* tkeys_out, uint32_t* vals_out, uint64_t* keys_out,
+                     int* tile_ranges, hipStream_t stream) {
+  PS_REQUIRE(max_pairs >= 0, "max_pairs must be >= 0");
+  PS_REQUIRE(d_n != nullptr || (n_host >= 0 && n_host <= max_pairs),
+             "num_rendered %d outside [0, %d]", n_host, max_pairs);
   PS_REQUIRE(space != nullptr && space_bytes >= sort_layout(max_pairs, nullptr, nullptr),
              "sort space too small");
   PS_REQUIRE(tiles > 0, "tile ranges need a tile count");
   PS_REQUIRE(tkeys_in != tkeys_out && vals_in != vals_out, "sort outputs must not alias inputs");
+  PS_REQUIRE(keys_out != nullptr, "keys_sorted is NULL");
   SortWs w;
   sort_layout(max_pairs, static_cast<char*>(space), &w);
-  const int st = digit_passes<uint32_t>(0, d_n, max_pairs, 0, bit_length((unsigned)tiles), w,
-                                        tkeys_in, vals_in, tkeys_out, vals_out, stream);
-  if (st != PS_OK || tile_ranges == nullptr) return st;
-  return tile_ranges_scan32_launch(d_n, max_pairs, tiles, tkeys_out, tile_ranges, stream);
+  const int st = digit_passes<uint32_t>(n_host, d_n, max_pairs, 0, bit_length((unsigned)tiles),
+                                        w, tkeys_in, vals_in, tkeys_out, vals_out, stream);
+  if (st != PS_OK) return st;
+  return tile_ids_finish_launch(n_host, d_n, max_pairs, tiles, tkeys_out, keys_out, tile_ranges,
+                                stream);
 }
 
 int sort_end_bit(int width, int height, int block_x, int block_y) {

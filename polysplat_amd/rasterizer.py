@@ -23,10 +23,23 @@ variants store per-Gaussian render features: FP32 in the reference's three
 arrays, or 24-byte records with FP16/BF16 conic and colour (projection,
 opacity and blending stay FP32).  The "default" variant always uses FP32.
 
-Other variants of the reference (top-k caches, tile reordering, split warps,
-the ES two-pass emission) are not part of this build and raise ValueError.
+``use_es`` selects the depth-first ("ES", early sorting) key pipeline of
+example.py:246-291 for every variant here, fused or staged: the Gaussians
+sorted by depth, their pairs emitted in that order as 32-bit tile ids and
+stably sorted by tile id.  Images and sorted values are identical to the
+default pipeline's; the sorted keys carry the tile id only (low word zero),
+as the reference rebuilds them.  "auto" takes it when the previous frame's
+pair count M is large in absolute terms and per Gaussian (M > T_M and
+M > rho * P, thresholds measured on MI355X; environment variables
+POLYSPLAT_ES_THRESHOLD_M and POLYSPLAT_ES_RATIO_THRESH override them,
+POLYSPLAT_ES_FORCE_TWO_PASS=1 forces it), never on a first frame.
+
+Other variants of the reference (top-k caches, tile reordering, split warps)
+are not part of this build and raise ValueError.
 """
 from __future__ import annotations
+
+import os
 
 import torch
 
@@ -42,6 +55,19 @@ VARIANT_IDS = {
 }
 STAGED = ("preranges_smem_persistent_lite", "preranges_smem_persistent_lite_dt")
 AUTO_CHOICE = "preranges_smem_persistent_lite_e2e_preranges"
+DEPTH_FIRST = 0x100  # PS_FORWARD_DEPTH_FIRST: OR-ed into a ps_forward variant
+
+# Depth-first pipeline selection ("auto").  On MI355X its extra per-Gaussian
+# work (depth sort, counts and records gathered in depth order) costs more
+# than the per-tile depth sort it removes unless frames hold many pairs per
+# Gaussian: on 3840x2160 frames it is 23% slower at 1.3 pairs per Gaussian,
+# 2% slower at 6.9 and level (within 2%) from 10 to 50.
+ES_THRESHOLD_M = 2_000_000
+ES_RATIO_THRESH = 10.0
+
+
+def _env_flag(name):
+    return os.environ.get(name, "0") not in ("0", "", "false", "False")
 
 
 class Scene:
@@ -115,6 +141,11 @@ class Rasterizer:
         self.MAX_NUM_TILES = int(MAX_NUM_TILES)
         self.feature_dtype = feature_dtype
         self.enable_es = enable_es
+        self._es_last_M = None  # previous frame's pair count, for "auto"
+        self._es_threshold_M = int(os.environ.get("POLYSPLAT_ES_THRESHOLD_M", ES_THRESHOLD_M))
+        self._es_ratio_thresh = float(os.environ.get("POLYSPLAT_ES_RATIO_THRESH",
+                                                     ES_RATIO_THRESH))
+        self._es_force_two_pass = _env_flag("POLYSPLAT_ES_FORCE_TWO_PASS")
         M = self.MAX_NUM_RENDERED
         P = scene.num_vertex
         self.gaussian_keys_unsorted = torch.zeros(M, device=dev, dtype=torch.int64)
@@ -135,6 +166,11 @@ class Rasterizer:
         self.workspace = torch.zeros(ops.forward_workspace_size(P), device=dev, dtype=torch.uint8)
         self._empty = torch.zeros(0, device=dev, dtype=torch.uint8)
         self._out_color = None
+        # The depth-first pipeline's 32-bit tile ids live in the unsorted
+        # keys' memory: unsorted ids in its first half, sorted in its second.
+        tk = self.gaussian_keys_unsorted.view(torch.int32)
+        self._es_tile_keys_unsorted = tk[:M]
+        self._es_tile_keys_sorted = tk[M:]
 
     def _get_out_color(self, height, width, device):
         if (self._out_color is None or self._out_color.shape[0] != height
@@ -152,14 +188,28 @@ class Rasterizer:
             return scene.shs
         return scene.shs_bf16 if self.feature_dtype == "bf16" else scene.shs_half
 
+    def _decide_use_es(self, scene, use_es):
+        """Depth-first pipeline or not (example.py:246-291, with this
+        build's own cost model: see the module docstring)."""
+        if use_es not in (True, False, "auto"):
+            raise ValueError(f"use_es must be True, False or 'auto' (got {use_es!r})")
+        if not self.enable_es:
+            return False
+        if use_es is True or use_es is False:
+            return use_es
+        if self._es_force_two_pass:
+            return True
+        if self._es_last_M is None:
+            return False
+        M = self._es_last_M
+        return M > self._es_threshold_M and M > self._es_ratio_thresh * scene.num_vertex
+
     def forward(self, scene, camera, bg_color, block_x=16, block_y=16, render_variant="default",
                 measure_preprocess=False, measure_sort=False, measure_render=False,
                 return_stats=False, use_es="auto"):
         requested = render_variant
         if render_variant == "auto":
             render_variant = AUTO_CHOICE
-        if use_es is True:
-            raise ValueError("the ES two-pass emission is not part of this build")
         if block_x != 16 or block_y != 16:
             raise ValueError("only 16x16 tiles are implemented")
         measuring = measure_preprocess or measure_sort or measure_render
@@ -169,14 +219,17 @@ class Rasterizer:
         if render_variant not in VARIANT_IDS and render_variant not in STAGED:
             raise ValueError(f"render_variant {render_variant!r} is not implemented")
         self._check_tiles(camera, block_x, block_y)
+        es = self._decide_use_es(scene, use_es)
         out_color = self._get_out_color(camera.height, camera.width, scene.device)
         times = {"preprocess_ms": None, "sort_ms": None, "render_ms": None}
         if render_variant in VARIANT_IDS and not (render_variant == "default" and measuring):
-            num_rendered = self._forward_fused(scene, camera, bg_color, render_variant, out_color)
+            num_rendered = self._forward_fused(scene, camera, bg_color, render_variant, out_color,
+                                               es)
         else:
             num_rendered = self._forward_staged(scene, camera, bg_color, render_variant,
                                                 out_color, measure_preprocess, measure_sort,
-                                                measure_render, times)
+                                                measure_render, times, es)
+        self._es_last_M = num_rendered
         if return_stats:
             return out_color, {
                 "requested_render_variant": requested,
@@ -187,7 +240,7 @@ class Rasterizer:
                 "preprocess_ms": times["preprocess_ms"],
                 "sort_ms": times["sort_ms"],
                 "render_ms": times["render_ms"],
-                "used_es": False,
+                "used_es": es,
             }
         sel = [times[k] for k, f in (("preprocess_ms", measure_preprocess),
                                      ("sort_ms", measure_sort),
@@ -196,7 +249,7 @@ class Rasterizer:
             return (out_color, *sel)
         return out_color
 
-    def _forward_fused(self, scene, camera, bg_color, render_variant, out_color):
+    def _forward_fused(self, scene, camera, bg_color, render_variant, out_color, es=False):
         variant = VARIANT_IDS[render_variant]
         fp32_path = render_variant == "default"
         prec = 0 if fp32_path else _PREC[self.feature_dtype]
@@ -207,14 +260,14 @@ class Rasterizer:
             scene.position, shs, scene.opacity, scene.cov3d,
             camera.width, camera.height, camera.position, camera.rotation,
             camera.focal_x, camera.focal_y, camera.zFar, camera.zNear,
-            variant, prec, self.curr_offset, self.list_sorting_space,
+            variant | (DEPTH_FIRST if es else 0), prec, self.curr_offset, self.list_sorting_space,
             self.gaussian_keys_unsorted, self.gaussian_values_unsorted,
             self.gaussian_keys_sorted, self.gaussian_values_sorted,
             self.points_xy, self.rgb_depth, self.conic_opacity, records, ranges,
             bg_color, out_color, self.workspace, True)
 
     def _forward_staged(self, scene, camera, bg_color, render_variant, out_color,
-                        m_pre, m_sort, m_render, times):
+                        m_pre, m_sort, m_render, times, es=False):
         ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
         fp32_path = render_variant == "default"
         prec = 0 if fp32_path else _PREC[self.feature_dtype]
@@ -225,7 +278,14 @@ class Rasterizer:
         e = [ev(), ev(), ev(), ev(), ev(), ev()]
         if m_pre:
             e[0].record()
-        if prec == 0:
+        if es:
+            # count pass, depth sort and pairs in depth order
+            # (example.py:494-522 and the first half of 550-576)
+            ops.preprocess_es(scene.position, shs, scene.opacity, scene.cov3d, *args_cam,
+                              self.points_xy, self.rgb_depth, self.conic_opacity, self.records,
+                              prec, self._es_tile_keys_unsorted, self.gaussian_values_unsorted,
+                              self.curr_offset, self.list_sorting_space)
+        elif prec == 0:
             ops.preprocess(scene.position, shs, scene.opacity, scene.cov3d, *args_cam,
                            self.points_xy, self.rgb_depth, self.conic_opacity,
                            self.gaussian_keys_unsorted, self.gaussian_values_unsorted,
@@ -241,10 +301,16 @@ class Rasterizer:
             raise RuntimeError("Too many k-v pairs!")
         if m_sort:
             e[2].record()
-        ops.sort_gaussian(num_rendered, camera.width, camera.height, 16, 16,
-                          self.list_sorting_space, self.gaussian_keys_unsorted,
-                          self.gaussian_values_unsorted, self.gaussian_keys_sorted,
-                          self.gaussian_values_sorted)
+        if es:
+            ops.es_tile_sort(num_rendered, camera.width, camera.height, 16, 16,
+                             self._es_tile_keys_unsorted, self.gaussian_values_unsorted,
+                             self._es_tile_keys_sorted, self.gaussian_values_sorted,
+                             self.gaussian_keys_sorted, self.list_sorting_space)
+        else:
+            ops.sort_gaussian(num_rendered, camera.width, camera.height, 16, 16,
+                              self.list_sorting_space, self.gaussian_keys_unsorted,
+                              self.gaussian_values_unsorted, self.gaussian_keys_sorted,
+                              self.gaussian_values_sorted)
         if m_sort:
             e[3].record()
         if not fp32_path:

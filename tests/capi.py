@@ -78,6 +78,9 @@ def bind(L):
         "ps_preprocess": ([I, P, P, I, P, P, P, P, P, P, P, I, P, P, I, P, P, S, P], I),
         "ps_preprocess_count_ptr": ([P], P),
         "ps_preprocess_status_ptr": ([P], P),
+        "ps_preprocess_depth_first": ([I, P, P, I, P, P, P, P, P, P, P, I, P, P, I, P, P, S, P,
+                                       S, P], I),
+        "ps_es_tile_sort": ([I, P, I, I, I, I, I, P, S, P, P, P, P, P, P, P], I),
         "ps_get_sort_buffer_size": ([I], S),
         "ps_sort_gaussian": ([I, P, I, I, I, I, I, P, S, P, P, P, P, P], I),
         "ps_fetch_num_rendered": ([P, P, P], I),

@@ -398,6 +398,7 @@ def test_posted_count_readback(emu):
 
 # ------------------------------------------------------------------ depth-first pipeline
 DEPTH_FIRST = 0x100
+TILE_WORD = np.uint64(0xFFFFFFFF00000000)
 
 
 def _flat_scene(P=600, seed=47):
@@ -441,7 +442,8 @@ def test_depth_first_pipeline_equals_default(emu, case, variant):
     st, n2, img_es = es.forward(bg, variant | DEPTH_FIRST)
     capi.check_with(emu, st)
     assert n == n2 == M
-    np.testing.assert_array_equal(es.ks[:M], ref["keys_sorted"])
+    # sorted keys carry the tile id only (example.py:560-576, sort.cu:205-215)
+    np.testing.assert_array_equal(es.ks[:M], ref["keys_sorted"] & TILE_WORD)
     np.testing.assert_array_equal(es.vs[:M], ref["vals_sorted"])
     np.testing.assert_array_equal(img_es, img_std)
     if variant == capi.V_LITE:
@@ -457,3 +459,43 @@ def test_depth_first_overflow_is_reported(emu):
     st, n, img = f.forward((0.5, 0.5, 0.5), capi.V_LITE | DEPTH_FIRST)
     assert st == capi.PS_ERR_OVERFLOW and n == M
     assert (img == 127).all()  # nothing sorted: every tile shows the background
+
+
+def test_depth_first_staged_entry_points(emu):
+    """ps_preprocess_depth_first then ps_es_tile_sort with the count from the
+    host: the pairs in depth order, their stable tile sort, the keys' tile
+    words and the ranges against the oracle."""
+    sc, spec = _scene(), scenes.make_camera(200, 120, 10.0, distance=12.0)
+    ref = oracle_frame(sc, spec, sh_type="fp32", feat_prec="fp32", fp64=False)
+    M = ref["pre"]["M"]
+    f = HF.HostFrame(emu, sc, spec, sh_type="fp32", feat_prec="fp32", capacity=M + 40,
+                     garbage=True, seed=3)
+    L = emu
+    ws_bytes = L.ps_preprocess_workspace_size(f.P)
+    ws = np.random.default_rng(4).integers(0, 256, ws_bytes, dtype=np.uint8)
+    tk = np.zeros(2 * f.cap, np.uint32)
+    st = L.ps_preprocess_depth_first(f.P, HF.ptr(f.pos), HF.ptr(f.shs), 0, HF.ptr(f.opac),
+                                     HF.ptr(f.cov), capi.C.byref(f.cam), HF.ptr(f.xy),
+                                     HF.ptr(f.rgbd), HF.ptr(f.conic), None, 0, HF.ptr(tk),
+                                     HF.ptr(f.vu), f.cap, HF.ptr(f.curr), HF.ptr(ws), ws_bytes,
+                                     HF.ptr(f.sort_space), f.sort_bytes, None)
+    capi.check_with(L, st)
+    n = int(f.curr[0])
+    assert n == M
+    # pairs in (depth, Gaussian id) order: ordering the oracle's canonical
+    # pairs by (depth, id) and keeping each Gaussian's tiles in row-major order
+    keys, vals = ref["pre"]["keys"], ref["pre"]["vals"]
+    order = np.lexsort((np.arange(M), vals, keys & np.uint64(0xFFFFFFFF)))
+    np.testing.assert_array_equal(tk[:M], (keys[order] >> np.uint64(32)).astype(np.uint32))
+    np.testing.assert_array_equal(f.vu[:M], vals[order])
+    st = L.ps_es_tile_sort(n, None, f.cap, spec.width, spec.height, 16, 16,
+                           HF.ptr(f.sort_space), f.sort_bytes, HF.ptr(tk), HF.ptr(f.vu),
+                           HF.ptr(tk[f.cap:]), HF.ptr(f.vs), HF.ptr(f.ks), HF.ptr(f.ranges), None)
+    capi.check_with(L, st)
+    np.testing.assert_array_equal(f.ks[:M], ref["keys_sorted"] & TILE_WORD)
+    np.testing.assert_array_equal(f.vs[:M], ref["vals_sorted"])
+    np.testing.assert_array_equal(f.ranges, O.tile_ranges(ref["keys_sorted"], f.tiles, 1))
+    # aliasing outputs and inputs is refused
+    assert L.ps_es_tile_sort(n, None, f.cap, spec.width, spec.height, 16, 16,
+                             HF.ptr(f.sort_space), f.sort_bytes, HF.ptr(tk), HF.ptr(f.vu),
+                             HF.ptr(tk), HF.ptr(f.vs), HF.ptr(f.ks), None, None) == capi.PS_ERR_INVALID

@@ -154,6 +154,55 @@ def test_sort_ranges_ops_bit_exact(case):
     np.testing.assert_array_equal(b.ku[:M].cpu().numpy().view(np.uint64), ref["pre"]["keys"])
 
 
+TILE_WORD = np.uint64(0xFFFFFFFF00000000)
+
+
+@pytest.mark.parametrize("feat", ["fp32", "fp16"])
+def test_depth_first_ops_bit_exact(case, feat):
+    """ops.preprocess_es (count pass, depth sort, pairs in depth order) and
+    ops.es_tile_sort with the reference's es_tile_sort signature
+    (pybind.cpp:306-327): sorted values and ranges as the oracle's, keys
+    holding the tile word only (sort.cu:205-215), features as preprocess
+    writes them."""
+    arrays, spec, scene, ref = case
+    b = _bufs(case, records=True)
+    M0 = ref["pre"]["M"]
+    cap = b.ku.numel()
+    tk = b.ku.view(torch.int32)
+    prec = 0 if feat == "fp32" else 1
+    b.curr.fill_(12345)  # overwritten, not appended to
+    ops.preprocess_es(scene.position, scene.shs if prec == 0 else scene.shs_half, scene.opacity,
+                      scene.cov3d, *_cam(spec), b.xy, b.rgbd, b.con, b.rec, prec, tk[:cap], b.vu,
+                      b.curr, b.space)
+    torch.cuda.synchronize()
+    M = int(b.curr.item())
+    assert M == M0
+    ops.es_tile_sort(M, spec.width, spec.height, 16, 16, tk[:cap], b.vu, tk[cap:], b.vs, b.ks,
+                     b.space, tile_ranges=b.ranges)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(b.ks[:M].cpu().numpy().view(np.uint64),
+                                  ref["keys_sorted"] & TILE_WORD)
+    np.testing.assert_array_equal(b.vs[:M].cpu().numpy().view(np.uint32), ref["vals_sorted"])
+    import oracle as O
+    np.testing.assert_array_equal(b.ranges.cpu().numpy(),
+                                  O.tile_ranges(ref["keys_sorted"], b.ranges.shape[0], 1))
+    if prec == 0:
+        ids = np.unique(ref["pre"]["vals"])
+        for t, k in ((b.xy, "xy"), (b.rgbd, "rgbd"), (b.con, "conic")):
+            np.testing.assert_array_equal(t.cpu().numpy()[ids].view(np.uint32),
+                                          ref["pre"][k][ids].view(np.uint32), err_msg=k)
+    # the reference signature without ranges, and its bounds
+    b.ks.zero_()
+    ops.es_tile_sort(M, spec.width, spec.height, 16, 16, tk[:cap], b.vu, tk[cap:], b.vs, b.ks,
+                     b.space)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(b.ks[:M].cpu().numpy().view(np.uint64),
+                                  ref["keys_sorted"] & TILE_WORD)
+    with pytest.raises(Exception):
+        ops.es_tile_sort(cap + 1, spec.width, spec.height, 16, 16, tk[:cap], b.vu, tk[cap:],
+                         b.vs, b.ks, b.space)
+
+
 def test_gather_ops_and_gathered_render(case):
     arrays, spec, scene, ref = case
     b = _bufs(case, records=True)
@@ -334,6 +383,47 @@ def test_rasterizer_forward(case, variant, feature_dtype):
     _, s2 = rast.forward(scene, spec, bg, render_variant=variant, measure_render=True,
                          return_stats=True)
     assert s2["render_ms"] > 0 and s2["preprocess_ms"] is None
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_rasterizer_depth_first(case, variant):
+    """use_es=True (example.py:246-291, 494-576) gives the same images as the
+    default pipeline in every variant and return form; "auto" follows the
+    previous frame's pair count; enable_es=False never takes it."""
+    arrays, spec, scene, ref = case
+    M = ref["pre"]["M"]
+    T = ((spec.width + 15) // 16) * ((spec.height + 15) // 16)
+    rast = polysplat.Rasterizer(scene, MAX_NUM_RENDERED=M + 4096, MAX_NUM_TILES=T,
+                                feature_dtype="fp16")
+    bg = _bg()
+    base = rast.forward(scene, spec, bg, render_variant=variant, use_es=False).cpu().numpy().copy()
+    img, stats = rast.forward(scene, spec, bg, render_variant=variant, use_es=True,
+                              return_stats=True)
+    assert stats["used_es"] is True and stats["num_rendered"] == M
+    np.testing.assert_array_equal(img.cpu().numpy(), base)
+    np.testing.assert_array_equal(rast.gaussian_values_sorted[:M].cpu().numpy().view(np.uint32),
+                                  ref["vals_sorted"])
+    res = rast.forward(scene, spec, bg, render_variant=variant, use_es=True,
+                       measure_preprocess=True, measure_sort=True, measure_render=True)
+    assert len(res) == 4 and all(isinstance(x, float) and x > 0 for x in res[1:])
+    timed_base = rast.forward(scene, spec, bg, render_variant=variant, use_es=False,
+                              measure_render=True)[0].cpu().numpy().copy()
+    np.testing.assert_array_equal(res[0].cpu().numpy(), timed_base)
+    # "auto": never on a first frame, then by the last pair count
+    rast2 = polysplat.Rasterizer(scene, MAX_NUM_RENDERED=M + 4096, MAX_NUM_TILES=T,
+                                 feature_dtype="fp16")
+    rast2._es_threshold_M, rast2._es_ratio_thresh = 0, 0.0
+    _, s1 = rast2.forward(scene, spec, bg, render_variant=variant, return_stats=True)
+    _, s2 = rast2.forward(scene, spec, bg, render_variant=variant, return_stats=True)
+    assert s1["used_es"] is False and s2["used_es"] is True
+    rast2._es_ratio_thresh = 1e9
+    _, s3 = rast2.forward(scene, spec, bg, render_variant=variant, return_stats=True)
+    assert s3["used_es"] is False
+    rast3 = polysplat.Rasterizer(scene, MAX_NUM_RENDERED=M + 4096, MAX_NUM_TILES=T,
+                                 enable_es=False, feature_dtype="fp16")
+    _, s4 = rast3.forward(scene, spec, bg, render_variant=variant, use_es=True,
+                          return_stats=True)
+    assert s4["used_es"] is False
 
 
 def test_rasterizer_auto_and_unsupported(case):

@@ -19,6 +19,7 @@ from tests.gpu_helpers import Gpu, oracle_frame, say
 pytestmark = pytest.mark.gpu
 
 BG = (0.0, 0.0, 0.0)
+TILE_WORD = np.uint64(0xFFFFFFFF00000000)  # sorted keys of the depth-first pipeline
 
 
 def _mixed_scene(P=60_000, seed=3):
@@ -166,9 +167,10 @@ def test_forward_pipeline_matches_stages(case):
 @pytest.mark.parametrize("variant", [capi.V_DEFAULT, capi.V_LITE, capi.V_FUSED])
 def test_depth_first_pipeline(case, variant):
     """Depth-first key pipeline (Gaussians sorted by depth, pairs emitted in
-    depth order as 32-bit tile ids, stable tile sort, rebuilt keys): the same
-    sorted keys, values and ranges as the oracle, and the same image bytes as
-    the default pipeline of the variant."""
+    depth order as 32-bit tile ids, stable tile sort, rebuilt keys): the
+    oracle's sorted values and ranges, its sorted keys' tile words (the
+    rebuilt keys carry the tile id only, sort.cu:205-215), and the same image
+    bytes as the default pipeline of the variant."""
     name, scene, spec, ref = case
     g = Gpu(scene, spec, capacity=_cap(ref))
     say("depth first", name, variant)
@@ -178,7 +180,8 @@ def test_depth_first_pipeline(case, variant):
     capi.check(st)
     M = ref["pre"]["M"]
     assert n == M
-    np.testing.assert_array_equal(g.ks[:M].cpu().numpy().view(np.uint64), ref["keys_sorted"])
+    np.testing.assert_array_equal(g.ks[:M].cpu().numpy().view(np.uint64),
+                                  ref["keys_sorted"] & TILE_WORD)
     np.testing.assert_array_equal(g.vs[:M].cpu().numpy().view(np.uint32), ref["vals_sorted"])
     if variant == capi.V_LITE:
         np.testing.assert_array_equal(g.ranges.cpu().numpy(),
@@ -199,7 +202,9 @@ def test_depth_first_overflow(gpu):
     st, n, img = g.forward(BG, capi.V_LITE | capi.DEPTH_FIRST)
     capi.check(st)
     assert n == M
-    np.testing.assert_array_equal(g.ks[:M].cpu().numpy().view(np.uint64), ref["keys_sorted"])
+    np.testing.assert_array_equal(g.ks[:M].cpu().numpy().view(np.uint64),
+                                  ref["keys_sorted"] & TILE_WORD)
+    np.testing.assert_array_equal(g.vs[:M].cpu().numpy().view(np.uint32), ref["vals_sorted"])
 
 
 def test_overflow_is_reported_and_nothing_is_written_past_capacity(gpu):
