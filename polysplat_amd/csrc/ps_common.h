@@ -186,6 +186,17 @@ struct Proj {
   bool valid;             // survived all culls and the rectangle is non-empty
 };
 
+// q = (x0, y0, x1, y1) / (bx, by, bx, by) for tile sizes other than 16.  Not
+// inlined, so that the 16x16 path does not evaluate these divisions
+// speculatively.
+struct Quot4 {
+  float x0, y0, x1, y1;
+};
+__host__ __device__ __attribute__((noinline)) inline Quot4 divide_by_block(Quot4 q, int bx,
+                                                                          int by) {
+  return Quot4{q.x0 / (float)bx, q.y0 / (float)by, q.x1 / (float)bx, q.y1 / (float)by};
+}
+
 // Cull + project + bounding rectangle (preprocess.cu:354-414).
 PS_HD Proj project(float px, float py, float pz, float opacity, const float* cov3,
                    const CamParams& cp) {
@@ -214,11 +225,20 @@ PS_HD Proj project(float px, float py, float pz, float opacity, const float* cov
   int height = f2i_trunc(1.414214f * __builtin_sqrtf(cov.z * o.power) + 1.0f);
   o.xy_x = ndc2pix(ndc_x, cp.W);
   o.xy_y = ndc2pix(ndc_y, cp.H);
-  const float bxf = (float)cp.block_x, byf = (float)cp.block_y;
-  o.rmin_x = imin(cp.grid_x, imax(0, f2i_trunc((o.xy_x - (float)width) / bxf)));
-  o.rmin_y = imin(cp.grid_y, imax(0, f2i_trunc((o.xy_y - (float)height) / byf)));
-  o.rmax_x = imin(cp.grid_x, imax(0, f2i_trunc((o.xy_x + (float)width) / bxf) + 1));
-  o.rmax_y = imin(cp.grid_y, imax(0, f2i_trunc((o.xy_y + (float)height) / byf) + 1));
+  Quot4 q{o.xy_x - (float)width, o.xy_y - (float)height, o.xy_x + (float)width,
+          o.xy_y + (float)height};
+  if (cp.block_x == 16 && cp.block_y == 16) {
+    // v / 16 == v * 0.0625 exactly (a power-of-two divisor): the quotient
+    // and the product are the same real number, rounded the same way.
+    q = Quot4{q.x0 * 0.0625f, q.y0 * 0.0625f, q.x1 * 0.0625f, q.y1 * 0.0625f};
+  } else {
+    q = divide_by_block(q, cp.block_x, cp.block_y);
+  }
+  const float qx0 = q.x0, qy0 = q.y0, qx1 = q.x1, qy1 = q.y1;
+  o.rmin_x = imin(cp.grid_x, imax(0, f2i_trunc(qx0)));
+  o.rmin_y = imin(cp.grid_y, imax(0, f2i_trunc(qy0)));
+  o.rmax_x = imin(cp.grid_x, imax(0, f2i_trunc(qx1) + 1));
+  o.rmax_y = imin(cp.grid_y, imax(0, f2i_trunc(qy1) + 1));
   o.valid = (o.rmax_x - o.rmin_x) * (o.rmax_y - o.rmin_y) > 0;
   return o;
 }

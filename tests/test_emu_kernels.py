@@ -69,12 +69,13 @@ def test_sort_segments_around_the_lds_limits(emu, mode, monkeypatch):
 
 @pytest.mark.parametrize("threads", [1, 8])
 def test_sort_long_segments_with_full_depth_words(emu, threads, monkeypatch):
-    # Segments for the 512-thread sort (up to 8192), the 1024-thread one (up
-    # to 32768) and past it, with depths over the full 32 bits (every digit
+    # Segments for each workgroup route (up to 4096, 8192 and 32768 pairs)
+    # and past them, with depths over the full 32 bits (every digit
     # pass runs) and many ties at the segment's largest depth (the padding
     # value).
     monkeypatch.setenv("EMU_THREADS", str(threads))
-    counts = [1000, 4096, 4097, 5000, 8192, 8193, 20000, 32767, 32768, 32769, 40000]
+    counts = [1000, 4096, 4097, 5000, 8192, 8193, 16383, 16384, 16385, 20000, 32767, 32768,
+              32769, 40000]
     rng = np.random.default_rng(123)
     t = np.repeat(np.arange(len(counts), dtype=np.uint64), counts)
     rng.shuffle(t)
@@ -256,6 +257,42 @@ def _emission_scene():
     sc = scenes.make_scene(20000, 6.0, [(0.9, -3.5, 0.5), (0.097, -2.0, 0.4), (0.003, 0.2, 0.3)],
                            seed=41)
     return sc, scenes.make_camera(640, 480, 6.0, distance=7.0)
+
+
+@pytest.mark.parametrize("block", [(16, 16), (8, 8), (32, 32), (16, 8), (12, 20)])
+def test_preprocess_tile_sizes(emu, block):
+    """Keys for tile sizes other than 16x16 (the rect bounds divide by the tile
+    size; 16 multiplies by 1/16, exact for a power of two) against the oracle
+    run with the same tile size."""
+    sc = _scene()
+    spec = scenes.make_camera(200, 120, 10.0, distance=12.0)
+    bx, by = block
+    cam_o = O.camera(spec.width, spec.height, spec.position.numpy(), spec.rotation.numpy(),
+                     spec.focal_x, spec.focal_y, spec.zFar, spec.zNear, bx, by)
+    ref = O.preprocess(sc.position.numpy(), sc.shs.numpy(), sc.opacity.numpy(),
+                       sc.cov3d.numpy(), cam_o)
+    M = ref["M"]
+    P = sc.num_vertex
+    pos = np.ascontiguousarray(sc.position.numpy())
+    shs = np.ascontiguousarray(sc.shs.numpy())
+    op = np.ascontiguousarray(sc.opacity.numpy())
+    cov = np.ascontiguousarray(sc.cov3d.numpy())
+    xy, rgbd, con = (np.zeros((P, 2), np.float32), np.zeros((P, 4), np.float32),
+                     np.zeros((P, 4), np.float32))
+    ku = np.zeros(M + 8, np.uint64)
+    vu = np.zeros(M + 8, np.uint32)
+    curr = np.zeros(1, np.int32)
+    nb = emu.ps_preprocess_workspace_size(P)
+    ws = np.zeros(nb, np.uint8)
+    cam = capi.camera(spec, block=bx)
+    cam.block_y = by
+    st = emu.ps_preprocess(P, HF.ptr(pos), HF.ptr(shs), 0, HF.ptr(op), HF.ptr(cov),
+                           capi.C.byref(cam), HF.ptr(xy), HF.ptr(rgbd), HF.ptr(con), None, 1,
+                           HF.ptr(ku), HF.ptr(vu), M + 8, HF.ptr(curr), HF.ptr(ws), nb, None)
+    capi.check_with(emu, st)
+    assert int(curr[0]) == M
+    np.testing.assert_array_equal(ku[:M], ref["keys"])
+    np.testing.assert_array_equal(vu[:M], ref["vals"])
 
 
 def _preprocess(L, sc, spec, capacity, extra=0, base=0, fill=-7, pool_taken=None):

@@ -207,6 +207,25 @@ def test_depth_first_overflow(gpu):
     np.testing.assert_array_equal(g.vs[:M].cpu().numpy().view(np.uint32), ref["vals_sorted"])
 
 
+@pytest.mark.parametrize("block", [(8, 8), (32, 32), (12, 20)])
+def test_preprocess_tile_sizes(gpu, block):
+    """ps_preprocess with tile sizes other than 16x16 against the oracle run
+    with the same tile size (keys and values bit-exact)."""
+    scene = _mixed_scene(20_000, 5)
+    spec = scenes.make_camera(1000, 700, 8.0)
+    bx, by = block
+    cam_o = O.camera(spec.width, spec.height, spec.position.numpy(), spec.rotation.numpy(),
+                     spec.focal_x, spec.focal_y, spec.zFar, spec.zNear, bx, by)
+    ref = O.preprocess(scene.position.numpy(), scene.shs.numpy(), scene.opacity.numpy(),
+                       scene.cov3d.numpy(), cam_o)
+    g = Gpu(scene, spec, capacity=ref["M"] + 64)
+    g.cam.block_x, g.cam.block_y = bx, by
+    say("tile size", block, ref["M"])
+    assert g.preprocess() == ref["M"]
+    np.testing.assert_array_equal(g.ku[:ref["M"]].cpu().numpy().view(np.uint64), ref["keys"])
+    np.testing.assert_array_equal(g.vu[:ref["M"]].cpu().numpy().view(np.uint32), ref["vals"])
+
+
 def test_overflow_is_reported_and_nothing_is_written_past_capacity(gpu):
     scene = scenes.make_config_scene("c1", seed=2, P=20_000)
     spec = scenes.config_camera("c1")

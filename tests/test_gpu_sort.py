@@ -138,12 +138,13 @@ def test_sort_segment_sizes_around_the_lds_limit(gpu, mode, monkeypatch):
 
 @pytest.mark.parametrize("mode", ["segmented", "lsd"])
 def test_sort_long_segments_with_full_depth_words(gpu, mode, monkeypatch):
-    # Segments for the 512-thread LDS sort (up to 8192), the 1024-thread
-    # one (up to 32768) and past it, depths over all 32 bits (every digit pass
+    # Segments for each workgroup LDS route (up to 4096, 8192 and 32768
+    # pairs) and past them, depths over all 32 bits (every digit pass
     # runs), with ties at the segment's largest depth (the padding value).
     if mode == "lsd":
         monkeypatch.setenv("POLYSPLAT_SORT", "lsd")
-    counts = [1000, 4096, 4097, 5000, 8192, 8193, 20000, 32767, 32768, 32769, 40000, 100000]
+    counts = [1000, 4096, 4097, 5000, 8192, 8193, 16383, 16384, 16385, 20000, 32767, 32768,
+              32769, 40000, 100000]
     rng = np.random.default_rng(123)
     t = np.repeat(np.arange(len(counts), dtype=np.uint64), counts)
     rng.shuffle(t)
