@@ -1,7 +1,10 @@
-"""Image drift metrics on uint8 images (scripts/correctness/check.py:118-145).
+"""Image drift metrics on uint8 images (scripts/correctness/check.py:118-156).
 
 PSNR is computed over the uint8 MSE of all channels; max/mean absolute
-difference and the fraction of pixels with any channel different."""
+difference and the fraction of pixels with any channel different; SSIM
+(Wang et al. 2004) with a 7x7 uniform window, K1 = 0.01, K2 = 0.03, sample
+covariances and the window-radius border cropped, averaged over channels;
+the check's OK / WARN / FAIL verdict."""
 from __future__ import annotations
 
 import math
@@ -13,6 +16,47 @@ def psnr_from_mse(mse: float) -> float:
     if mse <= 0.0:
         return math.inf
     return 20.0 * math.log10(255.0 / math.sqrt(mse))
+
+
+def ssim_uint8(a: np.ndarray, b: np.ndarray, win: int = 7) -> float:
+    """Mean structural similarity of two uint8 images [H, W, C] (or [H, W]),
+    data range 255."""
+    from scipy.ndimage import uniform_filter
+
+    x = np.asarray(a, np.float64)
+    y = np.asarray(b, np.float64)
+    if x.shape != y.shape:
+        raise ValueError(f"shape mismatch {x.shape} vs {y.shape}")
+    if x.ndim == 2:
+        x, y = x[..., None], y[..., None]
+    if min(x.shape[0], x.shape[1]) < win:
+        raise ValueError(f"images smaller than the {win}x{win} window")
+    c1, c2 = (0.01 * 255.0) ** 2, (0.03 * 255.0) ** 2
+    n = win * win
+    cov_norm = n / (n - 1.0)
+    r = (win - 1) // 2
+    vals = []
+    for ch in range(x.shape[2]):
+        X, Y = x[..., ch], y[..., ch]
+        f = lambda t: uniform_filter(t, size=win, mode="reflect")  # noqa: E731
+        ux, uy = f(X), f(Y)
+        vx = cov_norm * (f(X * X) - ux * ux)
+        vy = cov_norm * (f(Y * Y) - uy * uy)
+        vxy = cov_norm * (f(X * Y) - ux * uy)
+        s = ((2 * ux * uy + c1) * (2 * vxy + c2)) / ((ux * ux + uy * uy + c1) * (vx + vy + c2))
+        vals.append(float(s[r:s.shape[0] - r, r:s.shape[1] - r].mean()))
+    return float(np.mean(vals))
+
+
+def verdict(m: dict) -> str:
+    """OK: PSNR >= 30 dB and SSIM >= 0.95; WARN: PSNR >= 25 and SSIM >= 0.90;
+    FAIL otherwise (a missing SSIM passes its half of the test)."""
+    s = m.get("ssim")
+    if m["psnr_db"] >= 30 and (s is None or s >= 0.95):
+        return "OK"
+    if m["psnr_db"] >= 25 and (s is None or s >= 0.90):
+        return "WARN"
+    return "FAIL"
 
 
 def diff_metrics(a: np.ndarray, b: np.ndarray) -> dict:
