@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--config", default="c3")
     ap.add_argument("--scene-gaussians", type=int, default=0,
                     help="override the config's Gaussian count (profiling other sizes)")
+    ap.add_argument("--log-scale", type=float, default=None,
+                    help="per-axis log-scales from N(value, 0.4) instead of the config's "
+                         "mixture (more pairs per Gaussian as it grows)")
     ap.add_argument("--resolution", default="",
                     help="override the config's WxH (profiling other sizes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
@@ -182,13 +185,17 @@ def main():
     W, H = spec.width, spec.height
     T = ((W + 15) // 16) * ((H + 15) // 16)
     t0 = time.time()
-    arrays = scenes.make_config_scene(args.config, seed=1, device=dev,
-                                      P=args.scene_gaussians or None)
+    if args.log_scale is not None:
+        arrays = scenes.make_scene(args.scene_gaussians or cfg["P"], cfg["box"],
+                                   [(1.0, args.log_scale, 0.4)], seed=2, device=dev)
+    else:
+        arrays = scenes.make_config_scene(args.config, seed=1, device=dev,
+                                          P=args.scene_gaussians or None)
     torch.cuda.synchronize()
     log(f"scene {args.config}: P={arrays.num_vertex} {W}x{H} generated in {time.time() - t0:.1f}s")
     scene = polysplat.Scene.from_arrays(arrays, dev)
     P = scene.num_vertex
-    cap = 6 * P
+    cap = max(6 * P, 60_000_000) if args.log_scale is not None else 6 * P
     bg = torch.zeros(3)
 
     def make_rast(feat):
@@ -477,7 +484,9 @@ def main():
         "dtype": "fp32 blend, fp16 SH + features",
         "data": "synthetic (seeded generator, mixed-regime scale mixture)",
         "config": {"workload": f"{args.config}: {P} Gaussians, {W}x{H}, 16x16 tiles, "
-                               "FP16-feature persistent renderer",
+                               "FP16-feature persistent renderer" +
+                               ("" if args.log_scale is None else
+                                f", log-scales N({args.log_scale}, 0.4)"),
                    "gaussians": P, "width": W, "height": H, "num_rendered": M,
                    "num_rendered_needed": need},
         "fp32_plain_ms_per_frame": round(fp32, 4),
