@@ -66,6 +66,10 @@ def parse():
     ap.add_argument("--serve-tight-budget-gib", type=float, default=1.5)
     ap.add_argument("--serve-rate", type=float, default=240.0)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving stream")
+    ap.add_argument("--tile-orders", action="store_true",
+                    help="only time the FP16 persistent render over each tile order "
+                         "(row-major queue, heavy-first, zig-zag, Morton buckets) and the "
+                         "order computation, on this config's frame")
     return ap.parse_args()
 
 
@@ -154,6 +158,66 @@ def serve(args):
     return res
 
 
+def tile_orders(args, scene, spec, rast):
+    """Render ms of one FP16 persistent frame over each tile order (median of
+    --steps launches after --warmup), and the time to compute the order."""
+    import numpy as np
+    import torch
+
+    import polysplat_amd as polysplat
+
+    ops = polysplat.ops
+    W, H = spec.width, spec.height
+    gx = (W + 15) // 16
+    T = gx * ((H + 15) // 16)
+    bg = torch.zeros(3)
+    img, stats = rast.forward(scene, spec, bg, render_variant="preranges_smem_persistent_lite",
+                              return_stats=True)
+    M = stats["num_rendered"]
+    ref = img.cpu().numpy().copy()
+    out = rast._get_out_color(H, W, scene.device)
+
+    def median_ms(fn):
+        for _ in range(args.warmup):
+            fn()
+        ts = []
+        for _ in range(args.steps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return round(float(np.median(ts)), 4)
+
+    def base():
+        ops.render_records(M, W, H, rast.records, 1, False, rast.gaussian_values_sorted,
+                           rast.tile_ranges_buf, bg, out)
+
+    res = {"workload": f"{args.config}: {scene.num_vertex} Gaussians, {W}x{H}", "tiles": T,
+           "num_rendered": M, "render_ms": {"row_major_queue": median_ms(base)}}
+    lens = (rast.tile_ranges_buf[:T, 1] - rast.tile_ranges_buf[:T, 0]).cpu().numpy()
+    res["pairs_per_tile"] = {"mean": round(float(lens.mean()), 1), "max": int(lens.max())}
+    identical = True
+    for qmode in ("interleave", "contiguous"):
+        os.environ["POLYSPLAT_ORDER_QUEUES"] = qmode
+        for name, zig, bucket in (("heavy_first", 0, 0), ("zigzag264", 264, 0),
+                                  ("morton1280_zigzag792", 792, 1280)):
+            rast._tile_order(spec, ("persistent", zig, bucket))
+
+            def ordered():
+                ops.render_records_reordered(M, W, H, rast.records, 1,
+                                             rast.gaussian_values_sorted, rast.tile_desc_buf,
+                                             bg, out)
+            res["render_ms"][f"{name}_{qmode}"] = median_ms(ordered)
+            identical = identical and bool((out.cpu().numpy() == ref).all())
+    os.environ.pop("POLYSPLAT_ORDER_QUEUES", None)
+    res["order_ms"] = median_ms(lambda: rast._tile_order(spec, ("persistent", 0, 0)))
+    res["images_identical"] = identical
+    log(f"tile orders: {res}")
+    return res
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -231,6 +295,10 @@ def main():
                                      rast.conic_opacity, rec, rng, bg, rast._get_out_color(H, W, dev),
                                      rast.workspace, sync)
         return run
+
+    if args.tile_orders:
+        print(json.dumps(tile_orders(args, scene, spec, make_rast("fp16"))), flush=True)
+        return
 
     results = {}
     images = {}

@@ -243,6 +243,65 @@ int ps_render_16x16_persistent_noalloc(int num_rendered, const int* d_num_render
                                        const int* tile_ranges, int dt, const float bg[3],
                                        uint8_t* out_color, ps_stream_t stream);
 
+/* ------------------------------------------------------- tile reordering
+ * Tile order (render.cu:4411-4520, :6959-7100): the tiles sorted by their
+ * pair count tile_ranges[t][1] - tile_ranges[t][0], descending, ties in
+ * ascending tile id (the reference's stable descending radix sort).
+ * Outputs: tile_order[p] = tile at position p; tile_counts[p] = the sort
+ * key at position p; tile_ids[p] = p (may be NULL).  sort_temp:
+ * ps_get_tile_order_sort_temp_size(total_tiles) bytes of device memory.
+ * Replaces compute_tile_order (ops.h:258-266) and
+ * get_tile_order_sort_temp_size (ops.h:268); bound at pybind.cpp:461-482. */
+size_t ps_get_tile_order_sort_temp_size(int total_tiles);
+int ps_compute_tile_order(const int* tile_ranges, int total_tiles, uint32_t* tile_order,
+                          uint32_t* tile_counts, uint32_t* tile_ids, void* sort_temp,
+                          size_t sort_temp_bytes, ps_stream_t stream);
+/* Packed form: tile_desc[p] = (column, row, range start, range end) of the
+ * tile at order position zigzag(p) (groups of zigzag_group_size positions,
+ * every odd group reversed; 0 or 1 = no regrouping, render.cu:4838-4866).
+ * morton_bucket_size > 0 sorts by (min(count / bucket, 0xffff) << 16) |
+ * (0xffff - (Morton(column, row) & 0xffff)) instead of the count
+ * (render.cu:4496-4518).  Replaces compute_tile_order_packed (ops.h:277-288)
+ * and compute_tile_order_packed_morton (ops.h:290-302); bound at
+ * pybind.cpp:506-554. */
+int ps_compute_tile_order_packed(const int* tile_ranges, int total_tiles, int x_blocks,
+                                 int* tile_desc, uint32_t* tile_order, uint32_t* tile_counts,
+                                 uint32_t* tile_ids, void* sort_temp, size_t sort_temp_bytes,
+                                 int zigzag_group_size, int morton_bucket_size,
+                                 ps_stream_t stream);
+/* Plain renderer over a tile order: workgroup p renders tile tile_order[p]
+ * with its range from tile_ranges.  Replaces render_16x16_reordered
+ * (render.cu:7101-7125); bound at pybind.cpp:484-504. */
+int ps_render_16x16_reordered(int num_rendered, const int* d_num_rendered, int width, int height,
+                              const float* points_xy, const float* rgb_depth,
+                              const float* conic_opacity, const uint64_t* keys_sorted,
+                              const uint32_t* values_sorted, const int* tile_ranges,
+                              const uint32_t* tile_order, const float bg[3], uint8_t* out_color,
+                              ps_stream_t stream);
+/* Plain renderer over packed descriptors: workgroup p renders the tile and
+ * range of tile_desc[p] (as ps_compute_tile_order_packed writes them).
+ * Replaces render_16x16_reordered_v2 (render.cu:7127-7150); bound at
+ * pybind.cpp:556-575. */
+int ps_render_16x16_reordered_desc(int num_rendered, const int* d_num_rendered, int width,
+                                   int height, const float* points_xy, const float* rgb_depth,
+                                   const float* conic_opacity, const uint32_t* values_sorted,
+                                   const int* tile_desc, const float bg[3], uint8_t* out_color,
+                                   ps_stream_t stream);
+/* Persistent renderer over a tile order: the tile queue hands out order
+ * positions from the head, position p to the XCD queue p % 8 (environment
+ * POLYSPLAT_ORDER_QUEUES=contiguous: each XCD a contiguous eighth), each tile
+ * from tile_desc (packed, as ps_compute_tile_order_packed writes it) or,
+ * when tile_desc is NULL, tile_order and tile_ranges.  Other arguments as
+ * ps_render_16x16_persistent.  Replaces render_16x16_reordered_persistent
+ * (render.cu:7177-7215); bound at pybind.cpp:577-596. */
+int ps_render_16x16_reordered_persistent(int num_rendered, const int* d_num_rendered, int width,
+                                         int height, int feat_prec, const float* points_xy,
+                                         const float* rgb_depth, const float* conic_opacity,
+                                         const void* records, const uint32_t* values_sorted,
+                                         const int* tile_ranges, const uint32_t* tile_order,
+                                         const int* tile_desc, int dt, const float bg[3],
+                                         uint8_t* out_color, unsigned* queue, ps_stream_t stream);
+
 /* ---------------------------------------------------------------- forward
  * One frame on one stream: preprocess -> sort -> ranges -> render, with the
  * pair count passed between stages in device memory.  Replaces forward_fused

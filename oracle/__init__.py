@@ -58,6 +58,9 @@ def lib():
                                      C.c_longlong]
         L.orc_preprocess.restype = C.c_longlong
         L.orc_sort_pairs.argtypes = [C.c_longlong, P, P]
+        L.orc_morton.argtypes = [C.c_uint32, C.c_uint32]
+        L.orc_morton.restype = C.c_uint32
+        L.orc_tile_order.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P]
         L.orc_tile_ranges.argtypes = [C.c_longlong, P, C.c_int, P, C.c_int]
         L.orc_render_f32.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P, C.c_int, C.c_int, P, P,
                                      P, P]
@@ -223,6 +226,22 @@ def quantize_features(rgbd: np.ndarray, conic: np.ndarray, prec: str):
     lib().orc_quantize_features(P, PREC[prec], _p(np.ascontiguousarray(rgbd, np.float32)),
                                 _p(np.ascontiguousarray(conic, np.float32)), _p(rq), _p(cq))
     return rq, cq
+
+
+def morton(x: int, y: int) -> int:
+    return int(lib().orc_morton(x, y))
+
+
+def tile_order(ranges: np.ndarray, gx: int, zigzag: int = 0, bucket: int = 0):
+    """(order, sorted keys, packed descriptors [T, 4]) of the reordered
+    renderers for tile ranges [T, 2]."""
+    rg = np.ascontiguousarray(ranges, dtype=np.int32).reshape(-1, 2)
+    T = rg.shape[0]
+    order = np.zeros(max(T, 1), np.uint32)
+    keys = np.zeros(max(T, 1), np.uint32)
+    desc = np.zeros((max(T, 1), 4), np.int32)
+    lib().orc_tile_order(_p(rg), T, gx, zigzag, bucket, _p(order), _p(keys), _p(desc))
+    return order[:T], keys[:T], desc[:T]
 
 
 def tiles_of(W, H, bx=16, by=16):

@@ -441,6 +441,76 @@ void orc_tile_ranges(long long n, const uint64_t* keys, int tiles, int* ranges, 
   }
 }
 
+/* ------------------------------------------------------------- tile order */
+/* Morton code: bit i of x to bit 2i, bit i of y to bit 2i+1 (render.cu:4430-4441),
+ * one bit at a time. */
+uint32_t orc_morton(uint32_t x, uint32_t y) {
+  uint32_t z = 0;
+  int b;
+  for (b = 0; b < 16; ++b) {
+    z |= ((x >> b) & 1u) << (2 * b);
+    z |= ((y >> b) & 1u) << (2 * b + 1);
+  }
+  return z;
+}
+
+/* extractTileCounts / extractTileCountsMorton (render.cu:4414-4427, :4496-4518),
+ * SortPairsDescending (render.cu:6974-6978: stable, so ties keep ascending
+ * tile ids), gatherTileDesc / gatherTileDescZigzag (render.cu:4812-4866). */
+void orc_tile_order(const int* ranges, int tiles, int gx, int zigzag, int bucket,
+                    uint32_t* order, uint32_t* keys_sorted, int* desc) {
+  uint32_t* key = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(tiles > 0 ? tiles : 1));
+  uint32_t* tmp = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(tiles > 0 ? tiles : 1));
+  int t, p, w;
+  for (t = 0; t < tiles; ++t) {
+    uint32_t count = (uint32_t)(ranges[2 * t + 1] - ranges[2 * t]);
+    if (bucket > 0) {
+      uint32_t b = count / (uint32_t)bucket;
+      if (b > 0xffffu) b = 0xffffu;
+      uint32_t m = orc_morton((uint32_t)(t % gx), (uint32_t)(t / gx)) & 0xffffu;
+      key[t] = (b << 16) | (0xffffu - m);
+    } else {
+      key[t] = count;
+    }
+    order[t] = (uint32_t)t;
+  }
+  /* bottom-up merge sort by key, descending; ties: left run first (stable) */
+  for (w = 1; w < tiles; w *= 2) {
+    int lo;
+    for (lo = 0; lo < tiles; lo += 2 * w) {
+      int mid = lo + w < tiles ? lo + w : tiles;
+      int hi = lo + 2 * w < tiles ? lo + 2 * w : tiles;
+      int i = lo, j = mid, k = lo;
+      while (i < mid && j < hi) {
+        if (key[order[j]] > key[order[i]]) tmp[k++] = order[j++];
+        else tmp[k++] = order[i++];
+      }
+      while (i < mid) tmp[k++] = order[i++];
+      while (j < hi) tmp[k++] = order[j++];
+    }
+    memcpy(order, tmp, sizeof(uint32_t) * (size_t)tiles);
+  }
+  for (p = 0; p < tiles; ++p) keys_sorted[p] = key[order[p]];
+  if (desc) {
+    for (p = 0; p < tiles; ++p) {
+      int src = p;
+      if (zigzag > 1) {
+        int group = p / zigzag, slot = p % zigzag, base = group * zigzag;
+        int end = base + zigzag < tiles ? base + zigzag : tiles;
+        src = (group & 1) ? base + (end - base - 1 - slot) : base + slot;
+        if (src >= tiles) src = tiles - 1;
+      }
+      t = (int)order[src];
+      desc[4 * p + 0] = t % gx;
+      desc[4 * p + 1] = t / gx;
+      desc[4 * p + 2] = ranges[2 * t];
+      desc[4 * p + 3] = ranges[2 * t + 1];
+    }
+  }
+  free(key);
+  free(tmp);
+}
+
 /* ------------------------------------------------------------------ render */
 static uint8_t enc(float x) {
   float v = x * 255.0f;

@@ -80,6 +80,18 @@ float orc_bf16_to_f32(uint16_t h);
 void orc_quantize_features(int P, int prec, const float* rgbd, const float* conic,
                            float* rgbd_q, float* conic_q);
 
+/* Tile order of the reordered renderers (render.cu:4411-4520, :4790-4866,
+ * :6959-7087).  key[t] = count of tile t (ranges[2t+1] - ranges[2t]) or,
+ * with bucket > 0, (min(count / bucket, 0xffff) << 16) |
+ * (0xffff - (morton(t % gx, t / gx) & 0xffff)); order = tiles by key,
+ * descending, ties by ascending tile id; keys_sorted[p] = key of order[p];
+ * desc (may be NULL): 4 ints per position p = (column, row, range start,
+ * range end) of order[zigzag(p)], zigzag reversing every odd group of
+ * `zigzag` positions (no regrouping for zigzag <= 1). */
+uint32_t orc_morton(uint32_t x, uint32_t y);
+void orc_tile_order(const int* ranges, int tiles, int gx, int zigzag, int bucket,
+                    uint32_t* order, uint32_t* keys_sorted, int* desc);
+
 int orc_num_threads(void);
 
 #endif

@@ -33,6 +33,7 @@ HIP_SOURCES = {
     "ranges_gather.hip": [],
     "render.hip": [],
     "forward.hip": [],
+    "tile_order.hip": [],
 }
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

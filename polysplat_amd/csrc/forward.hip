@@ -28,15 +28,6 @@ int preprocess_es_launch(int P, const float* positions, const void* shs, int sh_
                          int capacity, int* curr_offset, void* workspace, size_t workspace_bytes,
                          void* sort_space, size_t sort_space_bytes, int sort_max_pairs,
                          hipStream_t stream);
-int render_plain_launch(int n_host, const int* d_n, int W, int H, const float* xy,
-                        const float* rgbd, const float* con, const uint64_t* keys,
-                        const uint32_t* order, const float bg[3], uint8_t* out,
-                        hipStream_t stream);
-int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_prec,
-                             const float* xy, const float* rgbd, const float* con, const void* rec,
-                             int gathered, const uint64_t* keys, const uint32_t* order,
-                             const int* ranges, int dt, const float bg[3], uint8_t* out,
-                             unsigned* queue, hipStream_t stream);
 
 FrameDev make_frame_dev(int P, const float* positions, const void* shs, const float* opacities,
                         const float* cov3d, const ps_camera* cam, float* xy, float* rgbd,

@@ -426,6 +426,156 @@ void render_records(int num_rendered, int width, int height, Tensor& records, in
                     "render_records");
 }
 
+// ------------------------------------------------------------------ tile reordering
+int64_t get_tile_order_sort_temp_size(int total_tiles) {
+  return (int64_t)ps_get_tile_order_sort_temp_size(total_tiles);
+}
+
+void check_order_bufs(int total_tiles, Tensor& tile_ranges, Tensor& tile_order,
+                      Tensor& tile_counts, Tensor& tile_ids, Tensor& sort_temp) {
+  need(total_tiles >= 0, "total_tiles must be >= 0");
+  need_dev(tile_ranges, "tile_ranges_buf", at::kInt);
+  need_dev(tile_order, "tile_order", at::kInt);
+  need_dev(tile_counts, "tile_counts_buf", at::kInt);
+  need_dev(tile_ids, "tile_ids_buf", at::kInt);
+  need(sort_temp.is_cuda() && sort_temp.is_contiguous(), "sort_temp must be a GPU tensor");
+  need(tile_ranges.numel() >= 2LL * total_tiles && tile_order.numel() >= total_tiles &&
+           tile_counts.numel() >= total_tiles && tile_ids.numel() >= total_tiles,
+       "tile buffers smaller than total_tiles");
+  need((size_t)sort_temp.nbytes() >= ps_get_tile_order_sort_temp_size(total_tiles),
+       "sort_temp smaller than get_tile_order_sort_temp_size(total_tiles)");
+}
+
+void compute_tile_order(Tensor& tile_ranges, int total_tiles, Tensor& tile_order,
+                        Tensor& tile_counts, Tensor& tile_ids, Tensor& sort_temp) {
+  check_order_bufs(total_tiles, tile_ranges, tile_order, tile_counts, tile_ids, sort_temp);
+  check(ps_compute_tile_order(ptr<int>(tile_ranges), total_tiles, ptr<uint32_t>(tile_order),
+                              ptr<uint32_t>(tile_counts), ptr<uint32_t>(tile_ids),
+                              sort_temp.data_ptr(), (size_t)sort_temp.nbytes(), cur_stream()),
+        "compute_tile_order");
+}
+
+void compute_tile_order_packed_common(Tensor& tile_ranges, int total_tiles, int x_blocks,
+                                      Tensor& tile_desc, Tensor& tile_order, Tensor& tile_counts,
+                                      Tensor& tile_ids, Tensor& sort_temp, int zigzag,
+                                      int bucket, const char* what) {
+  check_order_bufs(total_tiles, tile_ranges, tile_order, tile_counts, tile_ids, sort_temp);
+  need_dev(tile_desc, "tile_desc_buf", at::kInt);
+  need(tile_desc.numel() >= 4LL * total_tiles, "tile_desc_buf smaller than total_tiles x 4");
+  need(x_blocks > 0, "x_blocks must be positive");
+  check(ps_compute_tile_order_packed(ptr<int>(tile_ranges), total_tiles, x_blocks,
+                                     ptr<int>(tile_desc), ptr<uint32_t>(tile_order),
+                                     ptr<uint32_t>(tile_counts), ptr<uint32_t>(tile_ids),
+                                     sort_temp.data_ptr(), (size_t)sort_temp.nbytes(), zigzag,
+                                     bucket, cur_stream()),
+        what);
+}
+
+void compute_tile_order_packed(Tensor& tile_ranges, int total_tiles, int x_blocks,
+                               Tensor& tile_desc, Tensor& tile_order, Tensor& tile_counts,
+                               Tensor& tile_ids, Tensor& sort_temp, int zigzag_group_size) {
+  compute_tile_order_packed_common(tile_ranges, total_tiles, x_blocks, tile_desc, tile_order,
+                                   tile_counts, tile_ids, sort_temp, zigzag_group_size, 0,
+                                   "compute_tile_order_packed");
+}
+
+void compute_tile_order_packed_morton(Tensor& tile_ranges, int total_tiles, int x_blocks,
+                                      Tensor& tile_desc, Tensor& tile_order, Tensor& tile_counts,
+                                      Tensor& tile_ids, Tensor& sort_temp, int zigzag_group_size,
+                                      int morton_bucket_size) {
+  need(morton_bucket_size > 0, "morton_bucket_size must be positive");
+  compute_tile_order_packed_common(tile_ranges, total_tiles, x_blocks, tile_desc, tile_order,
+                                   tile_counts, tile_ids, sort_temp, zigzag_group_size,
+                                   morton_bucket_size, "compute_tile_order_packed_morton");
+}
+
+void render_16x16_reordered(int num_rendered, int width, int height, Tensor& points_xy,
+                            Tensor& rgb_depth, Tensor& conic_opacity, Tensor& keys_sorted,
+                            Tensor& values_sorted, Tensor& tile_ranges, Tensor& tile_order,
+                            Tensor& bg_color, Tensor& out_color) {
+  check_feats(points_xy, rgb_depth, conic_opacity);
+  need_dev(keys_sorted, "gaussian_keys_sorted", at::kLong);
+  need_dev(values_sorted, "gaussian_values_sorted", at::kInt);
+  need_dev(tile_ranges, "tile_ranges_buf", at::kInt);
+  need_dev(tile_order, "tile_order", at::kInt);
+  const int64_t T = tiles_of(width, height, 16, 16);
+  need(tile_ranges.numel() >= 2 * T && tile_order.numel() >= T,
+       "tile_ranges_buf or tile_order smaller than the frame's tiles");
+  check_out(out_color, width, height);
+  auto bg = host_floats(bg_color, 3, "bg_color");
+  check(ps_render_16x16_reordered(num_rendered, nullptr, width, height, ptr<float>(points_xy),
+                                  ptr<float>(rgb_depth), ptr<float>(conic_opacity),
+                                  ptr<uint64_t>(keys_sorted), ptr<uint32_t>(values_sorted),
+                                  ptr<int>(tile_ranges), ptr<uint32_t>(tile_order), bg.data(),
+                                  ptr<uint8_t>(out_color), cur_stream()),
+        "render_16x16_reordered");
+}
+
+void render_16x16_reordered_v2(int num_rendered, int width, int height, Tensor& points_xy,
+                               Tensor& rgb_depth, Tensor& conic_opacity, Tensor& keys_sorted,
+                               Tensor& values_sorted, Tensor& tile_desc, Tensor& bg_color,
+                               Tensor& out_color) {
+  check_feats(points_xy, rgb_depth, conic_opacity);
+  need_dev(keys_sorted, "gaussian_keys_sorted", at::kLong);
+  need_dev(values_sorted, "gaussian_values_sorted", at::kInt);
+  need_dev(tile_desc, "tile_desc_buf", at::kInt);
+  need(tile_desc.numel() >= 4 * tiles_of(width, height, 16, 16),
+       "tile_desc_buf smaller than the frame's tiles x 4");
+  check_out(out_color, width, height);
+  auto bg = host_floats(bg_color, 3, "bg_color");
+  check(ps_render_16x16_reordered_desc(num_rendered, nullptr, width, height,
+                                       ptr<float>(points_xy), ptr<float>(rgb_depth),
+                                       ptr<float>(conic_opacity), ptr<uint32_t>(values_sorted),
+                                       ptr<int>(tile_desc), bg.data(), ptr<uint8_t>(out_color),
+                                       cur_stream()),
+        "render_16x16_reordered_v2");
+}
+
+// Persistent render over packed tile descriptors, from FP32 features
+// (`records` undefined or empty) or 24-byte records (feat_prec 1 or 2).
+void reordered_persistent_common(int num_rendered, int width, int height, Tensor* xy,
+                                 Tensor* rgbd, Tensor* con, Tensor* records, int prec,
+                                 Tensor& values_sorted, Tensor& tile_desc, bool dt, Tensor& bg_color,
+                                 Tensor& out, const char* what) {
+  if (prec == PS_PREC_F32) check_feats(*xy, *rgbd, *con);
+  else need_dev(*records, "records", at::kByte);
+  need_dev(values_sorted, "gaussian_values_sorted", at::kInt);
+  need_dev(tile_desc, "tile_desc_buf", at::kInt);
+  need(tile_desc.numel() >= 4 * tiles_of(width, height, 16, 16),
+       "tile_desc_buf smaller than the frame's tiles x 4");
+  check_out(out, width, height);
+  auto bg = host_floats(bg_color, 3, "bg_color");
+  Tensor q = cached_workspace(out.device(), 2048, "queue");
+  check(ps_render_16x16_reordered_persistent(
+            num_rendered, nullptr, width, height, prec, xy ? ptr<float>(*xy) : nullptr,
+            rgbd ? ptr<float>(*rgbd) : nullptr, con ? ptr<float>(*con) : nullptr,
+            records ? records->data_ptr() : nullptr, ptr<uint32_t>(values_sorted), nullptr,
+            nullptr, ptr<int>(tile_desc), dt ? 1 : 0, bg.data(), ptr<uint8_t>(out),
+            ptr<unsigned>(q), cur_stream()),
+        what);
+}
+
+void render_16x16_reordered_persistent(int num_rendered, int width, int height,
+                                       Tensor& points_xy, Tensor& rgb_depth,
+                                       Tensor& conic_opacity, Tensor& keys_sorted,
+                                       Tensor& values_sorted, Tensor& tile_desc, Tensor& bg_color,
+                                       Tensor& out_color) {
+  need_dev(keys_sorted, "gaussian_keys_sorted", at::kLong);
+  reordered_persistent_common(num_rendered, width, height, &points_xy, &rgb_depth,
+                              &conic_opacity, nullptr, PS_PREC_F32, values_sorted, tile_desc,
+                              false, bg_color, out_color, "render_16x16_reordered_persistent");
+}
+
+void render_records_reordered(int num_rendered, int width, int height, Tensor& records,
+                              int feat_prec, Tensor& values_sorted, Tensor& tile_desc, Tensor& bg,
+                              Tensor& out, bool dt) {
+  need(feat_prec == PS_PREC_F16 || feat_prec == PS_PREC_BF16,
+       "render_records_reordered: feat_prec must be 1 (fp16) or 2 (bf16)");
+  reordered_persistent_common(num_rendered, width, height, nullptr, nullptr, nullptr, &records,
+                              feat_prec, values_sorted, tile_desc, dt, bg, out,
+                              "render_records_reordered");
+}
+
 void gather_records(int num_rendered, Tensor& values_sorted, Tensor& src, Tensor& dst) {
   need_dev(values_sorted, "sorted_values", at::kInt);
   need_dev(src, "src", at::kByte);
@@ -741,6 +891,17 @@ PYBIND11_MODULE(_C, m) {
   ops.def("render_records", &render_records, py::arg("num_rendered"), py::arg("width"),
           py::arg("height"), py::arg("records"), py::arg("feat_prec"), py::arg("gathered"),
           py::arg("values_sorted"), py::arg("tile_ranges"), py::arg("bg_color"),
+          py::arg("out_color"), py::arg("dt") = false);
+  ops.def("get_tile_order_sort_temp_size", &get_tile_order_sort_temp_size);
+  ops.def("compute_tile_order", &compute_tile_order);
+  ops.def("compute_tile_order_packed", &compute_tile_order_packed);
+  ops.def("compute_tile_order_packed_morton", &compute_tile_order_packed_morton);
+  ops.def("render_16x16_reordered", &render_16x16_reordered);
+  ops.def("render_16x16_reordered_v2", &render_16x16_reordered_v2);
+  ops.def("render_16x16_reordered_persistent", &render_16x16_reordered_persistent);
+  ops.def("render_records_reordered", &render_records_reordered, py::arg("num_rendered"),
+          py::arg("width"), py::arg("height"), py::arg("records"), py::arg("feat_prec"),
+          py::arg("values_sorted"), py::arg("tile_desc"), py::arg("bg_color"),
           py::arg("out_color"), py::arg("dt") = false);
   ops.def("forward_fused", &forward_fused);
   ops.def("forward_fused_e2e", &forward_fused_e2e);

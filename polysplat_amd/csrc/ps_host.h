@@ -122,6 +122,33 @@ inline int bit_length(unsigned n) {
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // Launchers shared between translation units.
+// Plain renderer (render.hip): one workgroup per tile, ranges from
+// tile_ranges or, when that is NULL, a search of keys; with a tile order,
+// workgroup p renders tile tile_order[p], with packed descriptors the tile
+// and range of descriptor p.
+int render_plain_launch(int n_host, const int* d_n, int W, int H, const float* xy,
+                        const float* rgbd, const float* con, const uint64_t* keys,
+                        const uint32_t* order, const float bg[3], uint8_t* out,
+                        hipStream_t stream, const int* ranges = nullptr,
+                        const uint32_t* tile_order = nullptr, const int* tile_desc = nullptr);
+// Tile order (tile_order.hip): tiles by pair count, descending, stable;
+// bucket > 0 selects the Morton-bucket keys; desc (int4 per position,
+// zig-zag regrouped when zigzag > 1) is optional.
+int tile_order_launch(const int* tile_ranges, int tiles, int gx, int zigzag, int bucket,
+                      uint32_t* order, uint32_t* keys_out, uint32_t* ids_out, int* desc,
+                      void* temp, size_t temp_bytes, hipStream_t stream);
+size_t tile_order_temp_size(int tiles);
+// Persistent renderer (render.hip).  tile_order: NULL for row-major tiles,
+// else a permutation of the frame's tiles, rendered in that order;
+// tile_desc: NULL, or int4 (column, row, start, end) per order position,
+// which then replaces tile_ranges and tile_order.
+int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_prec,
+                             const float* xy, const float* rgbd, const float* con, const void* rec,
+                             int gathered, const uint64_t* keys, const uint32_t* order,
+                             const int* ranges, int dt, const float bg[3], uint8_t* out,
+                             unsigned* queue, hipStream_t stream,
+                             const uint32_t* tile_order = nullptr,
+                             const int* tile_desc = nullptr);
 int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles,
                             const uint64_t* keys, int* ranges, hipStream_t stream,
                             uint32_t mask = 0xffffffffu, int* zero_counter = nullptr,

@@ -478,15 +478,39 @@ __global__ __launch_bounds__(kWave, 6) void ps_render_plain(
     const uint64_t* __restrict__ keys, const int* __restrict__ d_n, int n_host, int W, int H,
     int gx, const uint32_t* __restrict__ order, const float2* __restrict__ xy,
     const float4* __restrict__ rgbd, const float4* __restrict__ con,
-    const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out) {
+    const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
+    const int2* __restrict__ ranges, const uint32_t* __restrict__ tile_order,
+    const int4* __restrict__ tile_desc) {
   __shared__ float4 sA[2][kBatch];
   __shared__ float4 sB[2][kBatch];
   __shared__ float sC[2][kBatch];
   const int lane = threadIdx.x;
-  const int tile = blockIdx.y * gx + blockIdx.x;
-  const int n = d_n ? *d_n : n_host;
-  const int start = wave_bound(keys, n, (uint32_t)tile, false, lane);
-  const int end = wave_bound(keys, n, (uint32_t)tile, true, lane);
+  const int pos = blockIdx.y * gx + blockIdx.x;
+  const int tiles = gx * (int)gridDim.y;
+  // With a tile order, workgroup `pos` renders tile tile_order[pos]
+  // (render.cu:4642-4644), or the tile of packed descriptor pos (column,
+  // row, range start, range end; render.cu:4959-4962); an entry outside
+  // the frame is skipped.
+  int tile, start, end;
+  if (tile_desc) {
+    const int4 d = tile_desc[pos];
+    tile = d.y * gx + d.x;
+    if ((unsigned)d.x >= (unsigned)gx || (unsigned)tile >= (unsigned)tiles) return;
+    start = d.z;
+    end = d.w;
+  } else {
+  tile = tile_order ? (int)tile_order[pos] : pos;
+  if (tile_order && (unsigned)tile >= (unsigned)tiles) return;
+  if (ranges) {
+    const int2 r = ranges[tile];
+    start = r.x;
+    end = r.y;
+  } else {
+    const int n = d_n ? *d_n : n_host;
+    start = wave_bound(keys, n, (uint32_t)tile, false, lane);
+    end = wave_bound(keys, n, (uint32_t)tile, true, lane);
+  }
+  }
   render_tile<PREC, false, CHECK, false>(tile, start, end, W, H, gx, order, xy, rgbd, con, rec, bg,
                                          out, sA, sB, sC, lane);
 }
@@ -510,21 +534,37 @@ constexpr int kQueueDone = kQueueLines * kQueueStride;  // finished workgroups
 
 struct TileQueue {
   unsigned* q;
-  int tiles, lane, home, probe = 0;
-  __device__ TileQueue(unsigned* queue, int ntiles, int l)
-      : q(queue), tiles(ntiles), lane(l), home((int)(blockIdx.x % kQueueLines)) {}
-  __device__ int lo(int x) const { return (int)(((long long)x * tiles) / kQueueLines); }
-  // The next tile, or -1 when every queue is drained.  The claim is
+  int items, lane, home, probe = 0;
+  int group;  // items per tile (4 with quadrant items), kept on one queue
+  bool inter;
+  __device__ TileQueue(unsigned* queue, int nitems, int l, int items_per_tile, bool interleave)
+      : q(queue), items(nitems), lane(l), home((int)(blockIdx.x % kQueueLines)),
+        group(items_per_tile), inter(interleave) {}
+  __device__ int lo(int x) const { return (int)(((long long)x * items) / kQueueLines); }
+  // Items of queue x: a contiguous eighth [lo(x), lo(x + 1)), or, when the
+  // items follow a tile order (interleave), the items of every eighth tile
+  // of the order from tile x, so that each XCD starts on its share of the
+  // order's head and a tile's quadrants stay on one XCD.
+  __device__ int len(int x) const {
+    if (!inter) return lo(x + 1) - lo(x);
+    const int units = items / group;
+    return x < units ? group * ((units - 1 - x) / kQueueLines + 1) : 0;
+  }
+  __device__ int item(int x, int c) const {
+    if (!inter) return lo(x) + c;
+    return (x + kQueueLines * (c / group)) * group + c % group;
+  }
+  // The next item, or -1 when every queue is drained.  The claim is
   // compared unsigned, so a counter that does not start at zero can only
-  // end the queue early, never yield a tile outside it.
+  // end the queue early, never yield an item outside it.
   __device__ int claim() {
     while (probe < kQueueLines) {
       const int x = (home + probe) % kQueueLines;
-      const int a = lo(x), len = lo(x + 1) - a;
+      const int n = len(x);
       unsigned c = 0;
-      if (lane == 0 && len > 0) c = atomicAdd(&q[x * kQueueStride], 1u);
+      if (lane == 0 && n > 0) c = atomicAdd(&q[x * kQueueStride], 1u);
       c = (unsigned)bcast((int)c, 0);
-      if (len > 0 && c < (unsigned)len) return a + (int)c;
+      if (n > 0 && c < (unsigned)n) return item(x, (int)c);
       ++probe;  // drained for good: counters only grow
     }
     return -1;
@@ -555,7 +595,8 @@ __global__ __launch_bounds__(kWave, 7) void ps_render_persistent(
     const uint32_t* __restrict__ order, const float2* __restrict__ xy,
     const float4* __restrict__ rgbd, const float4* __restrict__ con,
     const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
-    unsigned* __restrict__ queue) {
+    unsigned* __restrict__ queue, const uint32_t* __restrict__ tile_order,
+    const int4* __restrict__ tile_desc, int interleave) {
   __shared__ float4 sA[2][kBatch];
   __shared__ float4 sB[2][kBatch];
   __shared__ float sC[2][kBatch];
@@ -563,17 +604,31 @@ __global__ __launch_bounds__(kWave, 7) void ps_render_persistent(
   const int pairs = d_n ? *d_n : n_host;
   const int n = RANGES ? 0 : pairs;
   if (quads == 2) quads = (long long)pairs >= (long long)kQuadPairs * tiles ? 1 : 0;
-  TileQueue tq(queue, quads ? 4 * tiles : tiles, lane);
+  const bool ordered = (tile_order != nullptr || tile_desc != nullptr) && interleave;
+  TileQueue tq(queue, quads ? 4 * tiles : tiles, lane, quads ? 4 : 1, ordered);
   for (int item = tq.claim(); item >= 0; item = tq.claim()) {
-    const int tile = quads ? item >> 2 : item;
-    int start, end;
-    if (RANGES) {
-      const int2 r = ranges[tile];
-      start = r.x;
-      end = r.y;
+    const int pos = quads ? item >> 2 : item;
+    int tile, start, end;
+    if (tile_desc) {
+      // Packed descriptor (column, row, range start, range end).
+      const int4 d = tile_desc[pos];
+      tile = d.y * gx + d.x;
+      start = d.z;
+      end = d.w;
+      if ((unsigned)d.x >= (unsigned)gx || (unsigned)tile >= (unsigned)tiles) continue;
     } else {
-      start = wave_bound(keys, n, (uint32_t)tile, false, lane);
-      end = wave_bound(keys, n, (uint32_t)tile, true, lane);
+      // A tile order is a permutation of the tiles; an entry outside the
+      // frame (a caller's bad order) is skipped rather than rendered.
+      tile = tile_order ? (int)tile_order[pos] : pos;
+      if (tile_order && (unsigned)tile >= (unsigned)tiles) continue;
+      if (RANGES) {
+        const int2 r = ranges[tile];
+        start = r.x;
+        end = r.y;
+      } else {
+        start = wave_bound(keys, n, (uint32_t)tile, false, lane);
+        end = wave_bound(keys, n, (uint32_t)tile, true, lane);
+      }
     }
     if (quads)
       render_quad<PREC, GATHERED, CHECK, DT>(tile, item & 3, start, end, W, H, gx, order, xy,
@@ -674,61 +729,70 @@ int quadrant_items(int tiles, int waves) {
   return tiles < 2 * waves ? 2 : 0;
 }
 
+// How the tile queue deals out the positions of a tile order: interleaved
+// (position p to XCD queue p % 8, so every XCD starts on the heaviest
+// tiles) or in contiguous eighths (each XCD a run of the order, which keeps
+// a spatial order's neighbourhoods on one XCD's L2).
+// POLYSPLAT_ORDER_QUEUES=contiguous selects the second.
+int order_queues_interleaved() {
+  const char* e = getenv("POLYSPLAT_ORDER_QUEUES");
+  return (e && e[0] == 'c') ? 0 : 1;
+}
+
+// Arguments of one persistent render launch.
+struct PersistArgs {
+  const uint64_t* keys;
+  const int* d_n;
+  int n_host;
+  const int* ranges;
+  int W, H;
+  const uint32_t* order;
+  const float* xy;
+  const float* rgbd;
+  const float* con;
+  const void* rec;
+  float3 bg;
+  uint8_t* out;
+  unsigned* queue;
+  const uint32_t* tile_order;  // NULL: tiles in row-major order
+  const int* tile_desc;        // NULL, or int4 (column, row, start, end) per position
+};
+
 template <int PREC, bool RANGES, bool GATHERED, int CHECK, bool DT>
-int launch_persistent(const uint64_t* keys, const int* d_n, int n_host, const int* ranges, int W,
-                      int H, const uint32_t* order, const float* xy, const float* rgbd,
-                      const float* con, const void* rec, float3 bg, uint8_t* out, unsigned* queue,
-                      hipStream_t stream) {
-  const int gx = (W + kTile - 1) / kTile, gy = (H + kTile - 1) / kTile;
+int launch_persistent(const PersistArgs& a, hipStream_t stream) {
+  const int gx = (a.W + kTile - 1) / kTile, gy = (a.H + kTile - 1) / kTile;
   const int tiles = gx * gy;
   auto k = ps_render_persistent<PREC, RANGES, GATHERED, CHECK, DT>;
   const int waves = persistent_grid(k, 1 << 30);
   const int quads = quadrant_items(tiles, waves);
   const int items = quads == 1 ? 4 * tiles : tiles;  // quads == 2: tiles >= waves
   const int grid = waves < items ? waves : items;
-  hipLaunchKernelGGL(k, dim3(grid), dim3(kWave), 0, stream, keys, d_n, n_host,
-                     reinterpret_cast<const int2*>(ranges), W, H, gx, tiles, quads, order,
-                     (const float2*)xy, (const float4*)rgbd, (const float4*)con,
-                     (const FeatRec16*)rec, bg, out, queue);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kWave), 0, stream, a.keys, a.d_n, a.n_host,
+                     reinterpret_cast<const int2*>(a.ranges), a.W, a.H, gx, tiles, quads, a.order,
+                     (const float2*)a.xy, (const float4*)a.rgbd, (const float4*)a.con,
+                     (const FeatRec16*)a.rec, a.bg, a.out, a.queue, a.tile_order,
+                     reinterpret_cast<const int4*>(a.tile_desc), order_queues_interleaved());
   PS_LAUNCH_CHECK();
   return PS_OK;
 }
 
 template <int PREC, bool RANGES, bool GATHERED>
-int dispatch_dt(bool dt, const uint64_t* keys, const int* d_n, int n_host, const int* ranges,
-                int W, int H, const uint32_t* order, const float* xy, const float* rgbd,
-                const float* con, const void* rec, float3 bg, uint8_t* out, unsigned* queue,
-                hipStream_t s) {
+int dispatch_dt(bool dt, const PersistArgs& a, hipStream_t s) {
   // Cadence of the saturation test: every 8 Gaussians with precomputed
   // ranges (render.cu:1814), every 16 with the in-kernel search (render.cu:2124).
   constexpr int kCheck = RANGES ? 8 : 16;
-  if (dt)
-    return launch_persistent<PREC, RANGES, GATHERED, kCheck, true>(keys, d_n, n_host, ranges, W, H,
-                                                                   order, xy, rgbd, con, rec, bg,
-                                                                   out, queue, s);
-  return launch_persistent<PREC, RANGES, GATHERED, kCheck, false>(keys, d_n, n_host, ranges, W, H,
-                                                                  order, xy, rgbd, con, rec, bg,
-                                                                  out, queue, s);
+  if (dt) return launch_persistent<PREC, RANGES, GATHERED, kCheck, true>(a, s);
+  return launch_persistent<PREC, RANGES, GATHERED, kCheck, false>(a, s);
 }
 
 template <int PREC>
-int dispatch_persistent(bool ranges_mode, bool gathered, bool dt, const uint64_t* keys,
-                        const int* d_n, int n_host, const int* ranges, int W, int H,
-                        const uint32_t* order, const float* xy, const float* rgbd,
-                        const float* con, const void* rec, float3 bg, uint8_t* out,
-                        unsigned* queue, hipStream_t s) {
-  if (ranges_mode) {
-    if (gathered)
-      return dispatch_dt<PREC, true, true>(dt, keys, d_n, n_host, ranges, W, H, order, xy, rgbd,
-                                           con, rec, bg, out, queue, s);
-    return dispatch_dt<PREC, true, false>(dt, keys, d_n, n_host, ranges, W, H, order, xy, rgbd,
-                                          con, rec, bg, out, queue, s);
+int dispatch_persistent(bool gathered, bool dt, const PersistArgs& a, hipStream_t s) {
+  if (a.ranges != nullptr || a.tile_desc != nullptr) {
+    if (gathered) return dispatch_dt<PREC, true, true>(dt, a, s);
+    return dispatch_dt<PREC, true, false>(dt, a, s);
   }
-  if (gathered)
-    return dispatch_dt<PREC, false, true>(dt, keys, d_n, n_host, ranges, W, H, order, xy, rgbd,
-                                          con, rec, bg, out, queue, s);
-  return dispatch_dt<PREC, false, false>(dt, keys, d_n, n_host, ranges, W, H, order, xy, rgbd, con,
-                                         rec, bg, out, queue, s);
+  if (gathered) return dispatch_dt<PREC, false, true>(dt, a, s);
+  return dispatch_dt<PREC, false, false>(dt, a, s);
 }
 
 }  // namespace
@@ -757,17 +821,22 @@ int render_batch_launch(const FrameDev* frames_dev, int nf, int feat_prec, const
 int render_plain_launch(int n_host, const int* d_n, int W, int H, const float* xy,
                         const float* rgbd, const float* con, const uint64_t* keys,
                         const uint32_t* order, const float bg[3], uint8_t* out,
-                        hipStream_t stream) {
+                        hipStream_t stream, const int* ranges, const uint32_t* tile_order,
+                        const int* tile_desc) {
   PS_REQUIRE(W > 0 && H > 0, "bad image size");
   PS_REQUIRE(out != nullptr && bg != nullptr, "output or background is NULL");
   PS_REQUIRE(d_n != nullptr || n_host >= 0, "negative num_rendered");
+  PS_REQUIRE(ranges != nullptr || keys != nullptr || tile_desc != nullptr,
+             "need keys_sorted, tile_ranges or tile descriptors");
   const int gx = (W + kTile - 1) / kTile, gy = (H + kTile - 1) / kTile;
   const float3 b = make_float3(bg[0], bg[1], bg[2]);
   // Saturation test every 2 Gaussians, as the reference's plain path does
   // (render.cu:2436-2447).
   hipLaunchKernelGGL((ps_render_plain<kFeatF32, 2>), dim3(gx, gy), dim3(kWave), 0, stream, keys, d_n,
                      n_host, W, H, gx, order, (const float2*)xy, (const float4*)rgbd,
-                     (const float4*)con, (const FeatRec16*)nullptr, b, out);
+                     (const float4*)con, (const FeatRec16*)nullptr, b, out,
+                     reinterpret_cast<const int2*>(ranges), tile_order,
+                     reinterpret_cast<const int4*>(tile_desc));
   PS_LAUNCH_CHECK();
   return PS_OK;
 }
@@ -776,27 +845,27 @@ int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_
                              const float* xy, const float* rgbd, const float* con, const void* rec,
                              int gathered, const uint64_t* keys, const uint32_t* order,
                              const int* ranges, int dt, const float bg[3], uint8_t* out,
-                             unsigned* queue, hipStream_t stream) {
+                             unsigned* queue, hipStream_t stream, const uint32_t* tile_order,
+                             const int* tile_desc) {
   PS_REQUIRE(W > 0 && H > 0, "bad image size");
   PS_REQUIRE(out != nullptr && bg != nullptr && queue != nullptr, "NULL output/bg/queue");
-  PS_REQUIRE(ranges != nullptr || keys != nullptr, "need tile_ranges or keys_sorted");
+  PS_REQUIRE(ranges != nullptr || keys != nullptr || tile_desc != nullptr,
+             "need tile_ranges, keys_sorted or tile descriptors");
   PS_REQUIRE(gathered || order != nullptr, "values_sorted is NULL");
   PS_REQUIRE(d_n != nullptr || n_host >= 0, "negative num_rendered");
-  const float3 b = make_float3(bg[0], bg[1], bg[2]);
-  const bool rm = ranges != nullptr, g = gathered != 0, d = dt != 0;
+  PersistArgs a{keys, d_n,  n_host, ranges, W, H, order, xy, rgbd, con, rec,
+                make_float3(bg[0], bg[1], bg[2]), out, queue, tile_order, tile_desc};
+  const bool g = gathered != 0, d = dt != 0;
   switch (feat_prec) {
     case PS_PREC_F32:
       PS_REQUIRE(xy && rgbd && con, "f32 feature arrays are NULL");
-      return dispatch_persistent<kFeatF32>(rm, g, d, keys, d_n, n_host, ranges, W, H, order, xy,
-                                           rgbd, con, rec, b, out, queue, stream);
+      return dispatch_persistent<kFeatF32>(g, d, a, stream);
     case PS_PREC_F16:
       PS_REQUIRE(rec != nullptr, "records are NULL");
-      return dispatch_persistent<kFeatF16>(rm, g, d, keys, d_n, n_host, ranges, W, H, order, xy,
-                                           rgbd, con, rec, b, out, queue, stream);
+      return dispatch_persistent<kFeatF16>(g, d, a, stream);
     case PS_PREC_BF16:
       PS_REQUIRE(rec != nullptr, "records are NULL");
-      return dispatch_persistent<kFeatBF16>(rm, g, d, keys, d_n, n_host, ranges, W, H, order, xy,
-                                            rgbd, con, rec, b, out, queue, stream);
+      return dispatch_persistent<kFeatBF16>(g, d, a, stream);
     default:
       set_last_error("unknown feature precision %d", feat_prec);
       return PS_ERR_INVALID;
@@ -889,4 +958,45 @@ extern "C" int ps_render_16x16_persistent_noalloc(
                                        points_xy, rgb_depth, conic_opacity, records, gathered,
                                        keys_sorted, values_sorted, tile_ranges, dt, bg, out_color,
                                        reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int ps_render_16x16_reordered(int num_rendered, const int* d_num_rendered, int width,
+                                         int height, const float* points_xy,
+                                         const float* rgb_depth, const float* conic_opacity,
+                                         const uint64_t* keys_sorted,
+                                         const uint32_t* values_sorted, const int* tile_ranges,
+                                         const uint32_t* tile_order, const float bg[3],
+                                         uint8_t* out_color, ps_stream_t stream) {
+  PS_REQUIRE(tile_ranges != nullptr && tile_order != nullptr, "tile_ranges or tile_order is NULL");
+  return ps::render_plain_launch(num_rendered, d_num_rendered, width, height, points_xy, rgb_depth,
+                                 conic_opacity, keys_sorted, values_sorted, bg, out_color,
+                                 reinterpret_cast<hipStream_t>(stream), tile_ranges, tile_order);
+}
+
+extern "C" int ps_render_16x16_reordered_persistent(
+    int num_rendered, const int* d_num_rendered, int width, int height, int feat_prec,
+    const float* points_xy, const float* rgb_depth, const float* conic_opacity,
+    const void* records, const uint32_t* values_sorted, const int* tile_ranges,
+    const uint32_t* tile_order, const int* tile_desc, int dt, const float bg[3],
+    uint8_t* out_color, unsigned* queue, ps_stream_t stream) {
+  PS_REQUIRE(tile_desc != nullptr || (tile_ranges != nullptr && tile_order != nullptr),
+             "need tile_desc, or tile_ranges and tile_order");
+  return ps::render_persistent_launch(num_rendered, d_num_rendered, width, height, feat_prec,
+                                      points_xy, rgb_depth, conic_opacity, records, 0, nullptr,
+                                      values_sorted, tile_desc ? nullptr : tile_ranges, dt, bg,
+                                      out_color, queue, reinterpret_cast<hipStream_t>(stream),
+                                      tile_desc ? nullptr : tile_order, tile_desc);
+}
+
+extern "C" int ps_render_16x16_reordered_desc(int num_rendered, const int* d_num_rendered,
+                                              int width, int height, const float* points_xy,
+                                              const float* rgb_depth, const float* conic_opacity,
+                                              const uint32_t* values_sorted, const int* tile_desc,
+                                              const float bg[3], uint8_t* out_color,
+                                              ps_stream_t stream) {
+  PS_REQUIRE(tile_desc != nullptr, "tile_desc is NULL");
+  return ps::render_plain_launch(num_rendered, d_num_rendered, width, height, points_xy, rgb_depth,
+                                 conic_opacity, nullptr, values_sorted, bg, out_color,
+                                 reinterpret_cast<hipStream_t>(stream), nullptr, nullptr,
+                                 tile_desc);
 }

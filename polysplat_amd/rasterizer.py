@@ -15,6 +15,18 @@ render_variant                                 pipeline
 "preranges_smem_persistent_lite[_dt]"          stage-by-stage from Python: FP16 SH
                                                preprocess, host count, sort, ranges,
                                                persistent render (example.py:480-781)
+"reordered"                                    FP32 SH -> sort -> ranges -> tile order
+                                               (heaviest first) -> plain render in that
+                                               order (example.py:617-625, :698-704)
+"reordered_v2", "reordered_zigzag<G>"          same with packed tile descriptors, groups
+                                               of G positions zig-zagged (G = 132)
+"reordered_persistent[_zigzag<G>]",            FP32 SH -> ... -> packed descriptors ->
+"reordered_persistent_morton<B>[_zigzag<G>]",  persistent render over them (Morton-bucket
+"reordered_persistent_adaptive"                keys of bucket B; example.py:626-645,
+                                               :712-718, :234-243)
+"preranges_smem_persistent[_morton..|_zigzag..  FP16 SH -> ... -> the same ordered
+ |_adaptive]"                                  persistent render (example.py:326-359,
+                                               :646-669)
 "auto"                                         resolves to the fastest of the above
 =============================================  ==========================================
 
@@ -34,8 +46,11 @@ M > rho * P, thresholds measured on MI355X; environment variables
 POLYSPLAT_ES_THRESHOLD_M and POLYSPLAT_ES_RATIO_THRESH override them,
 POLYSPLAT_ES_FORCE_TWO_PASS=1 forces it), never on a first frame.
 
-Other variants of the reference (top-k caches, tile reordering, split warps)
-are not part of this build and raise ValueError.
+The reordered variants change only the order in which tiles are rendered;
+their images are identical to the unordered renderers' (the persistent ones
+render 24-byte records when ``feature_dtype`` is fp16/bf16, the plain ones
+always FP32 features).  Other variants of the reference (top-k caches, split
+warps, other tile shapes) are not part of this build and raise ValueError.
 """
 from __future__ import annotations
 
@@ -64,6 +79,60 @@ DEPTH_FIRST = 0x100  # PS_FORWARD_DEPTH_FIRST: OR-ed into a ps_forward variant
 # 2% slower at 6.9 and level (within 2%) from 10 to 50.
 ES_THRESHOLD_M = 2_000_000
 ES_RATIO_THRESH = 10.0
+
+
+# Tile count up to which "..._adaptive" takes Morton-bucket keys (bucket
+# 1280, zig-zag 792) rather than count keys zig-zagged by 264
+# (example.py:53, :234-243).
+ADAPTIVE_TRUCK_TILE_THRESHOLD = 16384
+
+
+def _int_or(text, default):
+    return int(text) if text else default
+
+
+def tile_order_plan(variant, tiles):
+    """(renderer, zigzag group, Morton bucket) of a tile-reordering variant,
+    or None: renderer "order" (plain over tile_order), "desc" (plain over
+    packed descriptors) or "persistent" (persistent over packed
+    descriptors).  Suffix grammar and defaults of example.py:304-389."""
+    def adaptive():
+        if tiles <= ADAPTIVE_TRUCK_TILE_THRESHOLD:
+            return ("persistent", 792, 1280)
+        return ("persistent", 264, 0)
+
+    def morton_suffix(rest):
+        if "_zigzag" in rest:
+            b, g = rest.split("_zigzag", 1)
+            return ("persistent", _int_or(g, 792), _int_or(b, 1280))
+        return ("persistent", 0, _int_or(rest, 1280))
+
+    if variant == "reordered":
+        return ("order", 0, 0)
+    if variant == "reordered_v2":
+        return ("desc", 0, 0)
+    if variant.startswith("reordered_zigzag"):
+        return ("desc", _int_or(variant[len("reordered_zigzag"):], 132), 0)
+    if variant == "reordered_persistent_adaptive":
+        return adaptive()
+    if variant.startswith("reordered_persistent"):
+        rest = variant[len("reordered_persistent"):]
+        if rest == "":
+            return ("persistent", 0, 0)
+        if rest.startswith("_zigzag"):
+            return ("persistent", _int_or(rest[len("_zigzag"):], 264), 0)
+        if rest.startswith("_morton"):
+            return morton_suffix(rest[len("_morton"):])
+        return None
+    if variant.startswith("preranges_smem_persistent") and "_lite" not in variant:
+        rest = variant[len("preranges_smem_persistent"):]
+        if rest in ("", "_adaptive"):
+            return adaptive()
+        if rest.startswith("_zigzag"):
+            return ("persistent", _int_or(rest[len("_zigzag"):], 264), 0)
+        if rest.startswith("_morton"):
+            return morton_suffix(rest[len("_morton"):])
+    return None
 
 
 def _env_flag(name):
@@ -171,6 +240,14 @@ class Rasterizer:
         tk = self.gaussian_keys_unsorted.view(torch.int32)
         self._es_tile_keys_unsorted = tk[:M]
         self._es_tile_keys_sorted = tk[M:]
+        # Tile reordering buffers (example.py:128-136).
+        T = self.MAX_NUM_TILES
+        self.tile_order = torch.zeros(T, device=dev, dtype=torch.int32)
+        self.tile_counts_buf = torch.zeros(T, device=dev, dtype=torch.int32)
+        self.tile_ids_buf = torch.zeros(T, device=dev, dtype=torch.int32)
+        self.tile_order_sort_temp = torch.zeros(ops.get_tile_order_sort_temp_size(T), device=dev,
+                                                dtype=torch.int8)
+        self.tile_desc_buf = torch.zeros((T, 4), device=dev, dtype=torch.int32)
 
     def _get_out_color(self, height, width, device):
         if (self._out_color is None or self._out_color.shape[0] != height
@@ -216,13 +293,20 @@ class Rasterizer:
         if render_variant.startswith("preranges_smem_persistent_lite_e2e") and measuring:
             # E2E calls cannot expose per-stage events (example.py:362-368).
             render_variant = "preranges_smem_persistent_lite"
-        if render_variant not in VARIANT_IDS and render_variant not in STAGED:
+        tiles = ((camera.width + 15) // 16) * ((camera.height + 15) // 16)
+        order_plan = tile_order_plan(render_variant, tiles)
+        if (render_variant not in VARIANT_IDS and render_variant not in STAGED
+                and order_plan is None):
             raise ValueError(f"render_variant {render_variant!r} is not implemented")
         self._check_tiles(camera, block_x, block_y)
         es = self._decide_use_es(scene, use_es)
         out_color = self._get_out_color(camera.height, camera.width, scene.device)
         times = {"preprocess_ms": None, "sort_ms": None, "render_ms": None}
-        if render_variant in VARIANT_IDS and not (render_variant == "default" and measuring):
+        if order_plan is not None:
+            num_rendered = self._forward_staged(scene, camera, bg_color, render_variant,
+                                                out_color, measure_preprocess, measure_sort,
+                                                measure_render, times, es, order_plan)
+        elif render_variant in VARIANT_IDS and not (render_variant == "default" and measuring):
             num_rendered = self._forward_fused(scene, camera, bg_color, render_variant, out_color,
                                                es)
         else:
@@ -267,11 +351,17 @@ class Rasterizer:
             bg_color, out_color, self.workspace, True)
 
     def _forward_staged(self, scene, camera, bg_color, render_variant, out_color,
-                        m_pre, m_sort, m_render, times, es=False):
+                        m_pre, m_sort, m_render, times, es=False, order_plan=None):
         ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
-        fp32_path = render_variant == "default"
+        # The plain renderers (default, and reordered over a tile order or
+        # descriptors) read FP32 features; the reordered variants take FP32
+        # SH (example.py:524-531), the preranges_smem_persistent ones FP16.
+        fp32_path = render_variant == "default" or (order_plan is not None
+                                                    and order_plan[0] != "persistent")
         prec = 0 if fp32_path else _PREC[self.feature_dtype]
-        shs = self._shs_for(scene, half=not fp32_path)
+        half_sh = not fp32_path and (order_plan is None
+                                     or render_variant.startswith("preranges_smem_persistent"))
+        shs = self._shs_for(scene, half=half_sh)
         args_cam = (camera.width, camera.height, 16, 16, camera.position, camera.rotation,
                     camera.focal_x, camera.focal_y, camera.zFar, camera.zNear)
         self.curr_offset.fill_(0)
@@ -313,12 +403,17 @@ class Rasterizer:
                               self.gaussian_values_sorted)
         if m_sort:
             e[3].record()
-        if not fp32_path:
+        if not fp32_path or order_plan is not None:
             ops.precompute_tile_ranges(num_rendered, camera.width, camera.height, 16, 16,
                                        self.gaussian_keys_sorted, self.tile_ranges_buf)
+        if order_plan is not None:
+            self._tile_order(camera, order_plan)
         if m_render:
             e[4].record()
-        if fp32_path:
+        if order_plan is not None:
+            self._render_reordered(num_rendered, camera, bg_color, out_color, order_plan, prec,
+                                   render_variant.endswith("_dt"))
+        elif fp32_path:
             ops.render_16x16(num_rendered, camera.width, camera.height, self.points_xy,
                              self.rgb_depth, self.conic_opacity, self.gaussian_keys_sorted,
                              self.gaussian_values_sorted, bg_color, out_color)
@@ -347,3 +442,46 @@ class Rasterizer:
         if m_render:
             times["render_ms"] = e[4].elapsed_time(e[5])
         return num_rendered
+
+    def _tile_order(self, camera, plan):
+        """Tile order or packed descriptors of the frame (example.py:617-669)."""
+        kind, zigzag, bucket = plan
+        gx = (camera.width + 15) // 16
+        tiles = gx * ((camera.height + 15) // 16)
+        if kind == "order":
+            ops.compute_tile_order(self.tile_ranges_buf, tiles, self.tile_order,
+                                   self.tile_counts_buf, self.tile_ids_buf,
+                                   self.tile_order_sort_temp)
+        elif bucket > 0:
+            ops.compute_tile_order_packed_morton(self.tile_ranges_buf, tiles, gx,
+                                                 self.tile_desc_buf, self.tile_order,
+                                                 self.tile_counts_buf, self.tile_ids_buf,
+                                                 self.tile_order_sort_temp, zigzag, bucket)
+        else:
+            ops.compute_tile_order_packed(self.tile_ranges_buf, tiles, gx, self.tile_desc_buf,
+                                          self.tile_order, self.tile_counts_buf,
+                                          self.tile_ids_buf, self.tile_order_sort_temp, zigzag)
+
+    def _render_reordered(self, num_rendered, camera, bg_color, out_color, plan, prec, dt):
+        kind = plan[0]
+        W, H = camera.width, camera.height
+        if kind == "order":
+            ops.render_16x16_reordered(num_rendered, W, H, self.points_xy, self.rgb_depth,
+                                       self.conic_opacity, self.gaussian_keys_sorted,
+                                       self.gaussian_values_sorted, self.tile_ranges_buf,
+                                       self.tile_order, bg_color, out_color)
+        elif kind == "desc":
+            ops.render_16x16_reordered_v2(num_rendered, W, H, self.points_xy, self.rgb_depth,
+                                          self.conic_opacity, self.gaussian_keys_sorted,
+                                          self.gaussian_values_sorted, self.tile_desc_buf,
+                                          bg_color, out_color)
+        elif prec:
+            ops.render_records_reordered(num_rendered, W, H, self.records, prec,
+                                         self.gaussian_values_sorted, self.tile_desc_buf,
+                                         bg_color, out_color, dt)
+        else:
+            ops.render_16x16_reordered_persistent(num_rendered, W, H, self.points_xy,
+                                                  self.rgb_depth, self.conic_opacity,
+                                                  self.gaussian_keys_sorted,
+                                                  self.gaussian_values_sorted,
+                                                  self.tile_desc_buf, bg_color, out_color)

@@ -16,6 +16,7 @@ HEADER = ROOT / "include" / "polysplat_amd.h"
 
 PS_OK, PS_ERR_INVALID, PS_ERR_HIP, PS_ERR_OVERFLOW, PS_ERR_UNSUPPORTED = range(5)
 F32, F16, BF16 = 0, 1, 2
+PREC_IDS = {"fp32": F32, "fp16": F16, "bf16": BF16}
 V_DEFAULT, V_LITE, V_FUSED, V_DT = 0, 1, 2, 3
 DEPTH_FIRST = 0x100  # PS_FORWARD_DEPTH_FIRST, OR-ed into a variant
 MAX_BATCH_FRAMES = 64
@@ -93,6 +94,13 @@ def bind(L):
                                        I),
         "ps_render_16x16_persistent_noalloc": ([I, P, I, I, I, P, P, P, P, I, P, P, P, I, P, P,
                                                 P], I),
+        "ps_get_tile_order_sort_temp_size": ([I], S),
+        "ps_compute_tile_order": ([P, I, P, P, P, P, S, P], I),
+        "ps_compute_tile_order_packed": ([P, I, I, P, P, P, P, P, S, I, I, P], I),
+        "ps_render_16x16_reordered": ([I, P, I, I, P, P, P, P, P, P, P, P, P, P], I),
+        "ps_render_16x16_reordered_desc": ([I, P, I, I, P, P, P, P, P, P, P, P], I),
+        "ps_render_16x16_reordered_persistent": ([I, P, I, I, I, P, P, P, P, P, P, P, P, I, P,
+                                                  P, P, P], I),
         "ps_forward_workspace_size": ([I], S),
         "ps_forward": ([I, P, P, I, P, P, P, I, I, P, P, P, I, P, P], I),
         "ps_forward_wait": ([I, P], I),

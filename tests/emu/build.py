@@ -11,6 +11,8 @@ AddressSanitizer (load it with libasan preloaded).
 from __future__ import annotations
 
 import argparse
+import fcntl
+import os
 import subprocess
 import sys
 from concurrent.futures import ThreadPoolExecutor
@@ -19,12 +21,22 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent.parent
 CSRC = ROOT / "polysplat_amd" / "csrc"
-SOURCES = ["preprocess.hip", "sort.hip", "ranges_gather.hip", "render.hip", "forward.hip"]
+SOURCES = ["preprocess.hip", "sort.hip", "ranges_gather.hip", "render.hip", "forward.hip",
+           "tile_order.hip"]
 
 
 def build(asan: bool = False) -> Path:
+    """Builds under an exclusive file lock (test processes running in
+    parallel each ask for the build) and renames the library into place, so
+    no process loads a half-written one."""
     out_dir = HERE / "_build"
     out_dir.mkdir(exist_ok=True)
+    with open(out_dir / ".lock", "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        return _build(out_dir, asan)
+
+
+def _build(out_dir: Path, asan: bool) -> Path:
     flags = ["-std=c++20", "-O1", "-g", "-fPIC", "-pthread", "-ffp-contract=off",
              f"-I{HERE / 'include'}", f"-I{ROOT / 'include'}", "-Wno-attributes",
              "-DPS_ENABLE_BATCH"]
@@ -42,8 +54,10 @@ def build(asan: bool = False) -> Path:
     with ThreadPoolExecutor(max_workers=4) as ex:
         objs = list(ex.map(lambda j: cc(*j), jobs))
     lib = out_dir / ("libpolysplat_emu_asan.so" if asan else "libpolysplat_emu.so")
+    tmp = lib.with_name(lib.name + f".tmp{os.getpid()}")
     subprocess.run(["g++", "-shared", "-pthread", *(["-fsanitize=address"] if asan else []),
-                    *map(str, objs), "-o", str(lib)], check=True)
+                    *map(str, objs), "-o", str(tmp)], check=True)
+    os.replace(tmp, lib)
     return lib
 
 

@@ -135,6 +135,49 @@ def test_c3_reduced_precision_persistent_path(c3, prec, depth_first):
           f"{m64['max_abs_diff']}", flush=True)
 
 
+def _check_reordered(g, img, M, prec, what):
+    """Tile orders of the frame bit-exact against the oracle, and the
+    persistent render over each (packed or plain order) identical to the
+    unordered render."""
+    L = capi.lib()
+    T, gx = g.tiles, (g.spec.width + 15) // 16
+    rg = g.ranges.cpu().numpy()
+    tb = L.ps_get_tile_order_sort_temp_size(T)
+    temp = torch.zeros(tb, dtype=torch.uint8, device=DEV)
+    order, counts, ids = (torch.zeros(T, dtype=torch.int32, device=DEV) for _ in range(3))
+    desc = torch.zeros(T, 4, dtype=torch.int32, device=DEV)
+    for zig, bucket in ((0, 0), (264, 0), (792, 1280)):
+        capi.check(L.ps_compute_tile_order_packed(
+            capi.ptr(g.ranges), T, gx, capi.ptr(desc), capi.ptr(order), capi.ptr(counts),
+            capi.ptr(ids), capi.ptr(temp), tb, zig, bucket, capi.stream()))
+        torch.cuda.synchronize()
+        o_order, o_keys, o_desc = O.tile_order(rg, gx, zig, bucket)
+        np.testing.assert_array_equal(order.cpu().numpy().view(np.uint32), o_order)
+        np.testing.assert_array_equal(counts.cpu().numpy().view(np.uint32), o_keys)
+        np.testing.assert_array_equal(desc.cpu().numpy(), o_desc)
+        for use_desc in (True, False):
+            g.out.fill_(9)
+            capi.check(L.ps_render_16x16_reordered_persistent(
+                M, None, g.spec.width, g.spec.height, capi.PREC_IDS[prec], capi.ptr(g.xy),
+                capi.ptr(g.rgbd), capi.ptr(g.conic), capi.ptr(g.records), capi.ptr(g.vs),
+                capi.ptr(g.ranges), capi.ptr(order), capi.ptr(desc) if use_desc else None, 0,
+                capi.floats3(BG), capi.ptr(g.out), capi.ptr(g.queue), capi.stream()))
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(g.out.cpu().numpy(), img,
+                                          err_msg=f"{what} reordered {zig} {bucket} {use_desc}")
+    say(what, "tile orders and reordered renders ok")
+
+
+def test_c3_tile_orders_and_reordered_render(c3):
+    arrays, host, spec, ref = c3
+    M = ref["pre"]["M"]
+    g = Gpu(arrays, spec, sh_type="fp16", feat_prec="fp16", capacity=M + 4096)
+    st, n, img = g.forward(BG, capi.V_LITE)
+    capi.check(st)
+    assert n == M
+    _check_reordered(g, img, M, "fp16", "c3")
+
+
 # ------------------------------------------------------------------ C4
 @pytest.fixture(scope="module")
 def c4(gpu):
@@ -167,6 +210,8 @@ def test_c4_city_bit_exact_and_image(c4, depth_first):
     np.testing.assert_array_equal(rng, O.tile_ranges(ref["keys_sorted"], g.tiles, 1))
     m64 = _quality(img, ref, ref["image64"], "c4 fp16 persistent")
     print(f"c4 city: {M} pairs, PSNR {m64['psnr_db']:.2f} dB vs FP64", flush=True)
+    if not depth_first:
+        _check_reordered(g, img, M, "fp16", "c4")
 
 
 # ------------------------------------------------------------------ C5
