@@ -340,6 +340,29 @@ int ps_forward(int P, const float* positions, const void* shs, int sh_prec,
                int variant, int feat_prec, const ps_frame_buffers* fb, const float bg[3],
                uint8_t* out_color, int sync_count, int* num_rendered_out, ps_stream_t stream);
 
+/* ------------------------------------------------------------ frame graphs
+ * One frame's whole pipeline (what ps_forward enqueues for this scene,
+ * variant, feature precision, buffers and output) captured once into a HIP
+ * graph and replayed per request with a new camera pose and background.
+ * The graph is tied to cam->width x cam->height and to every pointer given
+ * at creation; the kernels read the camera and background from 512 bytes
+ * of the forward workspace's queue area, which ps_frame_graph_launch
+ * rewrites on the stream before each replay (with a one-wave kernel, so the
+ * call allocates nothing and copies nothing from the host).  Replays of one
+ * graph, and ps_forward calls with the same buffers, must not overlap:
+ * issue them on one stream.  sync_count and num_rendered_out as ps_forward.
+ * No counterpart in the reference (its per-frame host sync, pybind.cpp:
+ * 1122-1125, rules out capture); for serving loops that replay per
+ * (scene, resolution). */
+typedef struct ps_frame_graph ps_frame_graph;
+int ps_frame_graph_create(int P, const float* positions, const void* shs, int sh_prec,
+                          const float* opacities, const float* cov3d, const ps_camera* cam,
+                          int variant, int feat_prec, const ps_frame_buffers* fb,
+                          uint8_t* out_color, ps_frame_graph** out);
+int ps_frame_graph_launch(ps_frame_graph* graph, const ps_camera* cam, const float bg[3],
+                          int sync_count, int* num_rendered_out, ps_stream_t stream);
+int ps_frame_graph_destroy(ps_frame_graph* graph);
+
 /* Count of a frame posted with ps_forward(sync_count == 2): waits for the
  * frame's copy, writes its pair count to *num_rendered_out and returns
  * PS_ERR_OVERFLOW when its pairs did not fit (nothing was rendered then).

@@ -19,7 +19,7 @@ int preprocess_launch(int P, const float* positions, const void* shs, int sh_pre
                       float* points_xy, float* rgb_depth, float* conic_opacity, void* feat_records,
                       int feat_prec, uint64_t* keys, uint32_t* values, int capacity,
                       int* curr_offset, void* workspace, size_t workspace_bytes,
-                      hipStream_t stream, bool reset_offset);
+                      hipStream_t stream, bool reset_offset, const CamParams* cam_dev = nullptr);
 int sort_end_bit(int width, int height, int block_x, int block_y);
 int preprocess_es_launch(int P, const float* positions, const void* shs, int sh_prec,
                          const float* opacities, const float* cov3d, const ps_camera* cam,
@@ -27,7 +27,7 @@ int preprocess_es_launch(int P, const float* positions, const void* shs, int sh_
                          void* feat_records, int feat_prec, uint32_t* tiles, uint32_t* vals,
                          int capacity, int* curr_offset, void* workspace, size_t workspace_bytes,
                          void* sort_space, size_t sort_space_bytes, int sort_max_pairs,
-                         hipStream_t stream);
+                         hipStream_t stream, const CamParams* cam_dev = nullptr);
 
 FrameDev make_frame_dev(int P, const float* positions, const void* shs, const float* opacities,
                         const float* cov3d, const ps_camera* cam, float* xy, float* rgbd,
@@ -300,15 +300,18 @@ extern "C" size_t ps_forward_workspace_size(int P) {
   return ps::kQueueBytes + ps::align_up(ps_preprocess_workspace_size(P), 256);
 }
 
-extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh_prec,
-                          const float* opacities, const float* cov3d, const ps_camera* cam,
-                          int variant, int feat_prec, const ps_frame_buffers* fb,
-                          const float bg[3], uint8_t* out_color, int sync_count,
-                          int* num_rendered_out, ps_stream_t stream_) {
-  using namespace ps;
-  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+namespace ps {
+namespace {
+
+// Checks and enqueues one frame (no count readback).  cam_dev / bg_dev:
+// NULL, or the device copies of the camera and background that the kernels
+// read instead of their arguments (a captured graph's frames).
+int forward_enqueue(int P, const float* positions, const void* shs, int sh_prec,
+                    const float* opacities, const float* cov3d, const ps_camera* cam, int variant,
+                    int feat_prec, const ps_frame_buffers* fb, const float bg[3],
+                    uint8_t* out_color, hipStream_t stream, const CamParams* cam_dev,
+                    const float* bg_dev) {
   PS_REQUIRE(fb != nullptr && cam != nullptr && bg != nullptr, "NULL argument");
-  PS_REQUIRE(sync_count >= 0 && sync_count <= 2, "sync_count must be 0, 1 or 2");
   const bool depth_first = (variant & PS_FORWARD_DEPTH_FIRST) != 0;
   variant &= ~PS_FORWARD_DEPTH_FIRST;
   PS_REQUIRE(variant >= PS_VARIANT_DEFAULT && variant <= PS_VARIANT_PERSISTENT_DT,
@@ -327,11 +330,6 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
       variant == PS_VARIANT_PERSISTENT_LITE || variant == PS_VARIANT_PERSISTENT_DT;
   PS_REQUIRE(!need_ranges || fb->tile_ranges != nullptr, "tile_ranges is NULL");
 
-  int slot = -1;
-  if (sync_count == 2) {
-    const int fs = free_post_slot(&slot);
-    if (fs != PS_OK) return fs;
-  }
   // The queue words sit at the start of the workspace, where no other stage
   // writes, so a workspace reused for scenes of other sizes keeps them zero.
   unsigned* queue = static_cast<unsigned*>(fb->workspace);
@@ -346,7 +344,6 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
     // of keys_unsorted's memory), stably sorted by tile id (the sorted ids
     // in its second half); the 64-bit sorted keys (tile id << 32) and the
     // tile ranges from that sort.
-    PS_REQUIRE(fb->tile_ranges != nullptr || !need_ranges, "tile_ranges is NULL");
     uint32_t* tk = reinterpret_cast<uint32_t*>(fb->keys_unsorted);
     uint32_t* tk_sorted = tk + fb->capacity;
     st = preprocess_es_launch(P, positions, shs, sh_prec, opacities, cov3d, cam,
@@ -354,7 +351,7 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
                               f32 ? fb->conic_opacity : nullptr, f32 ? nullptr : fb->records,
                               feat_prec, tk, fb->values_unsorted, fb->capacity, fb->curr_offset,
                               ws, pre_bytes, fb->sort_space, fb->sort_space_bytes, fb->capacity,
-                              stream);
+                              stream, cam_dev);
     if (st != PS_OK) return st;
     st = tile_sort_launch(0, d_n, fb->capacity, tiles, fb->sort_space, fb->sort_space_bytes, tk,
                           fb->values_unsorted, tk_sorted, fb->values_sorted, fb->keys_sorted,
@@ -367,7 +364,7 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
                            f32 ? fb->points_xy : nullptr, f32 ? fb->rgb_depth : nullptr,
                            f32 ? fb->conic_opacity : nullptr, f32 ? nullptr : fb->records,
                            feat_prec, fb->keys_unsorted, fb->values_unsorted, fb->capacity,
-                           fb->curr_offset, ws, pre_bytes, stream, true);
+                           fb->curr_offset, ws, pre_bytes, stream, true, cam_dev);
     if (st != PS_OK) return st;
     // The sort leaves the tile ranges in tile_ranges when the renderer needs
     // them (its segment pass computes them anyway).
@@ -378,29 +375,33 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
     if (st != PS_OK) return st;
   }
   if (variant == PS_VARIANT_DEFAULT) {
-    st = render_plain_launch(0, d_n, cam->width, cam->height, fb->points_xy, fb->rgb_depth,
-                             fb->conic_opacity, fb->keys_sorted, fb->values_sorted, bg, out_color,
-                             stream);
-  } else {
-    st = render_persistent_launch(0, d_n, cam->width, cam->height, feat_prec, fb->points_xy,
+    return render_plain_launch(0, d_n, cam->width, cam->height, fb->points_xy, fb->rgb_depth,
+                               fb->conic_opacity, fb->keys_sorted, fb->values_sorted, bg,
+                               out_color, stream, nullptr, nullptr, nullptr, bg_dev);
+  }
+  return render_persistent_launch(0, d_n, cam->width, cam->height, feat_prec, fb->points_xy,
                                   fb->rgb_depth, fb->conic_opacity, fb->records, 0,
                                   fb->keys_sorted, fb->values_sorted,
                                   need_ranges ? fb->tile_ranges : nullptr,
                                   variant == PS_VARIANT_PERSISTENT_DT, bg, out_color, queue,
-                                  stream);
-  }
-  if (st != PS_OK) return st;
+                                  stream, nullptr, nullptr, bg_dev);
+}
+
+// The end-of-frame count: none (sync_count 0), waited for (1), or posted to
+// a pinned slot whose ticket goes to *num_rendered_out (2; `slot` reserved).
+int finish_count(const ps_frame_buffers* fb, int sync_count, int slot, int* num_rendered_out,
+                 hipStream_t stream) {
+  const int* status = ps_preprocess_status_ptr(static_cast<char*>(fb->workspace) + kQueueBytes);
   if (sync_count == 2) {
-    // Posted: the count is read back later through ps_forward_wait.
-    st = post_count(slot, fb->curr_offset, ps_preprocess_status_ptr(ws), fb->capacity, stream);
+    const int st = post_count(slot, fb->curr_offset, status, fb->capacity, stream);
     if (st != PS_OK) return st;
     if (num_rendered_out) *num_rendered_out = slot;
     return PS_OK;
   }
   if (sync_count) {
-    const int* ptrs[2] = {fb->curr_offset, ps_preprocess_status_ptr(ws)};
+    const int* ptrs[2] = {fb->curr_offset, status};
     int vals[2] = {0, 0};
-    st = read_back(ptrs, 2, vals, stream);
+    const int st = read_back(ptrs, 2, vals, stream);
     if (st != PS_OK) return st;
     if (num_rendered_out) *num_rendered_out = vals[0];
     if (vals[1] & 1) {
@@ -408,6 +409,173 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
       return PS_ERR_OVERFLOW;
     }
   }
+  return PS_OK;
+}
+
+}  // namespace
+}  // namespace ps
+
+extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh_prec,
+                          const float* opacities, const float* cov3d, const ps_camera* cam,
+                          int variant, int feat_prec, const ps_frame_buffers* fb,
+                          const float bg[3], uint8_t* out_color, int sync_count,
+                          int* num_rendered_out, ps_stream_t stream_) {
+  using namespace ps;
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+  PS_REQUIRE(fb != nullptr, "NULL argument");
+  PS_REQUIRE(sync_count >= 0 && sync_count <= 2, "sync_count must be 0, 1 or 2");
+  int slot = -1;
+  if (sync_count == 2) {
+    const int fs = free_post_slot(&slot);
+    if (fs != PS_OK) return fs;
+  }
+  const int st = forward_enqueue(P, positions, shs, sh_prec, opacities, cov3d, cam, variant,
+                                 feat_prec, fb, bg, out_color, stream, nullptr, nullptr);
+  if (st != PS_OK) return st;
+  return finish_count(fb, sync_count, slot, num_rendered_out, stream);
+}
+
+// ------------------------------------------------------------ frame graphs
+// A frame's whole pipeline (every kernel ps_forward enqueues) captured once
+// into a HIP graph, for one scene, resolution, variant and set of buffers,
+// and replayed per request.  The kernels take their camera and background
+// from a 512-byte block in the forward workspace's queue area (bytes
+// 1536..2047, which the tile queue does not use) instead of their
+// arguments, so a replay needs only that block rewritten first: a one-wave
+// kernel whose arguments carry the new camera does it, on the same stream.
+// Host work per frame: two launches instead of about twenty.
+struct ps_frame_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int width = 0, height = 0;
+  ps_frame_buffers fb;
+  ps::CamParams* cam_dev = nullptr;
+  float* bg_dev = nullptr;
+};
+
+namespace ps {
+namespace {
+
+constexpr size_t kFrameConstsOffset = 1536;
+static_assert(kFrameConstsOffset + sizeof(CamParams) + 4 * sizeof(float) <= kQueueBytes,
+              "camera block fits the queue area");
+
+// Camera and background of the next replay into device memory.
+__global__ __launch_bounds__(64) void ps_set_frame_consts(CamParams* __restrict__ dst,
+                                                          float* __restrict__ bg_dst,
+                                                          CamParams cp, float3 bg) {
+  const int l = threadIdx.x;
+  constexpr int kWords = (int)(sizeof(CamParams) / sizeof(uint32_t));
+  static_assert(kWords <= 64, "one word per lane");
+  if (l < kWords) reinterpret_cast<uint32_t*>(dst)[l] = reinterpret_cast<const uint32_t*>(&cp)[l];
+  if (l == 0) {
+    bg_dst[0] = bg.x;
+    bg_dst[1] = bg.y;
+    bg_dst[2] = bg.z;
+  }
+}
+
+int set_frame_consts(const ps_frame_graph* g, const ps_camera* cam, const float bg[3],
+                     hipStream_t stream) {
+  const CamParams cp = make_cam_params(cam);
+  hipLaunchKernelGGL(ps_set_frame_consts, dim3(1), dim3(64), 0, stream, g->cam_dev, g->bg_dev, cp,
+                     make_float3(bg[0], bg[1], bg[2]));
+  PS_LAUNCH_CHECK();
+  return PS_OK;
+}
+
+}  // namespace
+}  // namespace ps
+
+extern "C" int ps_frame_graph_create(int P, const float* positions, const void* shs, int sh_prec,
+                                     const float* opacities, const float* cov3d,
+                                     const ps_camera* cam, int variant, int feat_prec,
+                                     const ps_frame_buffers* fb, uint8_t* out_color,
+                                     ps_frame_graph** out) {
+  using namespace ps;
+  PS_REQUIRE(out != nullptr && fb != nullptr && cam != nullptr, "NULL argument");
+  *out = nullptr;
+  PS_REQUIRE(fb->workspace != nullptr && fb->workspace_bytes >= ps_forward_workspace_size(P),
+             "forward workspace too small");
+  ps_frame_graph* g = new ps_frame_graph();
+  g->width = cam->width;
+  g->height = cam->height;
+  g->fb = *fb;
+  char* qa = static_cast<char*>(fb->workspace) + kFrameConstsOffset;
+  g->cam_dev = reinterpret_cast<CamParams*>(qa);
+  g->bg_dev = reinterpret_cast<float*>(qa + sizeof(CamParams));
+  // Capture on a stream of the library's own (captures need a created
+  // stream; the caller's may be the null stream).  Default priority.
+  hipStream_t cs = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete g;
+    set_last_error("hipStreamCreateWithFlags failed: %s", hipGetErrorString(e));
+    return PS_ERR_HIP;
+  }
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
+  int st = PS_OK;
+  if (e != hipSuccess) {
+    set_last_error("hipStreamBeginCapture failed: %s", hipGetErrorString(e));
+    st = PS_ERR_HIP;
+  } else {
+    const float bg0[3] = {0.f, 0.f, 0.f};
+    st = forward_enqueue(P, positions, shs, sh_prec, opacities, cov3d, cam, variant, feat_prec,
+                         fb, bg0, out_color, cs, g->cam_dev, g->bg_dev);
+    char saved[512];
+    snprintf(saved, sizeof(saved), "%s", ps_last_error());
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(cs, &graph);
+    g->graph = graph;
+    if (st != PS_OK) {
+      set_last_error("%s", saved);
+    } else if (e2 != hipSuccess || graph == nullptr) {
+      set_last_error("hipStreamEndCapture failed: %s", hipGetErrorString(e2));
+      st = PS_ERR_HIP;
+    } else {
+      const hipError_t e3 = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+      if (e3 != hipSuccess) {
+        set_last_error("hipGraphInstantiate failed: %s", hipGetErrorString(e3));
+        st = PS_ERR_HIP;
+      }
+    }
+  }
+  (void)hipStreamDestroy(cs);
+  if (st != PS_OK) {
+    ps_frame_graph_destroy(g);
+    return st;
+  }
+  *out = g;
+  return PS_OK;
+}
+
+extern "C" int ps_frame_graph_launch(ps_frame_graph* g, const ps_camera* cam, const float bg[3],
+                                     int sync_count, int* num_rendered_out, ps_stream_t stream_) {
+  using namespace ps;
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+  PS_REQUIRE(g != nullptr && g->exec != nullptr && cam != nullptr && bg != nullptr,
+             "NULL argument");
+  PS_REQUIRE(cam->width == g->width && cam->height == g->height && cam->block_x == 16 &&
+                 cam->block_y == 16,
+             "the graph renders %dx%d frames with 16x16 tiles (camera: %dx%d, %dx%d tiles)",
+             g->width, g->height, cam->width, cam->height, cam->block_x, cam->block_y);
+  PS_REQUIRE(sync_count >= 0 && sync_count <= 2, "sync_count must be 0, 1 or 2");
+  int slot = -1;
+  if (sync_count == 2) {
+    const int fs = free_post_slot(&slot);
+    if (fs != PS_OK) return fs;
+  }
+  int st = set_frame_consts(g, cam, bg, stream);
+  if (st != PS_OK) return st;
+  PS_HIP_CHECK(hipGraphLaunch(g->exec, stream));
+  return finish_count(&g->fb, sync_count, slot, num_rendered_out, stream);
+}
+
+extern "C" int ps_frame_graph_destroy(ps_frame_graph* g) {
+  if (g == nullptr) return PS_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
   return PS_OK;
 }
 

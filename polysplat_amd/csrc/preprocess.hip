@@ -518,6 +518,7 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_count(
     int* __restrict__ state, const int* __restrict__ curr_offset, int reset_offset,
     uint64_t* __restrict__ pool, long long pool_cap, long long pool_overflow,
     uint32_t* __restrict__ dkey, uint32_t* __restrict__ plan) {
+  bind_camera(F);
   pre_count_body<SHP>(F, blockIdx.x, counts, block_sums, recs, chunk_sums, state, curr_offset,
                       reset_offset, pool, pool_cap, pool_overflow, dkey, plan);
 }
@@ -637,6 +638,7 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_emit(FrameDev F, const int
                                                            const int* __restrict__ state,
                                                            uint64_t* __restrict__ keys,
                                                            uint32_t* __restrict__ vals) {
+  bind_camera(F);
   pre_emit_body(F, blockIdx.x, counts, block_offsets, state, keys, vals);
 }
 
@@ -805,6 +807,7 @@ __global__ __launch_bounds__(kPreThreads) void ps_pre_emit_rec(
     const unsigned long long* __restrict__ chunk_sums, const EmitRec* __restrict__ recs,
     const uint64_t* __restrict__ pool, int* __restrict__ state, int* __restrict__ curr_offset,
     int capacity, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  bind_camera(F);
   const int g = blockIdx.x * kPreThreads + threadIdx.x;
   const int count = g < F.P ? counts[g] : 0;
   // The record is loaded before the scan (a lane that emits nothing ignores
@@ -867,6 +870,7 @@ __global__ __launch_bounds__(kPreThreads) void ps_es_emit(
     const unsigned long long* __restrict__ chunk_sums, const EmitRec* __restrict__ recs,
     const uint64_t* __restrict__ pool, int* __restrict__ state, int* __restrict__ curr_offset,
     int capacity, uint32_t* __restrict__ tiles, uint32_t* __restrict__ vals) {
+  bind_camera(F);
   __shared__ uint32_t s_k;
   const DepthOrder ord = depth_order(plan, vb0, vb1, &s_k);
   const int s = blockIdx.x * kPreThreads + threadIdx.x;
@@ -1008,7 +1012,7 @@ int preprocess_launch(int P, const float* positions, const void* shs, int sh_pre
                       float* points_xy, float* rgb_depth, float* conic_opacity, void* feat_records,
                       int feat_prec, uint64_t* keys, uint32_t* values, int capacity,
                       int* curr_offset, void* workspace, size_t workspace_bytes,
-                      hipStream_t stream, bool reset_offset) {
+                      hipStream_t stream, bool reset_offset, const CamParams* cam_dev) {
   int st = check_frame_args(P, positions, shs, opacities, cov3d, cam);
   if (st != PS_OK) return st;
   PS_REQUIRE(sh_prec == PS_PREC_F32 || sh_prec == PS_PREC_F16 || sh_prec == PS_PREC_BF16,
@@ -1024,8 +1028,9 @@ int preprocess_launch(int P, const float* positions, const void* shs, int sh_pre
 
   PreWs w;
   pre_layout(P, static_cast<char*>(workspace), &w);
-  const FrameDev F = make_frame_dev(P, positions, shs, opacities, cov3d, cam, points_xy, rgb_depth,
-                                    conic_opacity, feat_records, feat_prec, nullptr, nullptr);
+  FrameDev F = make_frame_dev(P, positions, shs, opacities, cov3d, cam, points_xy, rgb_depth,
+                              conic_opacity, feat_records, feat_prec, nullptr, nullptr);
+  F.cam_dev = cam_dev;
   const int nb = blocks_for(P);
   if (nb == 0) {
     // No Gaussians: the scan kernel alone publishes the (unchanged) offset.
@@ -1084,7 +1089,7 @@ int preprocess_es_launch(int P, const float* positions, const void* shs, int sh_
                          void* feat_records, int feat_prec, uint32_t* tiles, uint32_t* vals,
                          int capacity, int* curr_offset, void* workspace, size_t workspace_bytes,
                          void* sort_space, size_t sort_space_bytes, int sort_max_pairs,
-                         hipStream_t stream) {
+                         hipStream_t stream, const CamParams* cam_dev) {
   int st = check_frame_args(P, positions, shs, opacities, cov3d, cam);
   if (st != PS_OK) return st;
   PS_REQUIRE(sh_prec == PS_PREC_F32 || sh_prec == PS_PREC_F16 || sh_prec == PS_PREC_BF16,
@@ -1095,8 +1100,9 @@ int preprocess_es_launch(int P, const float* positions, const void* shs, int sh_
              "preprocess workspace too small");
   PreWs w;
   pre_layout(P, static_cast<char*>(workspace), &w);
-  const FrameDev F = make_frame_dev(P, positions, shs, opacities, cov3d, cam, points_xy, rgb_depth,
-                                    conic_opacity, feat_records, feat_prec, nullptr, nullptr);
+  FrameDev F = make_frame_dev(P, positions, shs, opacities, cov3d, cam, points_xy, rgb_depth,
+                              conic_opacity, feat_records, feat_prec, nullptr, nullptr);
+  F.cam_dev = cam_dev;
   const int nb = blocks_for(P);
   if (nb == 0) {
     hipLaunchKernelGGL(ps_pre_scan, dim3(1), dim3(1024), 0, stream, 0, w.block_sums,
@@ -1292,5 +1298,5 @@ extern "C" int ps_preprocess(int P, const float* positions, const void* shs, int
   return ps::preprocess_launch(P, positions, shs, sh_prec, opacities, cov3d, cam, points_xy,
                                rgb_depth, conic_opacity, feat_records, feat_prec, keys_unsorted,
                                values_unsorted, capacity, curr_offset, workspace, workspace_bytes,
-                               reinterpret_cast<hipStream_t>(stream), false);
+                               reinterpret_cast<hipStream_t>(stream), false, nullptr);
 }

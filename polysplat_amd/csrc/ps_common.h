@@ -396,7 +396,15 @@ struct FrameDev {
   int tiles;         // number of 16x16 tiles of this frame
   float bg[3];
   CamParams cp;
+  // When set, the camera is read from device memory instead of `cp` (frames
+  // replayed from a captured graph, whose kernel arguments are fixed).
+  const CamParams* cam_dev;
 };
+
+// The frame's camera: `cp`, or the device copy a replayed graph points at.
+__device__ __forceinline__ void bind_camera(FrameDev& F) {
+  if (F.cam_dev) F.cp = *F.cam_dev;
+}
 
 // Largest f with key >= field(frames[f]) (frames sorted by that field).
 template <typename Field>

@@ -25,7 +25,10 @@
 
 #include <atomic>
 #include <functional>
+#include <map>
+#include <mutex>
 #include <tuple>
+#include <vector>
 #include <type_traits>
 #include <utility>
 
@@ -215,6 +218,15 @@ inline void __builtin_amdgcn_sched_barrier(int) {}
 // ---------------------------------------------------------------- runtime
 typedef struct ihipStream_t* hipStream_t;
 typedef struct ihipEvent_t* hipEvent_t;
+typedef struct emu_graph* hipGraph_t;
+typedef struct emu_graph* hipGraphExec_t;
+typedef void* hipGraphNode_t;
+typedef enum {
+  hipStreamCaptureModeGlobal = 0,
+  hipStreamCaptureModeThreadLocal = 1,
+  hipStreamCaptureModeRelaxed = 2
+} hipStreamCaptureMode;
+enum { hipStreamDefault = 0, hipStreamNonBlocking = 1 };
 typedef enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorNotReady = 600 } hipError_t;
 typedef enum {
   hipMemcpyHostToHost = 0,
@@ -226,7 +238,71 @@ typedef enum {
 enum { hipHostMallocDefault = 0, hipEventDisableTiming = 2 };
 typedef enum { hipDeviceAttributeMultiprocessorCount = 0 } hipDeviceAttribute_t;
 
-inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
+// Graph capture: launches on a capturing stream are recorded (kernel,
+// launch shape and a copy of the arguments) instead of run; a launched
+// graph runs them in order.  Copies and event records on a capturing
+// stream are refused, so a captured path that would record them fails its
+// tests here.
+struct emu_graph {
+  std::vector<std::function<void()>> steps;
+};
+namespace emu {
+inline std::mutex& capture_mu() {
+  static std::mutex m;
+  return m;
+}
+inline std::map<hipStream_t, emu_graph*>& captures() {
+  static std::map<hipStream_t, emu_graph*> m;
+  return m;
+}
+inline emu_graph* capturing(hipStream_t s) {
+  std::lock_guard<std::mutex> g(capture_mu());
+  auto it = captures().find(s);
+  return it == captures().end() ? nullptr : it->second;
+}
+}  // namespace emu
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
+  *s = reinterpret_cast<hipStream_t>(new char[1]);
+  return hipSuccess;
+}
+inline hipError_t hipStreamDestroy(hipStream_t s) {
+  delete[] reinterpret_cast<char*>(s);
+  return hipSuccess;
+}
+inline hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
+  if (s == nullptr || emu::capturing(s)) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> g(emu::capture_mu());
+  emu::captures()[s] = new emu_graph();
+  return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g) {
+  emu_graph* c = emu::capturing(s);
+  if (!c) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> l(emu::capture_mu());
+  emu::captures().erase(s);
+  *g = c;
+  return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t*, char*,
+                                      size_t) {
+  *e = new emu_graph(*g);
+  return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t s) {
+  if (emu::capturing(s)) return hipErrorInvalidValue;
+  for (auto& f : e->steps) f();
+  return hipSuccess;
+}
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
+  delete e;
+  return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) {
+  delete g;
+  return hipSuccess;
+}
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t st) {
+  if (emu::capturing(st)) return hipErrorInvalidValue;
   memcpy(d, s, n);
   return hipSuccess;
 }
@@ -250,7 +326,9 @@ inline hipError_t hipEventDestroy(hipEvent_t e) {
   delete[] reinterpret_cast<char*>(e);
   return hipSuccess;
 }
-inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t s) {
+  return emu::capturing(s) ? hipErrorInvalidValue : hipSuccess;
+}
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }  // launches run to completion
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
@@ -286,9 +364,16 @@ inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, K, int, s
 }
 
 template <typename... P, typename... A>
-inline void emu_launch(void (*kernel)(P...), dim3 grid, dim3 block, A&&... args) {
+inline void emu_launch(hipStream_t stream, void (*kernel)(P...), dim3 grid, dim3 block,
+                       A&&... args) {
   std::tuple<std::decay_t<P>...> t(static_cast<std::decay_t<P>>(args)...);
+  if (emu_graph* c = emu::capturing(stream)) {
+    c->steps.push_back([kernel, grid, block, t]() {
+      emu::run_grid(grid, block, [&]() { std::apply(kernel, t); });
+    });
+    return;
+  }
   emu::run_grid(grid, block, [&]() { std::apply(kernel, t); });
 }
 #define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) \
-  emu_launch(kernel, dim3(grid), dim3(block), ##__VA_ARGS__)
+  emu_launch(stream, kernel, dim3(grid), dim3(block), ##__VA_ARGS__)
