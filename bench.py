@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--serve-tight-budget-gib", type=float, default=1.5)
     ap.add_argument("--serve-rate", type=float, default=240.0)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving stream")
+    ap.add_argument("--serve-graphs", choices=("on", "off"), default="on",
+                    help="replay each (scene, resolution) pipeline from a captured HIP graph")
     ap.add_argument("--tile-orders", action="store_true",
                     help="only time the FP16 persistent render over each tile order "
                          "(row-major queue, heavy-first, zig-zag, Morton buckets) and the "
@@ -103,7 +105,7 @@ def serve(args):
     sc = SV.make_resident_scenes(SV.SCENE_SIZES, device="cuda:0")
     torch.cuda.synchronize()
     log(f"serving: {len(sc)} resident scenes generated in {time.time() - t:.1f}s")
-    ex = SV.make_executor(sc)
+    ex = SV.make_executor(sc, graphs=args.serve_graphs == "on")
     est = SV.Estimates()
     t = time.time()
     SV.calibrate(ex, est, len(sc))
@@ -124,7 +126,9 @@ def serve(args):
     res["executor"] = (f"{type(ex).__name__}: one stream, frames in arrival order, enqueued "
                        "back to back with up to "
                        f"{ex.slots_n} groups in flight; pair counts read back from pinned "
-                       "slots after each frame (no per-frame host wait)")
+                       "slots after each frame (no per-frame host wait)" +
+                       ("; each (scene, resolution) pipeline replayed from a captured HIP graph "
+                        f"({ex.captures} captures)" if ex.use_graphs else ""))
     res["rate_req_s"] = args.serve_rate
     res["scenes"] = list(SV.SCENE_SIZES)
     res["resolutions"] = [f"{w}x{h}" for w, h in SV.RESOLUTIONS]
@@ -339,6 +343,56 @@ def main():
     log(f"fp16_persistent, other key pipeline: {other_ms:.3f} ms/frame, "
         f"identical image: {pipelines['other_image_identical']}")
     frame_fn(rast, variant)(sync=True)  # leave the timed pipeline's buffers behind
+    # ---------------------------------------------------------- graph replay
+    # The same FP16 frame replayed from a captured graph (ops.FrameGraph):
+    # device time per frame back to back, and host time to enqueue one frame
+    # (eager: the ~20 launches of forward_frame; graph: the constants kernel
+    # and the graph launch), each the median over --steps frames.
+    graph_line = None
+    try:
+        out_g = torch.empty((H, W, 3), dtype=torch.uint8, device=dev)
+        shs_h = scene.shs_half
+        fg = ops.FrameGraph(scene.position, shs_h, scene.opacity, scene.cov3d, W, H,
+                            spec.position, spec.rotation, spec.focal_x, spec.focal_y, spec.zFar,
+                            spec.zNear, VAR[variant], 1, rast.curr_offset,
+                            rast.list_sorting_space, rast.gaussian_keys_unsorted,
+                            rast.gaussian_values_unsorted, rast.gaussian_keys_sorted,
+                            rast.gaussian_values_sorted, rast.points_xy, rast.rgb_depth,
+                            rast.conic_opacity, rast.records, rast.tile_ranges_buf,
+                            rast.workspace)
+
+        def replay():
+            fg.launch(spec.position, spec.rotation, spec.focal_x, spec.focal_y, spec.zFar,
+                      spec.zNear, bg, out_g, 0)
+        replay()
+        torch.cuda.synchronize()
+        g_same = bool((out_g.cpu().numpy() == images["fp16_persistent"]).all())
+        for _ in range(args.warmup):
+            replay()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            replay()
+        torch.cuda.synchronize()
+        g_ms = (time.perf_counter() - t) * 1e3 / args.steps
+        eager = frame_fn(rast, variant)
+
+        def host_us(fn):
+            ts = []
+            for _ in range(args.steps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                fn()
+                ts.append((time.perf_counter() - t0) * 1e6)
+            torch.cuda.synchronize()
+            return round(float(np.median(ts)), 1)
+        graph_line = {"ms_per_frame": round(g_ms, 4), "image_identical": g_same,
+                      "host_enqueue_us_graph": host_us(replay),
+                      "host_enqueue_us_eager": host_us(lambda: eager(False))}
+        del fg
+        log(f"graph replay: {graph_line}")
+    except RuntimeError as err:  # reported, not fatal to the bench line
+        graph_line = {"error": str(err)}
     # ---------------------------------------------------------- per-stage timing
     rast, _ = paths["fp16_persistent"]
     M = results["fp16_persistent"]["num_rendered"]
@@ -571,6 +625,7 @@ def main():
         "roofline": roof,
         "tile_lists": tile_lists,
         "key_pipelines": pipelines,
+        "graph_replay": graph_line,
         "cpu_baseline": cpu,
     }
     if args.serve_requests > 0:

@@ -56,6 +56,14 @@ enum {
  * 64-bit keys.  It pays off when tiles hold long lists (many pairs per
  * tile), where the per-tile depth sort is slow. */
 #define PS_FORWARD_DEPTH_FIRST 0x100
+/* OR-ed into a ps_forward variant with tile ranges (PERSISTENT_LITE or _DT):
+ * the persistent renderer takes the tiles heaviest first (the order of
+ * ps_compute_tile_order, dealt interleaved to the XCD queues), computed on
+ * the device after the sort in the sort space, which must then hold
+ * ps_forward_tile_order_space(tiles) bytes.  Same image; pays on frames of
+ * many long tile lists (about 1000 pairs per tile and more), where a heavy
+ * tile started late sets the render time. */
+#define PS_FORWARD_TILE_ORDER 0x200
 
 /* Camera exactly as the reference bindings receive it (pybind.cpp:122-150):
  * rotation holds the 9 values of cameras.json "rotation" row by row. */
@@ -335,6 +343,8 @@ typedef struct ps_frame_buffers {
 } ps_frame_buffers;
 
 size_t ps_forward_workspace_size(int P);
+/* Sort-space bytes a frame of `tiles` tiles needs under PS_FORWARD_TILE_ORDER. */
+size_t ps_forward_tile_order_space(int tiles);
 int ps_forward(int P, const float* positions, const void* shs, int sh_prec,
                const float* opacities, const float* cov3d, const ps_camera* cam,
                int variant, int feat_prec, const ps_frame_buffers* fb, const float bg[3],
@@ -344,11 +354,12 @@ int ps_forward(int P, const float* positions, const void* shs, int sh_prec,
  * One frame's whole pipeline (what ps_forward enqueues for this scene,
  * variant, feature precision, buffers and output) captured once into a HIP
  * graph and replayed per request with a new camera pose and background.
- * The graph is tied to cam->width x cam->height and to every pointer given
- * at creation; the kernels read the camera and background from 512 bytes
- * of the forward workspace's queue area, which ps_frame_graph_launch
- * rewrites on the stream before each replay (with a one-wave kernel, so the
- * call allocates nothing and copies nothing from the host).  Replays of one
+ * The graph is tied to cam->width x cam->height and to every buffer of fb;
+ * each replay takes its own camera pose, background and output image
+ * (u8[height][width][3]).  The kernels read those from 512 bytes of the
+ * forward workspace's queue area, which ps_frame_graph_launch rewrites on
+ * the stream before each replay (with a one-wave kernel, so the call
+ * allocates nothing and copies nothing from the host).  Replays of one
  * graph, and ps_forward calls with the same buffers, must not overlap:
  * issue them on one stream.  sync_count and num_rendered_out as ps_forward.
  * No counterpart in the reference (its per-frame host sync, pybind.cpp:
@@ -358,9 +369,11 @@ typedef struct ps_frame_graph ps_frame_graph;
 int ps_frame_graph_create(int P, const float* positions, const void* shs, int sh_prec,
                           const float* opacities, const float* cov3d, const ps_camera* cam,
                           int variant, int feat_prec, const ps_frame_buffers* fb,
-                          uint8_t* out_color, ps_frame_graph** out);
+                          ps_frame_graph** out);
 int ps_frame_graph_launch(ps_frame_graph* graph, const ps_camera* cam, const float bg[3],
-                          int sync_count, int* num_rendered_out, ps_stream_t stream);
+                          uint8_t* out_color, int sync_count, int* num_rendered_out,
+                          ps_stream_t stream);
+/* Waits for the device to finish its queued work, then frees the graph. */
 int ps_frame_graph_destroy(ps_frame_graph* graph);
 
 /* Count of a frame posted with ps_forward(sync_count == 2): waits for the

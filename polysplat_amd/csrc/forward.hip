@@ -296,6 +296,11 @@ extern "C" int ps_es_tile_sort(int num_rendered, const int* d_num_rendered, int 
                               reinterpret_cast<hipStream_t>(stream));
 }
 
+extern "C" size_t ps_forward_tile_order_space(int tiles) {
+  return ps::align_up(ps::tile_order_temp_size(tiles < 0 ? 0 : tiles), 256) +
+         16 * (size_t)(tiles < 0 ? 0 : tiles);
+}
+
 extern "C" size_t ps_forward_workspace_size(int P) {
   return ps::kQueueBytes + ps::align_up(ps_preprocess_workspace_size(P), 256);
 }
@@ -303,17 +308,18 @@ extern "C" size_t ps_forward_workspace_size(int P) {
 namespace ps {
 namespace {
 
-// Checks and enqueues one frame (no count readback).  cam_dev / bg_dev:
-// NULL, or the device copies of the camera and background that the kernels
+// Checks and enqueues one frame (no count readback).  fc: NULL, or the
+// device copy of the camera, background and image pointer that the kernels
 // read instead of their arguments (a captured graph's frames).
 int forward_enqueue(int P, const float* positions, const void* shs, int sh_prec,
                     const float* opacities, const float* cov3d, const ps_camera* cam, int variant,
                     int feat_prec, const ps_frame_buffers* fb, const float bg[3],
-                    uint8_t* out_color, hipStream_t stream, const CamParams* cam_dev,
-                    const float* bg_dev) {
+                    uint8_t* out_color, hipStream_t stream, const FrameConsts* fc) {
+  const CamParams* cam_dev = fc ? &fc->cp : nullptr;
   PS_REQUIRE(fb != nullptr && cam != nullptr && bg != nullptr, "NULL argument");
   const bool depth_first = (variant & PS_FORWARD_DEPTH_FIRST) != 0;
-  variant &= ~PS_FORWARD_DEPTH_FIRST;
+  const bool tile_ordered = (variant & PS_FORWARD_TILE_ORDER) != 0;
+  variant &= ~(PS_FORWARD_DEPTH_FIRST | PS_FORWARD_TILE_ORDER);
   PS_REQUIRE(variant >= PS_VARIANT_DEFAULT && variant <= PS_VARIANT_PERSISTENT_DT,
              "unknown render variant %d", variant);
   PS_REQUIRE(cam->block_x == 16 && cam->block_y == 16,
@@ -329,6 +335,13 @@ int forward_enqueue(int P, const float* positions, const void* shs, int sh_prec,
   const bool need_ranges =
       variant == PS_VARIANT_PERSISTENT_LITE || variant == PS_VARIANT_PERSISTENT_DT;
   PS_REQUIRE(!need_ranges || fb->tile_ranges != nullptr, "tile_ranges is NULL");
+  const int gx = (cam->width + 15) / 16;
+  PS_REQUIRE(!tile_ordered || need_ranges,
+             "PS_FORWARD_TILE_ORDER needs a variant with tile ranges (PERSISTENT_LITE or _DT)");
+  PS_REQUIRE(!tile_ordered || fb->sort_space_bytes >=
+                                  ps_forward_tile_order_space(gx * ((cam->height + 15) / 16)),
+             "sort space too small for the tile order (%zu < %zu)", fb->sort_space_bytes,
+             ps_forward_tile_order_space(gx * ((cam->height + 15) / 16)));
 
   // The queue words sit at the start of the workspace, where no other stage
   // writes, so a workspace reused for scenes of other sizes keeps them zero.
@@ -377,14 +390,24 @@ int forward_enqueue(int P, const float* positions, const void* shs, int sh_prec,
   if (variant == PS_VARIANT_DEFAULT) {
     return render_plain_launch(0, d_n, cam->width, cam->height, fb->points_xy, fb->rgb_depth,
                                fb->conic_opacity, fb->keys_sorted, fb->values_sorted, bg,
-                               out_color, stream, nullptr, nullptr, nullptr, bg_dev);
+                               out_color, stream, nullptr, nullptr, nullptr, fc);
+  }
+  int* desc = nullptr;
+  if (tile_ordered) {
+    // The sort is done with its space (stream order): the tile order's
+    // scratch at its start, the packed descriptors after it.
+    const size_t tb = tile_order_temp_size(tiles);
+    desc = reinterpret_cast<int*>(static_cast<char*>(fb->sort_space) + align_up(tb, 256));
+    st = tile_order_launch(fb->tile_ranges, tiles, gx, 0, 0, nullptr, nullptr, nullptr, desc,
+                           fb->sort_space, tb, stream);
+    if (st != PS_OK) return st;
   }
   return render_persistent_launch(0, d_n, cam->width, cam->height, feat_prec, fb->points_xy,
                                   fb->rgb_depth, fb->conic_opacity, fb->records, 0,
                                   fb->keys_sorted, fb->values_sorted,
                                   need_ranges ? fb->tile_ranges : nullptr,
                                   variant == PS_VARIANT_PERSISTENT_DT, bg, out_color, queue,
-                                  stream, nullptr, nullptr, bg_dev);
+                                  stream, nullptr, desc, fc);
 }
 
 // The end-of-frame count: none (sync_count 0), waited for (1), or posted to
@@ -430,7 +453,7 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
     if (fs != PS_OK) return fs;
   }
   const int st = forward_enqueue(P, positions, shs, sh_prec, opacities, cov3d, cam, variant,
-                                 feat_prec, fb, bg, out_color, stream, nullptr, nullptr);
+                                 feat_prec, fb, bg, out_color, stream, nullptr);
   if (st != PS_OK) return st;
   return finish_count(fb, sync_count, slot, num_rendered_out, stream);
 }
@@ -438,50 +461,37 @@ extern "C" int ps_forward(int P, const float* positions, const void* shs, int sh
 // ------------------------------------------------------------ frame graphs
 // A frame's whole pipeline (every kernel ps_forward enqueues) captured once
 // into a HIP graph, for one scene, resolution, variant and set of buffers,
-// and replayed per request.  The kernels take their camera and background
-// from a 512-byte block in the forward workspace's queue area (bytes
-// 1536..2047, which the tile queue does not use) instead of their
-// arguments, so a replay needs only that block rewritten first: a one-wave
-// kernel whose arguments carry the new camera does it, on the same stream.
-// Host work per frame: two launches instead of about twenty.
+// and replayed per request.  The kernels take their camera, background and
+// output image from a block of per-frame constants in the forward
+// workspace's queue area (bytes 1536..2047, which the tile queue does not
+// use) instead of their arguments, so a replay needs only that block
+// rewritten first: a one-wave kernel whose arguments carry the new values
+// does it, on the same stream.  Host work per frame: two launches instead of
+// about twenty.
 struct ps_frame_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int width = 0, height = 0;
   ps_frame_buffers fb;
-  ps::CamParams* cam_dev = nullptr;
-  float* bg_dev = nullptr;
+  ps::FrameConsts* consts = nullptr;
 };
 
 namespace ps {
 namespace {
 
 constexpr size_t kFrameConstsOffset = 1536;
-static_assert(kFrameConstsOffset + sizeof(CamParams) + 4 * sizeof(float) <= kQueueBytes,
-              "camera block fits the queue area");
+static_assert(kFrameConstsOffset + sizeof(FrameConsts) <= kQueueBytes,
+              "frame constants fit the queue area");
+static_assert(kFrameConstsOffset % alignof(FrameConsts) == 0, "aligned frame constants");
 
-// Camera and background of the next replay into device memory.
-__global__ __launch_bounds__(64) void ps_set_frame_consts(CamParams* __restrict__ dst,
-                                                          float* __restrict__ bg_dst,
-                                                          CamParams cp, float3 bg) {
+// The next replay's camera, background and image pointer into device memory.
+__global__ __launch_bounds__(64) void ps_set_frame_consts(FrameConsts* __restrict__ dst,
+                                                          FrameConsts v) {
+  constexpr int kWords = (int)(sizeof(FrameConsts) / sizeof(uint32_t));
+  static_assert(kWords <= 64 && sizeof(FrameConsts) % sizeof(uint32_t) == 0,
+                "one word per lane");
   const int l = threadIdx.x;
-  constexpr int kWords = (int)(sizeof(CamParams) / sizeof(uint32_t));
-  static_assert(kWords <= 64, "one word per lane");
-  if (l < kWords) reinterpret_cast<uint32_t*>(dst)[l] = reinterpret_cast<const uint32_t*>(&cp)[l];
-  if (l == 0) {
-    bg_dst[0] = bg.x;
-    bg_dst[1] = bg.y;
-    bg_dst[2] = bg.z;
-  }
-}
-
-int set_frame_consts(const ps_frame_graph* g, const ps_camera* cam, const float bg[3],
-                     hipStream_t stream) {
-  const CamParams cp = make_cam_params(cam);
-  hipLaunchKernelGGL(ps_set_frame_consts, dim3(1), dim3(64), 0, stream, g->cam_dev, g->bg_dev, cp,
-                     make_float3(bg[0], bg[1], bg[2]));
-  PS_LAUNCH_CHECK();
-  return PS_OK;
+  if (l < kWords) reinterpret_cast<uint32_t*>(dst)[l] = reinterpret_cast<const uint32_t*>(&v)[l];
 }
 
 }  // namespace
@@ -490,8 +500,7 @@ int set_frame_consts(const ps_frame_graph* g, const ps_camera* cam, const float 
 extern "C" int ps_frame_graph_create(int P, const float* positions, const void* shs, int sh_prec,
                                      const float* opacities, const float* cov3d,
                                      const ps_camera* cam, int variant, int feat_prec,
-                                     const ps_frame_buffers* fb, uint8_t* out_color,
-                                     ps_frame_graph** out) {
+                                     const ps_frame_buffers* fb, ps_frame_graph** out) {
   using namespace ps;
   PS_REQUIRE(out != nullptr && fb != nullptr && cam != nullptr, "NULL argument");
   *out = nullptr;
@@ -501,10 +510,9 @@ extern "C" int ps_frame_graph_create(int P, const float* positions, const void* 
   g->width = cam->width;
   g->height = cam->height;
   g->fb = *fb;
-  char* qa = static_cast<char*>(fb->workspace) + kFrameConstsOffset;
-  g->cam_dev = reinterpret_cast<CamParams*>(qa);
-  g->bg_dev = reinterpret_cast<float*>(qa + sizeof(CamParams));
-  // Capture on a stream of the library's own (captures need a created
+  g->consts = reinterpret_cast<FrameConsts*>(static_cast<char*>(fb->workspace) +
+                                             kFrameConstsOffset);
+  // Capture on a stream of the library's own (a capture needs a created
   // stream; the caller's may be the null stream).  Default priority.
   hipStream_t cs = nullptr;
   hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
@@ -513,15 +521,17 @@ extern "C" int ps_frame_graph_create(int P, const float* positions, const void* 
     set_last_error("hipStreamCreateWithFlags failed: %s", hipGetErrorString(e));
     return PS_ERR_HIP;
   }
-  e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
   int st = PS_OK;
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
   if (e != hipSuccess) {
     set_last_error("hipStreamBeginCapture failed: %s", hipGetErrorString(e));
     st = PS_ERR_HIP;
   } else {
+    // The captured arguments' background and image are placeholders: the
+    // kernels read the constants block.
     const float bg0[3] = {0.f, 0.f, 0.f};
     st = forward_enqueue(P, positions, shs, sh_prec, opacities, cov3d, cam, variant, feat_prec,
-                         fb, bg0, out_color, cs, g->cam_dev, g->bg_dev);
+                         fb, bg0, nullptr, cs, g->consts);
     char saved[512];
     snprintf(saved, sizeof(saved), "%s", ps_last_error());
     hipGraph_t graph = nullptr;
@@ -550,11 +560,13 @@ extern "C" int ps_frame_graph_create(int P, const float* positions, const void* 
 }
 
 extern "C" int ps_frame_graph_launch(ps_frame_graph* g, const ps_camera* cam, const float bg[3],
-                                     int sync_count, int* num_rendered_out, ps_stream_t stream_) {
+                                     uint8_t* out_color, int sync_count, int* num_rendered_out,
+                                     ps_stream_t stream_) {
   using namespace ps;
   hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
   PS_REQUIRE(g != nullptr && g->exec != nullptr && cam != nullptr && bg != nullptr,
              "NULL argument");
+  PS_REQUIRE(out_color != nullptr, "out_color is NULL");
   PS_REQUIRE(cam->width == g->width && cam->height == g->height && cam->block_x == 16 &&
                  cam->block_y == 16,
              "the graph renders %dx%d frames with 16x16 tiles (camera: %dx%d, %dx%d tiles)",
@@ -565,14 +577,24 @@ extern "C" int ps_frame_graph_launch(ps_frame_graph* g, const ps_camera* cam, co
     const int fs = free_post_slot(&slot);
     if (fs != PS_OK) return fs;
   }
-  int st = set_frame_consts(g, cam, bg, stream);
-  if (st != PS_OK) return st;
+  FrameConsts v;
+  memset(&v, 0, sizeof(v));
+  v.cp = make_cam_params(cam);
+  v.bg[0] = bg[0];
+  v.bg[1] = bg[1];
+  v.bg[2] = bg[2];
+  v.out = out_color;
+  hipLaunchKernelGGL(ps_set_frame_consts, dim3(1), dim3(64), 0, stream, g->consts, v);
+  PS_LAUNCH_CHECK();
   PS_HIP_CHECK(hipGraphLaunch(g->exec, stream));
   return finish_count(&g->fb, sync_count, slot, num_rendered_out, stream);
 }
 
 extern "C" int ps_frame_graph_destroy(ps_frame_graph* g) {
   if (g == nullptr) return PS_OK;
+  // Replays may still be queued on any stream: let the device finish them
+  // before their executable graph goes away.
+  if (g->exec) (void)hipDeviceSynchronize();
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   delete g;

@@ -601,20 +601,19 @@ struct FrameArgs {
   Tensor* ranges;
 };
 
-int forward_common(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width, int height,
-                   int block_x, int block_y, Tensor& position, Tensor& rotation, float fx,
-                   float fy, float zfar, float znear, int variant, int feat_prec,
-                   const FrameArgs& a, Tensor& bg_color, Tensor& out, Tensor* workspace,
-                   int sync, const char* what) {
+// The frame's buffers as ps_frame_buffers, after checking them; `ws_out`
+// receives the workspace tensor used (the caller's, or the module's cached
+// one when none is given and `allow_cached`).
+ps_frame_buffers frame_buffers(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width,
+                               int height, int feat_prec, const FrameArgs& a, Tensor* workspace,
+                               bool allow_cached, Tensor& ws_out, const char* what) {
   check_scene(pts, shs, op, cov);
-  need(block_x == 16 && block_y == 16, "only 16x16 tiles are implemented");
   need_dev(*a.curr_offset, "curr_offset", at::kInt);
   need_dev(*a.sort_space, "list_sorting_space", at::kChar);
   need_dev(*a.ku, "gaussian_keys_unsorted", at::kLong);
   need_dev(*a.vu, "gaussian_values_unsorted", at::kInt);
   need_dev(*a.ks, "gaussian_keys_sorted", at::kLong);
   need_dev(*a.vs, "gaussian_values_sorted", at::kInt);
-  check_out(out, width, height);
   const int P = (int)op.size(0);
   ps_frame_buffers fb;
   memset(&fb, 0, sizeof(fb));
@@ -657,10 +656,26 @@ int forward_common(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width,
     need((size_t)workspace->numel() >= wsb, "workspace too small");
     ws = *workspace;
   } else {
+    need(allow_cached, std::string(what) + ": needs a workspace of forward_workspace_size(P)");
     ws = cached_workspace(op.device(), wsb, "forward");
   }
   fb.workspace = ws.data_ptr();
   fb.workspace_bytes = (size_t)ws.numel();
+  ws_out = ws;
+  return fb;
+}
+
+int forward_common(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width, int height,
+                   int block_x, int block_y, Tensor& position, Tensor& rotation, float fx,
+                   float fy, float zfar, float znear, int variant, int feat_prec,
+                   const FrameArgs& a, Tensor& bg_color, Tensor& out, Tensor* workspace,
+                   int sync, const char* what) {
+  need(block_x == 16 && block_y == 16, "only 16x16 tiles are implemented");
+  check_out(out, width, height);
+  Tensor ws;
+  ps_frame_buffers fb =
+      frame_buffers(pts, shs, op, cov, width, height, feat_prec, a, workspace, true, ws, what);
+  const int P = (int)op.size(0);
   ps_camera cam = make_camera(width, height, block_x, block_y, position, rotation, fx, fy, zfar,
                               znear);
   auto bg = host_floats(bg_color, 3, "bg_color");
@@ -741,6 +756,60 @@ int forward_frame_post(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int wi
                         zfar, znear, variant, feat_prec, a, bg, out, &workspace, 2,
                         "forward_frame_post");
 }
+
+// A frame's pipeline captured once as a HIP graph (ps_frame_graph_create)
+// and replayed per request with a new pose, background and output image.
+// Holds references to every buffer the graph points at, so none is freed
+// while the graph lives.  Replays and ops calls on the same buffers must be
+// issued on one stream.
+class FrameGraph {
+ public:
+  FrameGraph(Tensor& pts, Tensor& shs, Tensor& op, Tensor& cov, int width, int height,
+             Tensor& position, Tensor& rotation, float fx, float fy, float zfar, float znear,
+             int variant, int feat_prec, Tensor& curr_offset, Tensor& sort_space, Tensor& ku,
+             Tensor& vu, Tensor& ks, Tensor& vs, Tensor& xy, Tensor& rgbd, Tensor& con,
+             Tensor& records, Tensor& tile_ranges, Tensor& workspace)
+      : width_(width), height_(height) {
+    FrameArgs a{&curr_offset, &sort_space, &ku, &vu, &ks, &vs, &xy, &rgbd, &con, &records,
+                tile_ranges.numel() > 0 ? &tile_ranges : nullptr};
+    Tensor ws;
+    ps_frame_buffers fb = frame_buffers(pts, shs, op, cov, width, height, feat_prec, a,
+                                        &workspace, false, ws, "FrameGraph");
+    ps_camera cam = make_camera(width, height, 16, 16, position, rotation, fx, fy, zfar, znear);
+    keep_ = {pts, shs, op, cov, curr_offset, sort_space, ku, vu, ks, vs, xy, rgbd, con, records,
+             tile_ranges, ws};
+    check(ps_frame_graph_create((int)op.size(0), ptr<float>(pts), shs.data_ptr(),
+                                sh_prec_of(shs), ptr<float>(op), ptr<float>(cov), &cam, variant,
+                                feat_prec, &fb, &g_),
+          "FrameGraph");
+  }
+  ~FrameGraph() { ps_frame_graph_destroy(g_); }
+  FrameGraph(const FrameGraph&) = delete;
+  FrameGraph& operator=(const FrameGraph&) = delete;
+
+  // Replays the frame on torch's current stream.  sync 1: returns the pair
+  // count (RuntimeError on overflow); 0: returns -1; 2: returns a ticket for
+  // forward_wait.
+  int launch(Tensor& position, Tensor& rotation, float fx, float fy, float zfar, float znear,
+             Tensor& bg_color, Tensor& out, int sync) {
+    check_out(out, width_, height_);
+    ps_camera cam = make_camera(width_, height_, 16, 16, position, rotation, fx, fy, zfar, znear);
+    auto bg = host_floats(bg_color, 3, "bg_color");
+    int n = -1;
+    const int st = ps_frame_graph_launch(g_, &cam, bg.data(), ptr<uint8_t>(out), sync, &n,
+                                         cur_stream());
+    if (st == PS_ERR_OVERFLOW)
+      throw std::runtime_error(std::string("FrameGraph.launch: Too many k-v pairs! (") +
+                               ps_last_error() + ")");
+    check(st, "FrameGraph.launch");
+    return n;
+  }
+
+ private:
+  ps_frame_graph* g_ = nullptr;
+  int width_, height_;
+  std::vector<Tensor> keep_;
+};
 
 // (pair count, whether the pairs overflowed the key buffers: nothing was
 // rendered then and the frame has to be run again with more room).
@@ -909,6 +978,13 @@ PYBIND11_MODULE(_C, m) {
   ops.def("forward_frame", &forward_frame);
   ops.def("forward_frame_post", &forward_frame_post);
   ops.def("forward_wait", &forward_wait);
+  py::class_<FrameGraph>(ops, "FrameGraph")
+      .def(py::init<Tensor&, Tensor&, Tensor&, Tensor&, int, int, Tensor&, Tensor&, float, float,
+                    float, float, int, int, Tensor&, Tensor&, Tensor&, Tensor&, Tensor&, Tensor&,
+                    Tensor&, Tensor&, Tensor&, Tensor&, Tensor&, Tensor&>())
+      .def("launch", &FrameGraph::launch, py::arg("position"), py::arg("rotation"),
+           py::arg("focal_x"), py::arg("focal_y"), py::arg("zFar"), py::arg("zNear"),
+           py::arg("bg_color"), py::arg("out_color"), py::arg("sync") = 1);
   ops.def("forward_query", [](int ticket) {
     const int r = ps_forward_query(ticket);
     if (r < 0) check(-r, "forward_query");

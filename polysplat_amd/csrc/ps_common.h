@@ -406,6 +406,14 @@ __device__ __forceinline__ void bind_camera(FrameDev& F) {
   if (F.cam_dev) F.cp = *F.cam_dev;
 }
 
+// Per-frame constants of a frame replayed from a captured graph, in device
+// memory: the kernels read them instead of their (captured) arguments.
+struct FrameConsts {
+  CamParams cp;
+  float bg[4];
+  uint8_t* out;  // u8[H][W][3]
+};
+
 // Largest f with key >= field(frames[f]) (frames sorted by that field).
 template <typename Field>
 __device__ __forceinline__ int frame_search(const FrameDev* frames, int nf, int key, Field field) {

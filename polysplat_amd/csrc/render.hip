@@ -480,12 +480,15 @@ __global__ __launch_bounds__(kWave, 6) void ps_render_plain(
     const float4* __restrict__ rgbd, const float4* __restrict__ con,
     const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
     const int2* __restrict__ ranges, const uint32_t* __restrict__ tile_order,
-    const int4* __restrict__ tile_desc, const float* __restrict__ bg_dev) {
+    const int4* __restrict__ tile_desc, const FrameConsts* __restrict__ fc) {
   __shared__ float4 sA[2][kBatch];
   __shared__ float4 sB[2][kBatch];
   __shared__ float sC[2][kBatch];
   const int lane = threadIdx.x;
-  if (bg_dev) bg = make_float3(bg_dev[0], bg_dev[1], bg_dev[2]);
+  if (fc) {  // a replayed graph's frame: background and image from device memory
+    bg = make_float3(fc->bg[0], fc->bg[1], fc->bg[2]);
+    out = fc->out;
+  }
   const int pos = blockIdx.y * gx + blockIdx.x;
   const int tiles = gx * (int)gridDim.y;
   // With a tile order, workgroup `pos` renders tile tile_order[pos]
@@ -597,12 +600,15 @@ __global__ __launch_bounds__(kWave, 7) void ps_render_persistent(
     const float4* __restrict__ rgbd, const float4* __restrict__ con,
     const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
     unsigned* __restrict__ queue, const uint32_t* __restrict__ tile_order,
-    const int4* __restrict__ tile_desc, int interleave, const float* __restrict__ bg_dev) {
+    const int4* __restrict__ tile_desc, int interleave, const FrameConsts* __restrict__ fc) {
   __shared__ float4 sA[2][kBatch];
   __shared__ float4 sB[2][kBatch];
   __shared__ float sC[2][kBatch];
   const int lane = threadIdx.x;
-  if (bg_dev) bg = make_float3(bg_dev[0], bg_dev[1], bg_dev[2]);
+  if (fc) {  // a replayed graph's frame: background and image from device memory
+    bg = make_float3(fc->bg[0], fc->bg[1], fc->bg[2]);
+    out = fc->out;
+  }
   const int pairs = d_n ? *d_n : n_host;
   const int n = RANGES ? 0 : pairs;
   if (quads == 2) quads = (long long)pairs >= (long long)kQuadPairs * tiles ? 1 : 0;
@@ -758,7 +764,7 @@ struct PersistArgs {
   unsigned* queue;
   const uint32_t* tile_order;  // NULL: tiles in row-major order
   const int* tile_desc;        // NULL, or int4 (column, row, start, end) per position
-  const float* bg_dev;         // NULL, or the background in device memory (replaces bg)
+  const FrameConsts* fc;       // NULL, or background and image in device memory
 };
 
 template <int PREC, bool RANGES, bool GATHERED, int CHECK, bool DT>
@@ -774,8 +780,7 @@ int launch_persistent(const PersistArgs& a, hipStream_t stream) {
                      reinterpret_cast<const int2*>(a.ranges), a.W, a.H, gx, tiles, quads, a.order,
                      (const float2*)a.xy, (const float4*)a.rgbd, (const float4*)a.con,
                      (const FeatRec16*)a.rec, a.bg, a.out, a.queue, a.tile_order,
-                     reinterpret_cast<const int4*>(a.tile_desc), order_queues_interleaved(),
-                     a.bg_dev);
+                     reinterpret_cast<const int4*>(a.tile_desc), order_queues_interleaved(), a.fc);
   PS_LAUNCH_CHECK();
   return PS_OK;
 }
@@ -826,9 +831,9 @@ int render_plain_launch(int n_host, const int* d_n, int W, int H, const float* x
                         const float* rgbd, const float* con, const uint64_t* keys,
                         const uint32_t* order, const float bg[3], uint8_t* out,
                         hipStream_t stream, const int* ranges, const uint32_t* tile_order,
-                        const int* tile_desc, const float* bg_dev) {
+                        const int* tile_desc, const FrameConsts* fc) {
   PS_REQUIRE(W > 0 && H > 0, "bad image size");
-  PS_REQUIRE(out != nullptr && bg != nullptr, "output or background is NULL");
+  PS_REQUIRE((out != nullptr || fc != nullptr) && bg != nullptr, "output or background is NULL");
   PS_REQUIRE(d_n != nullptr || n_host >= 0, "negative num_rendered");
   PS_REQUIRE(ranges != nullptr || keys != nullptr || tile_desc != nullptr,
              "need keys_sorted, tile_ranges or tile descriptors");
@@ -840,7 +845,7 @@ int render_plain_launch(int n_host, const int* d_n, int W, int H, const float* x
                      n_host, W, H, gx, order, (const float2*)xy, (const float4*)rgbd,
                      (const float4*)con, (const FeatRec16*)nullptr, b, out,
                      reinterpret_cast<const int2*>(ranges), tile_order,
-                     reinterpret_cast<const int4*>(tile_desc), bg_dev);
+                     reinterpret_cast<const int4*>(tile_desc), fc);
   PS_LAUNCH_CHECK();
   return PS_OK;
 }
@@ -850,15 +855,16 @@ int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_
                              int gathered, const uint64_t* keys, const uint32_t* order,
                              const int* ranges, int dt, const float bg[3], uint8_t* out,
                              unsigned* queue, hipStream_t stream, const uint32_t* tile_order,
-                             const int* tile_desc, const float* bg_dev) {
+                             const int* tile_desc, const FrameConsts* fc) {
   PS_REQUIRE(W > 0 && H > 0, "bad image size");
-  PS_REQUIRE(out != nullptr && bg != nullptr && queue != nullptr, "NULL output/bg/queue");
+  PS_REQUIRE((out != nullptr || fc != nullptr) && bg != nullptr && queue != nullptr,
+             "NULL output/bg/queue");
   PS_REQUIRE(ranges != nullptr || keys != nullptr || tile_desc != nullptr,
              "need tile_ranges, keys_sorted or tile descriptors");
   PS_REQUIRE(gathered || order != nullptr, "values_sorted is NULL");
   PS_REQUIRE(d_n != nullptr || n_host >= 0, "negative num_rendered");
   PersistArgs a{keys, d_n,  n_host, ranges, W, H, order, xy, rgbd, con, rec,
-                make_float3(bg[0], bg[1], bg[2]), out, queue, tile_order, tile_desc, bg_dev};
+                make_float3(bg[0], bg[1], bg[2]), out, queue, tile_order, tile_desc, fc};
   const bool g = gathered != 0, d = dt != 0;
   switch (feat_prec) {
     case PS_PREC_F32:

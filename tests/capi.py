@@ -19,6 +19,7 @@ F32, F16, BF16 = 0, 1, 2
 PREC_IDS = {"fp32": F32, "fp16": F16, "bf16": BF16}
 V_DEFAULT, V_LITE, V_FUSED, V_DT = 0, 1, 2, 3
 DEPTH_FIRST = 0x100  # PS_FORWARD_DEPTH_FIRST, OR-ed into a variant
+TILE_ORDER = 0x200  # PS_FORWARD_TILE_ORDER
 MAX_BATCH_FRAMES = 64
 
 
@@ -101,7 +102,11 @@ def bind(L):
         "ps_render_16x16_reordered_desc": ([I, P, I, I, P, P, P, P, P, P, P, P], I),
         "ps_render_16x16_reordered_persistent": ([I, P, I, I, I, P, P, P, P, P, P, P, P, I, P,
                                                   P, P, P], I),
+        "ps_frame_graph_create": ([I, P, P, I, P, P, P, I, I, P, P], I),
+        "ps_frame_graph_launch": ([P, P, P, P, I, P, P], I),
+        "ps_frame_graph_destroy": ([P], I),
         "ps_forward_workspace_size": ([I], S),
+        "ps_forward_tile_order_space": ([I], S),
         "ps_forward": ([I, P, P, I, P, P, P, I, I, P, P, P, I, P, P], I),
         "ps_forward_wait": ([I, P], I),
         "ps_forward_query": ([I], I),

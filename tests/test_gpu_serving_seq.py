@@ -43,12 +43,20 @@ def _alone(sc, r, radius):
     return n, out.cpu().numpy()
 
 
-@pytest.mark.parametrize("executor", ["SequentialExecutor", "PipelinedExecutor"])
+@pytest.mark.parametrize("executor", ["SequentialExecutor", "PipelinedExecutor",
+                                      "PipelinedExecutor+graphs",
+                                      "PipelinedExecutor+graphs+tile_order"])
 def test_serving_equals_frames_rendered_alone(gpu, executor):
     radius = 1.5 * BOX
     scenes = SV.make_resident_scenes(SIZES, device="cuda:0", box=BOX,
                                      mix=[(0.9, -3.5, 0.5), (0.1, -2.0, 0.4)])
-    ex = getattr(SV, executor)(scenes, radius=radius)
+    if "+graphs" in executor:
+        # "+tile_order": every frame heaviest-first (thresholds at one pair)
+        order = 1 if executor.endswith("+tile_order") else 0
+        ex = SV.PipelinedExecutor(scenes, radius=radius, graphs=True, tile_order_pairs=order,
+                                  tile_order_per_tile=order)
+    else:
+        ex = getattr(SV, executor)(scenes, radius=radius)
     est = SV.Estimates()
     say("sequential serving calibrate")
     SV.calibrate(ex, est, len(SIZES), resolutions=RES, azimuths=(0.0,), repeats=1)
@@ -64,6 +72,11 @@ def test_serving_equals_frames_rendered_alone(gpu, executor):
     summary = SV.summarize(done, sched.rejected, wall_s=ex.now_s(), batches=loop.batches)
     say("summary", summary, "grows", ex.grows)
     assert all(r.latency_ms > 0 and math.isfinite(r.latency_ms) for r in done)
+    if "+graphs" in executor:
+        # one capture per (scene, resolution), again after each key-buffer growth
+        assert 0 < ex.captures <= len(SIZES) * len(RES) * (1 + ex.grows)
+    if executor.endswith("+tile_order"):
+        assert ex.ordered_frames > 0
     for r in done:
         n, img = _alone(scenes[r.scene], r, radius)
         assert n == r.pairs, (r.rid, n, r.pairs)

@@ -8,6 +8,8 @@ byte-identical to the same renderer without a tile order, and within one
 8-bit level of the oracle run with the same precisions and saturation
 cadence.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 import torch
@@ -17,7 +19,9 @@ import polysplat_amd as polysplat
 from polysplat_amd import rasterizer as R
 from polysplat_amd import scenes
 from polysplat_amd.metrics import diff_metrics
-from tests.gpu_helpers import oracle_frame, say
+from tests import capi
+from tests.gpu_helpers import PREC, Gpu, oracle_frame, say
+from tests.test_gpu_graph import _fb
 
 pytestmark = pytest.mark.gpu
 
@@ -249,3 +253,42 @@ def test_rasterizer_reordered_variants(case, variant, feature_dtype):
                        measure_sort=True, measure_render=True)
     assert len(res) == 4 and all(isinstance(x, float) and x > 0 for x in res[1:])
     np.testing.assert_array_equal(res[0].cpu().numpy(), plain)
+
+
+@pytest.mark.parametrize("variant,prec", [(capi.V_LITE, "fp16"), (capi.V_LITE, "fp32"),
+                                          (capi.V_DT, "fp16"),
+                                          (capi.V_LITE | capi.DEPTH_FIRST, "bf16")])
+def test_forward_with_tile_order_flag(case, variant, prec):
+    """ps_forward with PS_FORWARD_TILE_ORDER: the same image, count and
+    sorted pairs as without it, eagerly and replayed from a graph."""
+    arrays, spec, scene = case
+    a = Gpu(arrays, spec, sh_type=prec, feat_prec=prec)
+    st, n, ref = a.forward(BG, variant)
+    capi.check(st)
+    b = Gpu(arrays, spec, sh_type=prec, feat_prec=prec)
+    st, m, img = b.forward(BG, variant | capi.TILE_ORDER)
+    capi.check(st)
+    assert m == n
+    np.testing.assert_array_equal(img, ref)
+    np.testing.assert_array_equal(b.vs[:n].cpu().numpy(), a.vs[:n].cpu().numpy())
+    L = capi.lib()
+    graph = C.c_void_p()
+    capi.check(L.ps_frame_graph_create(b.P, capi.ptr(b.pos), capi.ptr(b.shs), PREC[prec],
+                                       capi.ptr(b.opac), capi.ptr(b.cov), C.byref(b.cam),
+                                       variant | capi.TILE_ORDER, PREC[prec],
+                                       C.byref(_fb(b)), C.byref(graph)))
+    try:
+        for _ in range(3):
+            out = torch.full_like(b.out, 5)
+            k = C.c_int(-1)
+            capi.check(L.ps_frame_graph_launch(graph, C.byref(b.cam), capi.floats3(BG),
+                                               capi.ptr(out), 1, C.byref(k), capi.stream()))
+            assert k.value == n
+            np.testing.assert_array_equal(out.cpu().numpy(), ref)
+    finally:
+        L.ps_frame_graph_destroy(graph)
+    # the flag needs tile ranges, and sort space for the order
+    assert b.forward(BG, capi.V_FUSED | capi.TILE_ORDER)[0] == capi.PS_ERR_INVALID
+    small = Gpu(arrays, spec, sh_type=prec, feat_prec=prec, capacity=64)
+    assert small.sort_bytes < L.ps_forward_tile_order_space(small.tiles)
+    assert small.forward(BG, variant | capi.TILE_ORDER)[0] == capi.PS_ERR_INVALID

@@ -194,6 +194,12 @@ def test_emulated_reordered_renders_equal_unordered(emu):
                 HF.ptr(out), HF.ptr(q), None))
             np.testing.assert_array_equal(out, img, err_msg=f"{prec} desc={use_desc}")
             assert not q.any()  # the queue words are left zero
+        if prec == "fp16":
+            # ps_forward with PS_FORWARD_TILE_ORDER: the same frame
+            st, m, img2 = f.forward(bg, capi.V_LITE | capi.TILE_ORDER)
+            capi.check_with(L, st)
+            assert m == n
+            np.testing.assert_array_equal(img2, img)
         if prec == "fp32":
             # plain renderer: row-major, over the order, over the descriptors
             base = np.full_like(img, 3)
