@@ -262,10 +262,11 @@ def test_forward_with_tile_order_flag(case, variant, prec):
     """ps_forward with PS_FORWARD_TILE_ORDER: the same image, count and
     sorted pairs as without it, eagerly and replayed from a graph."""
     arrays, spec, scene = case
-    a = Gpu(arrays, spec, sh_type=prec, feat_prec=prec)
+    cap = 2_000_000
+    a = Gpu(arrays, spec, sh_type=prec, feat_prec=prec, capacity=cap)
     st, n, ref = a.forward(BG, variant)
     capi.check(st)
-    b = Gpu(arrays, spec, sh_type=prec, feat_prec=prec)
+    b = Gpu(arrays, spec, sh_type=prec, feat_prec=prec, capacity=cap)
     st, m, img = b.forward(BG, variant | capi.TILE_ORDER)
     capi.check(st)
     assert m == n
