@@ -295,6 +295,25 @@ int ps_render_16x16_reordered_desc(int num_rendered, const int* d_num_rendered, 
                                    const float* conic_opacity, const uint32_t* values_sorted,
                                    const int* tile_desc, const float bg[3], uint8_t* out_color,
                                    ps_stream_t stream);
+/* Top-k cache renderers: values_sorted may hold, instead of a Gaussian id,
+ * 0x80000000 | slot, naming slot `slot` (< num_topk) of compact FP32
+ * feature arrays topk_xy f32[num_topk][2], topk_rgb_depth and topk_conic
+ * f32[num_topk][4] (the most-referenced Gaussians' features, copied by the
+ * caller); ranges by an in-kernel search of keys_sorted; saturation test
+ * every 2 Gaussians.  persistent == 0: one workgroup per tile, replaces
+ * render_16x16_topk_smem (render.cu:6698-6723); persistent != 0: the tile
+ * queue (`queue` as in ps_render_16x16_persistent), replaces
+ * render_16x16_topk_smem_persistent[_v2] (render.cu:6725-6800); bound at
+ * pybind.cpp:366-437.  The reference stages the cache in shared memory; here
+ * it is read through L2, where 40 bytes per slot stay resident (a per-
+ * workgroup LDS copy would cut the renderer's waves per SIMD several-fold). */
+int ps_render_16x16_topk(int num_rendered, const int* d_num_rendered, int width, int height,
+                         const float* points_xy, const float* rgb_depth,
+                         const float* conic_opacity, const uint64_t* keys_sorted,
+                         const uint32_t* values_sorted, const float* topk_xy,
+                         const float* topk_rgb_depth, const float* topk_conic, int num_topk,
+                         int persistent, const float bg[3], uint8_t* out_color, unsigned* queue,
+                         ps_stream_t stream);
 /* Persistent renderer over a tile order: the tile queue hands out order
  * positions from the head, position p to the XCD queue p % 8 (environment
  * POLYSPLAT_ORDER_QUEUES=contiguous: each XCD a contiguous eighth), each tile

@@ -531,6 +531,51 @@ void render_16x16_reordered_v2(int num_rendered, int width, int height, Tensor& 
         "render_16x16_reordered_v2");
 }
 
+// Top-k cache renders (pybind.cpp:366-437): values may hold 0x80000000 |
+// slot for the compact top-k feature arrays.
+void topk_common(int num_rendered, int width, int height, Tensor& xy, Tensor& rgbd, Tensor& con,
+                 Tensor& keys_sorted, Tensor& values_sorted, Tensor& topk_xy, Tensor& topk_rgbd,
+                 Tensor& topk_con, int num_topk, Tensor& bg_color, Tensor& out, bool persistent,
+                 const char* what) {
+  check_feats(xy, rgbd, con);
+  need_dev(keys_sorted, "gaussian_keys_sorted", at::kLong);
+  need_dev(values_sorted, "gaussian_values_sorted", at::kInt);
+  need(num_topk >= 0, std::string(what) + ": num_topk must be >= 0");
+  if (num_topk > 0) {
+    check_feats(topk_xy, topk_rgbd, topk_con);
+    need(topk_xy.numel() >= 2LL * num_topk && topk_rgbd.numel() >= 4LL * num_topk &&
+             topk_con.numel() >= 4LL * num_topk,
+         std::string(what) + ": top-k arrays smaller than num_topk");
+  }
+  check_out(out, width, height);
+  auto bg = host_floats(bg_color, 3, "bg_color");
+  Tensor q = cached_workspace(out.device(), 2048, "queue");
+  check(ps_render_16x16_topk(num_rendered, nullptr, width, height, ptr<float>(xy),
+                             ptr<float>(rgbd), ptr<float>(con), ptr<uint64_t>(keys_sorted),
+                             ptr<uint32_t>(values_sorted),
+                             num_topk > 0 ? ptr<float>(topk_xy) : nullptr,
+                             num_topk > 0 ? ptr<float>(topk_rgbd) : nullptr,
+                             num_topk > 0 ? ptr<float>(topk_con) : nullptr, num_topk,
+                             persistent ? 1 : 0, bg.data(), ptr<uint8_t>(out), ptr<unsigned>(q),
+                             cur_stream()),
+        what);
+}
+
+void render_topk_smem(int num_rendered, int width, int height, Tensor& xy, Tensor& rgbd,
+                      Tensor& con, Tensor& ks, Tensor& vs, Tensor& txy, Tensor& trgbd,
+                      Tensor& tcon, int num_topk, Tensor& bg, Tensor& out) {
+  topk_common(num_rendered, width, height, xy, rgbd, con, ks, vs, txy, trgbd, tcon, num_topk, bg,
+              out, false, "render_16x16_topk_smem");
+}
+
+void render_topk_smem_persistent(int num_rendered, int width, int height, Tensor& xy,
+                                 Tensor& rgbd, Tensor& con, Tensor& ks, Tensor& vs, Tensor& txy,
+                                 Tensor& trgbd, Tensor& tcon, int num_topk, Tensor& bg,
+                                 Tensor& out) {
+  topk_common(num_rendered, width, height, xy, rgbd, con, ks, vs, txy, trgbd, tcon, num_topk, bg,
+              out, true, "render_16x16_topk_smem_persistent");
+}
+
 // Persistent render over packed tile descriptors, from FP32 features
 // (`records` undefined or empty) or 24-byte records (feat_prec 1 or 2).
 void reordered_persistent_common(int num_rendered, int width, int height, Tensor* xy,
@@ -967,6 +1012,9 @@ PYBIND11_MODULE(_C, m) {
   ops.def("compute_tile_order_packed_morton", &compute_tile_order_packed_morton);
   ops.def("render_16x16_reordered", &render_16x16_reordered);
   ops.def("render_16x16_reordered_v2", &render_16x16_reordered_v2);
+  ops.def("render_16x16_topk_smem", &render_topk_smem);
+  ops.def("render_16x16_topk_smem_persistent", &render_topk_smem_persistent);
+  ops.def("render_16x16_topk_smem_persistent_v2", &render_topk_smem_persistent);
   ops.def("render_16x16_reordered_persistent", &render_16x16_reordered_persistent);
   ops.def("render_records_reordered", &render_records_reordered, py::arg("num_rendered"),
           py::arg("width"), py::arg("height"), py::arg("records"), py::arg("feat_prec"),

@@ -406,6 +406,17 @@ __device__ __forceinline__ void bind_camera(FrameDev& F) {
   if (F.cam_dev) F.cp = *F.cam_dev;
 }
 
+// Top-k cache (render.cu:396-513, :600-623, example.py:168-197): the FP32
+// features of the most-referenced Gaussians in compact arrays; a sorted
+// value with bit 31 set names slot (value & 0x7fffffff) of them instead of a
+// Gaussian id.  The compact arrays (40 bytes per slot) stay resident in L2.
+struct TopKCache {
+  const float2* xy = nullptr;
+  const float4* rgbd = nullptr;
+  const float4* con = nullptr;
+  int n = 0;
+};
+
 // Per-frame constants of a frame replayed from a captured graph, in device
 // memory: the kernels read them instead of their (captured) arguments.
 struct FrameConsts {

@@ -131,7 +131,7 @@ int render_plain_launch(int n_host, const int* d_n, int W, int H, const float* x
                         const uint32_t* order, const float bg[3], uint8_t* out,
                         hipStream_t stream, const int* ranges = nullptr,
                         const uint32_t* tile_order = nullptr, const int* tile_desc = nullptr,
-                        const FrameConsts* fc = nullptr);
+                        const FrameConsts* fc = nullptr, const TopKCache& tk = TopKCache());
 // Tile order (tile_order.hip): tiles by pair count, descending, stable;
 // bucket > 0 selects the Morton-bucket keys; desc (int4 per position,
 // zig-zag regrouped when zigzag > 1) is optional.
@@ -144,14 +144,16 @@ size_t tile_order_temp_size(int tiles);
 // tile_desc: NULL, or int4 (column, row, start, end) per order position,
 // which then replaces tile_ranges and tile_order; fc: NULL, or per-frame
 // constants in device memory whose background and image pointer the kernel
-// reads at run time instead of bg and out (frames replayed from a graph).
+// reads at run time instead of bg and out (frames replayed from a graph);
+// tk: the top-k feature cache that packed values refer to (FP32 features).
 int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_prec,
                              const float* xy, const float* rgbd, const float* con, const void* rec,
                              int gathered, const uint64_t* keys, const uint32_t* order,
                              const int* ranges, int dt, const float bg[3], uint8_t* out,
                              unsigned* queue, hipStream_t stream,
                              const uint32_t* tile_order = nullptr,
-                             const int* tile_desc = nullptr, const FrameConsts* fc = nullptr);
+                             const int* tile_desc = nullptr, const FrameConsts* fc = nullptr,
+                             const TopKCache& tk = TopKCache());
 int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles,
                             const uint64_t* keys, int* ranges, hipStream_t stream,
                             uint32_t mask = 0xffffffffu, int* zero_counter = nullptr,

@@ -79,13 +79,29 @@ struct Feat {
   float x, y, w, a, b, c, r, g, bl;
 };
 
+// Stands in for the slots past the end of a tile's list.
+__device__ __forceinline__ Feat neutral_feat() {
+  Feat f;
+  f.x = f.y = f.a = f.b = f.c = f.r = f.g = f.bl = 0.0f;
+  f.w = -__builtin_inff();
+  return f;
+}
+
 template <int PREC>
 __device__ __forceinline__ Feat load_feat(int idx, const float2* __restrict__ xy,
                                           const float4* __restrict__ rgbd,
                                           const float4* __restrict__ con,
-                                          const FeatRec16* __restrict__ rec) {
+                                          const FeatRec16* __restrict__ rec, const TopKCache& tk) {
   Feat f;
   if (PREC == kFeatF32) {
+    if (tk.xy != nullptr && idx < 0) {  // a cached Gaussian (bit 31 set)
+      const int slot = idx & 0x7fffffff;
+      if (slot >= tk.n) return neutral_feat();  // not a slot of the cache: nothing
+      xy = tk.xy;
+      rgbd = tk.rgbd;
+      con = tk.con;
+      idx = slot;
+    }
     const float2 p = xy[idx];
     const float4 q = con[idx];
     const float4 c = rgbd[idx];
@@ -109,14 +125,6 @@ __device__ __forceinline__ Feat load_feat(int idx, const float2* __restrict__ xy
     f.a = v[0] * inv; f.b = v[1] * inv; f.c = v[2] * inv;
     f.r = v[3]; f.g = v[4]; f.bl = v[5];
   }
-  return f;
-}
-
-// Stands in for the slots past the end of a tile's list.
-__device__ __forceinline__ Feat neutral_feat() {
-  Feat f;
-  f.x = f.y = f.a = f.b = f.c = f.r = f.g = f.bl = 0.0f;
-  f.w = -__builtin_inff();
   return f;
 }
 
@@ -223,8 +231,9 @@ template <int PREC, bool GATHERED, int CHECK, bool DT>
 __device__ __forceinline__ void render_tile(
     int tile, int start, int end, int W, int H, int gx, const uint32_t* __restrict__ order,
     const float2* __restrict__ xy, const float4* __restrict__ rgbd, const float4* __restrict__ con,
-    const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
-    float4 (*sA)[kBatch], float4 (*sB)[kBatch], float (*sC)[kBatch], int lane) {
+    const FeatRec16* __restrict__ rec, const TopKCache& tk, float3 bg,
+    uint8_t* __restrict__ out, float4 (*sA)[kBatch], float4 (*sB)[kBatch], float (*sC)[kBatch],
+    int lane) {
   static_assert(kBatch % CHECK == 0, "a batch holds whole check groups");
   const int tx = tile % gx, ty = tile / gx;
   // The tile's centre, and this lane's pixel offsets from it.
@@ -241,7 +250,7 @@ __device__ __forceinline__ void render_tile(
     Feat pre = neutral_feat();
     if (lane < n) {
       const int idx = GATHERED ? start + lane : (int)order[start + lane];
-      pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
+      pre = load_feat<PREC>(idx, xy, rgbd, con, rec, tk);
     }
     int cur = 0;
     {
@@ -259,7 +268,7 @@ __device__ __forceinline__ void render_tile(
     pre = neutral_feat();
     if (kBatch + lane < n) {
       const int idx = GATHERED ? start + kBatch + lane : (int)order[start + kBatch + lane];
-      pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
+      pre = load_feat<PREC>(idx, xy, rgbd, con, rec, tk);
     }
     __syncthreads();
     for (int b0 = 0;; b0 += kBatch) {
@@ -349,7 +358,7 @@ __device__ __forceinline__ void render_tile(
       const int nb = b0 + 2 * kBatch + lane;
       if (nb < n) {
         const int idx = GATHERED ? start + nb : (int)order[start + nb];
-        pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
+        pre = load_feat<PREC>(idx, xy, rgbd, con, rec, tk);
       }
       __syncthreads();
     }
@@ -385,8 +394,9 @@ template <int PREC, bool GATHERED, int CHECK, bool DT>
 __device__ __forceinline__ void render_quad(
     int tile, int q, int start, int end, int W, int H, int gx, const uint32_t* __restrict__ order,
     const float2* __restrict__ xy, const float4* __restrict__ rgbd, const float4* __restrict__ con,
-    const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
-    float4 (*sA)[kBatch], float4 (*sB)[kBatch], float (*sC)[kBatch], int lane) {
+    const FeatRec16* __restrict__ rec, const TopKCache& tk, float3 bg,
+    uint8_t* __restrict__ out, float4 (*sA)[kBatch], float4 (*sB)[kBatch], float (*sC)[kBatch],
+    int lane) {
   const int tx = tile % gx, ty = tile / gx;
   const float cx = (float)(tx * kTile) + 7.5f, cy = (float)(ty * kTile) + 7.5f;
   const float u = (float)(lane & 7) + 8.0f * (float)(q & 1) - 7.5f;
@@ -397,7 +407,7 @@ __device__ __forceinline__ void render_quad(
     Feat pre = neutral_feat();
     if (lane < n) {
       const int idx = GATHERED ? start + lane : (int)order[start + lane];
-      pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
+      pre = load_feat<PREC>(idx, xy, rgbd, con, rec, tk);
     }
     int cur = 0;
     {
@@ -410,7 +420,7 @@ __device__ __forceinline__ void render_quad(
     pre = neutral_feat();
     if (kBatch + lane < n) {
       const int idx = GATHERED ? start + kBatch + lane : (int)order[start + kBatch + lane];
-      pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
+      pre = load_feat<PREC>(idx, xy, rgbd, con, rec, tk);
     }
     __syncthreads();
     for (int b0 = 0;; b0 += kBatch) {
@@ -451,7 +461,7 @@ __device__ __forceinline__ void render_quad(
       const int nb = b0 + 2 * kBatch + lane;
       if (nb < n) {
         const int idx = GATHERED ? start + nb : (int)order[start + nb];
-        pre = load_feat<PREC>(idx, xy, rgbd, con, rec);
+        pre = load_feat<PREC>(idx, xy, rgbd, con, rec, tk);
       }
       __syncthreads();
     }
@@ -480,7 +490,7 @@ __global__ __launch_bounds__(kWave, 6) void ps_render_plain(
     const float4* __restrict__ rgbd, const float4* __restrict__ con,
     const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
     const int2* __restrict__ ranges, const uint32_t* __restrict__ tile_order,
-    const int4* __restrict__ tile_desc, const FrameConsts* __restrict__ fc) {
+    const int4* __restrict__ tile_desc, const FrameConsts* __restrict__ fc, TopKCache tk) {
   __shared__ float4 sA[2][kBatch];
   __shared__ float4 sB[2][kBatch];
   __shared__ float sC[2][kBatch];
@@ -515,8 +525,8 @@ __global__ __launch_bounds__(kWave, 6) void ps_render_plain(
     end = wave_bound(keys, n, (uint32_t)tile, true, lane);
   }
   }
-  render_tile<PREC, false, CHECK, false>(tile, start, end, W, H, gx, order, xy, rgbd, con, rec, bg,
-                                         out, sA, sB, sC, lane);
+  render_tile<PREC, false, CHECK, false>(tile, start, end, W, H, gx, order, xy, rgbd, con, rec, tk,
+                                         bg, out, sA, sB, sC, lane);
 }
 
 // Tile queue of the persistent renderer: one claim counter per XCD (128 bytes
@@ -600,7 +610,8 @@ __global__ __launch_bounds__(kWave, 7) void ps_render_persistent(
     const float4* __restrict__ rgbd, const float4* __restrict__ con,
     const FeatRec16* __restrict__ rec, float3 bg, uint8_t* __restrict__ out,
     unsigned* __restrict__ queue, const uint32_t* __restrict__ tile_order,
-    const int4* __restrict__ tile_desc, int interleave, const FrameConsts* __restrict__ fc) {
+    const int4* __restrict__ tile_desc, int interleave, const FrameConsts* __restrict__ fc,
+    TopKCache tk) {
   __shared__ float4 sA[2][kBatch];
   __shared__ float4 sB[2][kBatch];
   __shared__ float sC[2][kBatch];
@@ -640,10 +651,10 @@ __global__ __launch_bounds__(kWave, 7) void ps_render_persistent(
     }
     if (quads)
       render_quad<PREC, GATHERED, CHECK, DT>(tile, item & 3, start, end, W, H, gx, order, xy,
-                                             rgbd, con, rec, bg, out, sA, sB, sC, lane);
+                                             rgbd, con, rec, tk, bg, out, sA, sB, sC, lane);
     else
       render_tile<PREC, GATHERED, CHECK, DT>(tile, start, end, W, H, gx, order, xy, rgbd, con,
-                                             rec, bg, out, sA, sB, sC, lane);
+                                             rec, tk, bg, out, sA, sB, sC, lane);
   }
   tq.finish();
 }
@@ -674,7 +685,7 @@ __global__ __launch_bounds__(kWave, 6) void ps_render_persistent_batch(
     const FrameDev& F = frames[lo];
     const int2 r = ranges[tile];
     render_tile<PREC, false, CHECK, false>(tile - F.tile0, r.x, r.y, F.cp.W, F.cp.H, F.cp.grid_x,
-                                           order, F.xy, F.rgbd, F.conic, F.rec,
+                                           order, F.xy, F.rgbd, F.conic, F.rec, TopKCache{},
                                            make_float3(F.bg[0], F.bg[1], F.bg[2]), F.out, sA, sB,
                                            sC, lane);
     unsigned next = 0;
@@ -765,6 +776,7 @@ struct PersistArgs {
   const uint32_t* tile_order;  // NULL: tiles in row-major order
   const int* tile_desc;        // NULL, or int4 (column, row, start, end) per position
   const FrameConsts* fc;       // NULL, or background and image in device memory
+  TopKCache tk;                // top-k feature cache (FP32 features only)
 };
 
 template <int PREC, bool RANGES, bool GATHERED, int CHECK, bool DT>
@@ -780,7 +792,8 @@ int launch_persistent(const PersistArgs& a, hipStream_t stream) {
                      reinterpret_cast<const int2*>(a.ranges), a.W, a.H, gx, tiles, quads, a.order,
                      (const float2*)a.xy, (const float4*)a.rgbd, (const float4*)a.con,
                      (const FeatRec16*)a.rec, a.bg, a.out, a.queue, a.tile_order,
-                     reinterpret_cast<const int4*>(a.tile_desc), order_queues_interleaved(), a.fc);
+                     reinterpret_cast<const int4*>(a.tile_desc), order_queues_interleaved(), a.fc,
+                     a.tk);
   PS_LAUNCH_CHECK();
   return PS_OK;
 }
@@ -831,7 +844,7 @@ int render_plain_launch(int n_host, const int* d_n, int W, int H, const float* x
                         const float* rgbd, const float* con, const uint64_t* keys,
                         const uint32_t* order, const float bg[3], uint8_t* out,
                         hipStream_t stream, const int* ranges, const uint32_t* tile_order,
-                        const int* tile_desc, const FrameConsts* fc) {
+                        const int* tile_desc, const FrameConsts* fc, const TopKCache& tk) {
   PS_REQUIRE(W > 0 && H > 0, "bad image size");
   PS_REQUIRE((out != nullptr || fc != nullptr) && bg != nullptr, "output or background is NULL");
   PS_REQUIRE(d_n != nullptr || n_host >= 0, "negative num_rendered");
@@ -845,9 +858,34 @@ int render_plain_launch(int n_host, const int* d_n, int W, int H, const float* x
                      n_host, W, H, gx, order, (const float2*)xy, (const float4*)rgbd,
                      (const float4*)con, (const FeatRec16*)nullptr, b, out,
                      reinterpret_cast<const int2*>(ranges), tile_order,
-                     reinterpret_cast<const int4*>(tile_desc), fc);
+                     reinterpret_cast<const int4*>(tile_desc), fc, tk);
   PS_LAUNCH_CHECK();
   return PS_OK;
+}
+
+// Top-k renders (render.cu:6698-6798): the values may name slots of the
+// cache (bit 31); ranges from an in-kernel search of the keys; saturation
+// test every 2 Gaussians as the reference's top-k kernels do
+// (render.cu:2780-2847).  Plain: one workgroup per tile; persistent: the
+// tile queue.
+int render_topk_launch(int n_host, const int* d_n, int W, int H, const float* xy,
+                       const float* rgbd, const float* con, const uint64_t* keys,
+                       const uint32_t* values, const TopKCache& tk, bool persistent,
+                       const float bg[3], uint8_t* out, unsigned* queue, hipStream_t stream) {
+  PS_REQUIRE(W > 0 && H > 0, "bad image size");
+  PS_REQUIRE(out != nullptr && bg != nullptr && keys != nullptr && values != nullptr,
+             "NULL output, background, keys or values");
+  PS_REQUIRE(xy && rgbd && con, "f32 feature arrays are NULL");
+  PS_REQUIRE(tk.n >= 0 && (tk.n == 0 || (tk.xy && tk.rgbd && tk.con)),
+             "top-k arrays are NULL for %d slots", tk.n);
+  PS_REQUIRE(d_n != nullptr || n_host >= 0, "negative num_rendered");
+  if (!persistent)
+    return render_plain_launch(n_host, d_n, W, H, xy, rgbd, con, keys, values, bg, out, stream,
+                               nullptr, nullptr, nullptr, nullptr, tk);
+  PS_REQUIRE(queue != nullptr, "queue is NULL");
+  PersistArgs a{keys, d_n, n_host, nullptr, W, H, values, xy, rgbd, con, nullptr,
+                make_float3(bg[0], bg[1], bg[2]), out, queue, nullptr, nullptr, nullptr, tk};
+  return launch_persistent<kFeatF32, false, false, 2, false>(a, stream);
 }
 
 int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_prec,
@@ -855,7 +893,7 @@ int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_
                              int gathered, const uint64_t* keys, const uint32_t* order,
                              const int* ranges, int dt, const float bg[3], uint8_t* out,
                              unsigned* queue, hipStream_t stream, const uint32_t* tile_order,
-                             const int* tile_desc, const FrameConsts* fc) {
+                             const int* tile_desc, const FrameConsts* fc, const TopKCache& tk) {
   PS_REQUIRE(W > 0 && H > 0, "bad image size");
   PS_REQUIRE((out != nullptr || fc != nullptr) && bg != nullptr && queue != nullptr,
              "NULL output/bg/queue");
@@ -864,7 +902,7 @@ int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_
   PS_REQUIRE(gathered || order != nullptr, "values_sorted is NULL");
   PS_REQUIRE(d_n != nullptr || n_host >= 0, "negative num_rendered");
   PersistArgs a{keys, d_n,  n_host, ranges, W, H, order, xy, rgbd, con, rec,
-                make_float3(bg[0], bg[1], bg[2]), out, queue, tile_order, tile_desc, fc};
+                make_float3(bg[0], bg[1], bg[2]), out, queue, tile_order, tile_desc, fc, tk};
   const bool g = gathered != 0, d = dt != 0;
   switch (feat_prec) {
     case PS_PREC_F32:
@@ -1009,4 +1047,22 @@ extern "C" int ps_render_16x16_reordered_desc(int num_rendered, const int* d_num
                                  conic_opacity, nullptr, values_sorted, bg, out_color,
                                  reinterpret_cast<hipStream_t>(stream), nullptr, nullptr,
                                  tile_desc);
+}
+
+extern "C" int ps_render_16x16_topk(int num_rendered, const int* d_num_rendered, int width,
+                                    int height, const float* points_xy, const float* rgb_depth,
+                                    const float* conic_opacity, const uint64_t* keys_sorted,
+                                    const uint32_t* values_sorted, const float* topk_xy,
+                                    const float* topk_rgb_depth, const float* topk_conic,
+                                    int num_topk, int persistent, const float bg[3],
+                                    uint8_t* out_color, unsigned* queue, ps_stream_t stream) {
+  ps::TopKCache tk;
+  tk.xy = reinterpret_cast<const float2*>(topk_xy);
+  tk.rgbd = reinterpret_cast<const float4*>(topk_rgb_depth);
+  tk.con = reinterpret_cast<const float4*>(topk_conic);
+  tk.n = num_topk;
+  if (num_topk == 0) tk.xy = nullptr;  // nothing cached: values are plain ids
+  return ps::render_topk_launch(num_rendered, d_num_rendered, width, height, points_xy, rgb_depth,
+                                conic_opacity, keys_sorted, values_sorted, tk, persistent != 0,
+                                bg, out_color, queue, reinterpret_cast<hipStream_t>(stream));
 }

@@ -27,6 +27,11 @@ render_variant                                 pipeline
 "preranges_smem_persistent[_morton..|_zigzag..  FP16 SH -> ... -> the same ordered
  |_adaptive]"                                  persistent render (example.py:326-359,
                                                :646-669)
+"topk_smem", "topk_smem_persistent"           FP32 SH -> sort -> top-k cache of the most
+                                               referenced Gaussians' features, values
+                                               packed (bit 31 | slot) -> plain / persistent
+                                               render (example.py:168-197, :676-697)
+"topk_smem_persistent_v2"                      the persistent top-k renderer with no cache
 "auto"                                         resolves to the fastest of the above
 =============================================  ==========================================
 
@@ -49,7 +54,8 @@ POLYSPLAT_ES_FORCE_TWO_PASS=1 forces it), never on a first frame.
 The reordered variants change only the order in which tiles are rendered;
 their images are identical to the unordered renderers' (the persistent ones
 render 24-byte records when ``feature_dtype`` is fp16/bf16, the plain ones
-always FP32 features).  Other variants of the reference (top-k caches, split
+always FP32 features).  The top-k variants leave gaussian_values_sorted
+packed, as the reference does.  Other variants of the reference (split
 warps, other tile shapes) are not part of this build and raise ValueError.
 """
 from __future__ import annotations
@@ -69,6 +75,8 @@ VARIANT_IDS = {
     "preranges_smem_persistent_lite_e2e": 2,
 }
 STAGED = ("preranges_smem_persistent_lite", "preranges_smem_persistent_lite_dt")
+TOPK = ("topk_smem", "topk_smem_persistent", "topk_smem_persistent_v2")
+SMEM_TOPK_GAUSSIANS = 1365  # cached Gaussians of the top-k variants (example.py:9)
 AUTO_CHOICE = "preranges_smem_persistent_lite_e2e_preranges"
 DEPTH_FIRST = 0x100  # PS_FORWARD_DEPTH_FIRST: OR-ed into a ps_forward variant
 
@@ -248,6 +256,13 @@ class Rasterizer:
         self.tile_order_sort_temp = torch.zeros(ops.get_tile_order_sort_temp_size(T), device=dev,
                                                 dtype=torch.int8)
         self.tile_desc_buf = torch.zeros((T, 4), device=dev, dtype=torch.int32)
+        # Top-k feature cache (example.py:120-123).
+        self.smem_topk_lookup = torch.full((max(P, 1),), -1, device=dev, dtype=torch.int16)
+        self.smem_topk_xy = torch.zeros((SMEM_TOPK_GAUSSIANS, 2), device=dev, dtype=torch.float32)
+        self.smem_topk_rgb_depth = torch.zeros((SMEM_TOPK_GAUSSIANS, 4), device=dev,
+                                               dtype=torch.float32)
+        self.smem_topk_conic = torch.zeros((SMEM_TOPK_GAUSSIANS, 4), device=dev,
+                                           dtype=torch.float32)
 
     def _get_out_color(self, height, width, device):
         if (self._out_color is None or self._out_color.shape[0] != height
@@ -296,13 +311,13 @@ class Rasterizer:
         tiles = ((camera.width + 15) // 16) * ((camera.height + 15) // 16)
         order_plan = tile_order_plan(render_variant, tiles)
         if (render_variant not in VARIANT_IDS and render_variant not in STAGED
-                and order_plan is None):
+                and render_variant not in TOPK and order_plan is None):
             raise ValueError(f"render_variant {render_variant!r} is not implemented")
         self._check_tiles(camera, block_x, block_y)
         es = self._decide_use_es(scene, use_es)
         out_color = self._get_out_color(camera.height, camera.width, scene.device)
         times = {"preprocess_ms": None, "sort_ms": None, "render_ms": None}
-        if order_plan is not None:
+        if order_plan is not None or render_variant in TOPK:
             num_rendered = self._forward_staged(scene, camera, bg_color, render_variant,
                                                 out_color, measure_preprocess, measure_sort,
                                                 measure_render, times, es, order_plan)
@@ -356,8 +371,8 @@ class Rasterizer:
         # The plain renderers (default, and reordered over a tile order or
         # descriptors) read FP32 features; the reordered variants take FP32
         # SH (example.py:524-531), the preranges_smem_persistent ones FP16.
-        fp32_path = render_variant == "default" or (order_plan is not None
-                                                    and order_plan[0] != "persistent")
+        fp32_path = (render_variant == "default" or render_variant in TOPK
+                     or (order_plan is not None and order_plan[0] != "persistent"))
         prec = 0 if fp32_path else _PREC[self.feature_dtype]
         half_sh = not fp32_path and (order_plan is None
                                      or render_variant.startswith("preranges_smem_persistent"))
@@ -403,14 +418,26 @@ class Rasterizer:
                               self.gaussian_values_sorted)
         if m_sort:
             e[3].record()
-        if not fp32_path or order_plan is not None:
+        if (not fp32_path or order_plan is not None) and render_variant not in TOPK:
             ops.precompute_tile_ranges(num_rendered, camera.width, camera.height, 16, 16,
                                        self.gaussian_keys_sorted, self.tile_ranges_buf)
         if order_plan is not None:
             self._tile_order(camera, order_plan)
+        num_topk = 0
+        if render_variant in ("topk_smem", "topk_smem_persistent"):
+            num_topk = self._prepare_topk_smem_cache(scene, num_rendered)
         if m_render:
             e[4].record()
-        if order_plan is not None:
+        if render_variant in TOPK:
+            fn = {"topk_smem": ops.render_16x16_topk_smem,
+                  "topk_smem_persistent": ops.render_16x16_topk_smem_persistent,
+                  "topk_smem_persistent_v2": ops.render_16x16_topk_smem_persistent_v2}
+            fn[render_variant](num_rendered, camera.width, camera.height, self.points_xy,
+                               self.rgb_depth, self.conic_opacity, self.gaussian_keys_sorted,
+                               self.gaussian_values_sorted, self.smem_topk_xy,
+                               self.smem_topk_rgb_depth, self.smem_topk_conic, num_topk,
+                               bg_color, out_color)
+        elif order_plan is not None:
             self._render_reordered(num_rendered, camera, bg_color, out_color, order_plan, prec,
                                    render_variant.endswith("_dt"))
         elif fp32_path:
@@ -485,3 +512,29 @@ class Rasterizer:
                                                   self.gaussian_keys_sorted,
                                                   self.gaussian_values_sorted,
                                                   self.tile_desc_buf, bg_color, out_color)
+
+    def _prepare_topk_smem_cache(self, scene, num_rendered):
+        """The SMEM_TOPK_GAUSSIANS Gaussians referenced by the most pairs:
+        their FP32 features copied to the compact arrays, and every sorted
+        value naming one of them replaced by 0x80000000 | its slot
+        (example.py:168-197).  Which of equally referenced Gaussians are
+        cached does not change the image."""
+        num_topk = min(SMEM_TOPK_GAUSSIANS, scene.num_vertex)
+        if num_topk <= 0 or num_rendered <= 0:
+            return 0
+        rendered_ids = self.gaussian_values_sorted[:num_rendered].to(torch.int64)
+        counts = torch.bincount(rendered_ids, minlength=scene.num_vertex)
+        topk_ids = torch.topk(counts, k=num_topk, largest=True, sorted=False).indices
+        self.smem_topk_lookup.fill_(-1)
+        self.smem_topk_lookup[topk_ids] = torch.arange(num_topk, device=scene.device,
+                                                       dtype=torch.int16)
+        vals = self.gaussian_values_sorted[:num_rendered]
+        slots = self.smem_topk_lookup[vals.to(torch.int64)].to(torch.int32)
+        hit = slots >= 0
+        packed = vals.clone()
+        packed[hit] = slots[hit] | (-(1 << 31))
+        vals.copy_(packed)
+        self.smem_topk_xy[:num_topk].copy_(torch.index_select(self.points_xy, 0, topk_ids))
+        self.smem_topk_rgb_depth[:num_topk].copy_(torch.index_select(self.rgb_depth, 0, topk_ids))
+        self.smem_topk_conic[:num_topk].copy_(torch.index_select(self.conic_opacity, 0, topk_ids))
+        return num_topk

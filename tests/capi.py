@@ -100,6 +100,7 @@ def bind(L):
         "ps_compute_tile_order_packed": ([P, I, I, P, P, P, P, P, S, I, I, P], I),
         "ps_render_16x16_reordered": ([I, P, I, I, P, P, P, P, P, P, P, P, P, P], I),
         "ps_render_16x16_reordered_desc": ([I, P, I, I, P, P, P, P, P, P, P, P], I),
+        "ps_render_16x16_topk": ([I, P, I, I, P, P, P, P, P, P, P, P, I, I, P, P, P, P], I),
         "ps_render_16x16_reordered_persistent": ([I, P, I, I, I, P, P, P, P, P, P, P, P, I, P,
                                                   P, P, P], I),
         "ps_frame_graph_create": ([I, P, P, I, P, P, P, I, I, P, P], I),

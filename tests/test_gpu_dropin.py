@@ -433,7 +433,7 @@ def test_rasterizer_auto_and_unsupported(case):
     img, stats = rast.forward(scene, spec, _bg(), render_variant="auto", return_stats=True)
     assert stats["requested_render_variant"] == "auto"
     assert stats["num_rendered"] == ref["pre"]["M"]
-    for bad in ("topk_smem", "reordered_persistent_hilbert", "split"):
+    for bad in ("topk_smem_v3", "reordered_persistent_hilbert", "split"):
         with pytest.raises(ValueError):
             rast.forward(scene, spec, _bg(), render_variant=bad)
     with pytest.raises(ValueError):
