@@ -225,15 +225,22 @@ void orc_sh_color(const float p[3], const float cp[3], const float sh[48], float
   int ch;
   for (ch = 0; ch < 3; ++ch) {
     const float* v = sh + ch; /* coefficient k at v[3k] */
-    float r = SH_C0 * v[0] + 0.5f;
-    r = r - SH_C1 * y * v[3] + SH_C1 * z * v[6] - SH_C1 * x * v[9];
-    r = r + SH_C2[0] * xy * v[12] + SH_C2[1] * yz * v[15] + SH_C2[2] * (2.0f * zz - xx - yy) * v[18] +
-        SH_C2[3] * xz * v[21] + SH_C2[4] * (xx - yy) * v[24];
-    r = r + SH_C3[0] * y * (3.0f * xx - yy) * v[27] + SH_C3[1] * xy * z * v[30] +
-        SH_C3[2] * y * (4.0f * zz - xx - yy) * v[33] +
-        SH_C3[3] * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * v[36] +
-        SH_C3[4] * x * (4.0f * zz - xx - yy) * v[39] + SH_C3[5] * z * (xx - yy) * v[42] +
-        SH_C3[6] * x * (xx - 3.0f * yy) * v[45];
+    /* basis value of coefficient k, each product in the reference's
+     * left-to-right order; the degree-1 terms it subtracts are negated */
+    const float b[16] = {SH_C0, -(SH_C1 * y), SH_C1 * z, -(SH_C1 * x),
+                         SH_C2[0] * xy, SH_C2[1] * yz, SH_C2[2] * (2.0f * zz - xx - yy),
+                         SH_C2[3] * xz, SH_C2[4] * (xx - yy),
+                         SH_C3[0] * y * (3.0f * xx - yy), SH_C3[1] * xy * z,
+                         SH_C3[2] * y * (4.0f * zz - xx - yy),
+                         SH_C3[3] * z * (2.0f * zz - 3.0f * xx - 3.0f * yy),
+                         SH_C3[4] * x * (4.0f * zz - xx - yy), SH_C3[5] * z * (xx - yy),
+                         SH_C3[6] * x * (xx - 3.0f * yy)};
+    /* the sum r = 0.5 + sum_k b[k] * v[3k], accumulated in term order with one
+     * correctly rounded fused multiply-add per term (the contraction the
+     * reference's compiler applies to `r - b * sh + ...`) */
+    float r = fmaf(b[0], v[0], 0.5f);
+    int k;
+    for (k = 1; k < 16; ++k) r = fmaf(b[k], v[3 * k], r);
     out[ch] = r > 0.0f ? r : 0.0f;
   }
 }

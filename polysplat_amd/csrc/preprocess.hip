@@ -329,33 +329,45 @@ __device__ __forceinline__ void load_cov(const float* cov3, int g, float c[6]) {
   c[0] = a.x; c[1] = a.y; c[2] = b.x; c[3] = b.y; c[4] = d.x; c[5] = d.y;
 }
 
+// The 96 bytes of a Gaussian's 16-bit SH as loaded (two coefficients per
+// word, coefficient-major).
+struct ShRaw16 {
+  uint4 q[6];
+};
+
+__device__ __forceinline__ ShRaw16 load_sh_raw16(const void* shs, int g) {
+  const uint4* p = reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(shs) + (size_t)g * 48);
+  ShRaw16 r;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) r.q[k] = p[k];
+  return r;
+}
+
 template <int SHP>
-__device__ __forceinline__ void load_sh(const void* shs, int g, float sh[48]) {
-  if (SHP == kShF32) {
-    const float4* p = reinterpret_cast<const float4*>(static_cast<const float*>(shs) + (size_t)g * 48);
+__device__ __forceinline__ void sh_from_raw16(const ShRaw16& raw, float sh[48]) {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-      float4 v = p[k];
-      sh[4 * k + 0] = v.x; sh[4 * k + 1] = v.y; sh[4 * k + 2] = v.z; sh[4 * k + 3] = v.w;
-    }
-  } else {
-    const uint4* p = reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(shs) + (size_t)g * 48);
+  for (int k = 0; k < 6; ++k) {
+    const uint32_t w4[4] = {raw.q[k].x, raw.q[k].y, raw.q[k].z, raw.q[k].w};
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      uint4 v = p[k];
-      uint32_t w4[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        uint16_t lo = (uint16_t)(w4[j] & 0xffffu), hi = (uint16_t)(w4[j] >> 16);
-        if (SHP == kShF16) {
-          sh[8 * k + 2 * j] = __half2float(__ushort_as_half(lo));
-          sh[8 * k + 2 * j + 1] = __half2float(__ushort_as_half(hi));
-        } else {
-          sh[8 * k + 2 * j] = bf16_to_f32(lo);
-          sh[8 * k + 2 * j + 1] = bf16_to_f32(hi);
-        }
+    for (int j = 0; j < 4; ++j) {
+      const uint16_t lo = (uint16_t)(w4[j] & 0xffffu), hi = (uint16_t)(w4[j] >> 16);
+      if (SHP == kShF16) {
+        sh[8 * k + 2 * j] = __half2float(__ushort_as_half(lo));
+        sh[8 * k + 2 * j + 1] = __half2float(__ushort_as_half(hi));
+      } else {
+        sh[8 * k + 2 * j] = bf16_to_f32(lo);
+        sh[8 * k + 2 * j + 1] = bf16_to_f32(hi);
       }
     }
+  }
+}
+
+__device__ __forceinline__ void load_sh_f32(const void* shs, int g, float sh[48]) {
+  const float4* p = reinterpret_cast<const float4*>(static_cast<const float*>(shs) + (size_t)g * 48);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    float4 v = p[k];
+    sh[4 * k + 0] = v.x; sh[4 * k + 1] = v.y; sh[4 * k + 2] = v.z; sh[4 * k + 3] = v.w;
   }
 }
 
@@ -377,14 +389,18 @@ __device__ __forceinline__ Proj project_gaussian(const FrameDev& F, int g, float
 }
 
 // Render features of an emitting Gaussian: SH colour, scaled conic, and the
-// FP32 arrays and/or the 16-bit record, whichever the frame asks for.
+// FP32 arrays and/or the 16-bit record, whichever the frame asks for.  16-bit
+// SH arrives already loaded (`raw`); FP32 SH is read here.
 template <int SHP>
 __device__ __forceinline__ void write_features(const FrameDev& F, int g, const Proj& pj, float px,
-                                               float py, float pz) {
+                                               float py, float pz, const ShRaw16& raw) {
   const float4 con = make_float4((-0.5f * kLog2e) * pj.con_a, -kLog2e * pj.con_b,
                                  (-0.5f * kLog2e) * pj.con_c, pj.log2_opacity);
   float sh[48];
-  load_sh<SHP>(F.shs, g, sh);
+  if (SHP == kShF32)
+    load_sh_f32(F.shs, g, sh);
+  else
+    sh_from_raw16<SHP>(raw, sh);
   const F3 rgb = sh_color(px, py, pz, F.cp.campos, sh);
   if (F.xy) F.xy[g] = make_float2(pj.xy_x, pj.xy_y);
   if (F.conic) F.conic[g] = con;
@@ -430,6 +446,12 @@ __device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* _
   const CamParams& cp = F.cp;
   float px, py, pz;
   const Proj pj = project_gaussian(F, g, px, py, pz);
+  // 16-bit SH (96 bytes, 24 registers) of a Gaussian that survived the
+  // culls is loaded here, so that the read, the largest of the pass, overlaps
+  // the tile sweeps instead of following them; FP32 SH (192 bytes) is read
+  // when the features are written.
+  ShRaw16 shr{};
+  if (SHP != kShF32 && pj.valid) shr = load_sh_raw16(F.shs, g);
   const int area = rect_area(pj);
   const bool single_ok = area == 1 && tile_hits(pj.rmin_x, pj.rmin_y, pj, cp.block_x, cp.block_y);
   // Hit-pool words for rects of more than kMaskTiles tiles (see
@@ -477,7 +499,7 @@ __device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* _
                        : 0u;
     recs[(size_t)vb * kPreThreads + tid] = r;
   }
-  if (count > 0) write_features<SHP>(F, g, pj, px, py, pz);
+  if (count > 0) write_features<SHP>(F, g, pj, px, py, pz, shr);
   if (dkey) {  // uniform: the depth-first pipeline's depth words and plan
     if (g < F.P) dkey[g] = count > 0 ? f2u(pj.depth) : 0u;
     __shared__ uint32_t s_or[kPreThreads / kWave], s_and[kPreThreads / kWave];

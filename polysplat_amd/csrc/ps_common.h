@@ -287,7 +287,11 @@ namespace ps {
 // ---------------------------------------------------------------------------
 // Degree-3 SH colour (preprocess.cu:159-255).  `sh` holds 48 floats laid out
 // as 16 coefficients x RGB.  Evaluation order follows the reference so that
-// the FP32 result is reproducible on the host.
+// the FP32 result is reproducible on the host.  Each coefficient is
+// accumulated by one fused multiply-add, in the reference's term order (the
+// contraction its compiler applies to `r - b * sh + ...`); with 16-bit SH the
+// widening of each coefficient folds into the same instruction
+// (v_fma_mix_f32), so a channel costs 16 instructions.
 // ---------------------------------------------------------------------------
 PS_HD F3 sh_color(float px, float py, float pz, const float* campos, const float* sh) {
   float dx = px - campos[0], dy = py - campos[1], dz = pz - campos[2];
@@ -306,13 +310,15 @@ PS_HD F3 sh_color(float px, float py, float pz, const float* campos, const float
   float b13 = kShC3[4] * x * (4.0f * zz - xx - yy);
   float b14 = kShC3[5] * z * (xx - yy);
   float b15 = kShC3[6] * x * (xx - 3.0f * yy);
+  // Basis weights in term order; the degree-1 terms the reference subtracts
+  // enter negated (r - b * s == fma(-b, s, r) exactly).
+  const float basis[16] = {kShC0, -c1y, c1z, -c1x, b4,  b5,  b6,  b7,
+                           b8,    b9,   b10, b11,  b12, b13, b14, b15};
   float out[3];
   for (int c = 0; c < 3; ++c) {
-    float r = kShC0 * sh[c] + 0.5f;
-    r = r - c1y * sh[3 + c] + c1z * sh[6 + c] - c1x * sh[9 + c];
-    r = r + b4 * sh[12 + c] + b5 * sh[15 + c] + b6 * sh[18 + c] + b7 * sh[21 + c] + b8 * sh[24 + c];
-    r = r + b9 * sh[27 + c] + b10 * sh[30 + c] + b11 * sh[33 + c] + b12 * sh[36 + c] +
-        b13 * sh[39 + c] + b14 * sh[42 + c] + b15 * sh[45 + c];
+    float r = fmaf(basis[0], sh[c], 0.5f);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) r = fmaf(basis[k], sh[3 * k + c], r);
     out[c] = fmaxf_(r, 0.0f);
   }
   F3 o;

@@ -206,6 +206,82 @@ def test_sh_single_coefficient_known_answers(k):
         np.testing.assert_allclose(got, ref, atol=3e-6)
 
 
+def _round_f32(q):
+    """A rational rounded once to the nearest float32 (ties to even), in
+    integer arithmetic: the exact result of one fused multiply-add."""
+    from fractions import Fraction
+    q = Fraction(q)
+    if q == 0:
+        return F32(0.0)
+    sign, q = (-1 if q < 0 else 1), abs(q)
+    e = q.numerator.bit_length() - q.denominator.bit_length()
+    if Fraction(2) ** e > q:
+        e -= 1
+    e = max(e, -126)                     # subnormals share the 2^-149 grid
+    scaled = q / Fraction(2) ** (e - 23)  # 24-bit significand grid
+    n, rem = divmod(scaled.numerator, scaled.denominator)
+    if 2 * rem > scaled.denominator or (2 * rem == scaled.denominator and n % 2 == 1):
+        n += 1
+    return F32(sign * float(Fraction(n) * Fraction(2) ** (e - 23)))
+
+
+def _fma32(a, b, c):
+    from fractions import Fraction
+    return _round_f32(Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c)))
+
+
+def test_round_f32_known_answers():
+    from fractions import Fraction
+    assert _round_f32(Fraction(1, 3)) == F32(1) / F32(3)
+    assert _round_f32(1 + Fraction(1, 2 ** 24)) == F32(1.0)           # tie -> even
+    assert _round_f32(1 + Fraction(3, 2 ** 24)) == F32(1 + 2 ** -22)  # tie -> even (up)
+    assert _round_f32(Fraction(2 ** -149)) == np.float32(2 ** -149)
+    # the fused result differs from the two-step one where the product's
+    # low bits matter
+    a = F32(1 + 2 ** -12)  # a * a = 1 + 2^-11 + 2^-24
+    assert _fma32(a, a, F32(-1.0)) == F32(2 ** -11 + 2 ** -24)
+    assert a * a - F32(1.0) == F32(2 ** -11)
+
+
+def test_sh_accumulates_by_fused_multiply_adds_in_term_order():
+    """The colour sum 0.5 + sum_k basis_k * sh_k is one correctly rounded
+    fused multiply-add per term, in term order (the reference's
+    `r - b * sh + ...` contracted), and the basis products keep the
+    reference's left-to-right order.  An unfused chain fails this check."""
+    rng = np.random.default_rng(21)
+    C0, C1 = F32(0.28209479177387814), F32(0.4886025119029199)
+    C2 = [F32(v) for v in (1.0925484305920792, -1.0925484305920792, 0.31539156525252005,
+                           -1.0925484305920792, 0.5462742152960396)]
+    C3 = [F32(v) for v in (-0.5900435899266435, 2.890611442640554, -0.4570457994644658,
+                           0.3731763325901154, -0.4570457994644658, 1.445305721320277,
+                           -0.5900435899266435)]
+    unfused_differs = 0
+    for _ in range(300):
+        p = rng.normal(size=3).astype(F32) * F32(4)
+        sh = (rng.normal(size=48) * 0.6).astype(F32)
+        l2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2]
+        rs = F32(1.0) / np.sqrt(l2, dtype=F32)
+        x, y, z = p[0] * rs, p[1] * rs, p[2] * rs
+        xx, yy, zz, xy, yz, xz = x * x, y * y, z * z, x * y, y * z, x * z
+        two, three, four = F32(2), F32(3), F32(4)
+        b = [C0, -(C1 * y), C1 * z, -(C1 * x), C2[0] * xy, C2[1] * yz,
+             C2[2] * (two * zz - xx - yy), C2[3] * xz, C2[4] * (xx - yy),
+             C3[0] * y * (three * xx - yy), C3[1] * xy * z, C3[2] * y * (four * zz - xx - yy),
+             C3[3] * z * (two * zz - three * xx - three * yy), C3[4] * x * (four * zz - xx - yy),
+             C3[5] * z * (xx - yy), C3[6] * x * (xx - three * yy)]
+        got = O.sh_color(p, np.zeros(3, F32), sh)
+        for ch in range(3):
+            r = _fma32(b[0], sh[ch], F32(0.5))
+            u = b[0] * sh[ch] + F32(0.5)
+            for k in range(1, 16):
+                r = _fma32(b[k], sh[3 * k + ch], r)
+                u = u + b[k] * sh[3 * k + ch]
+            want = max(r, F32(0.0))
+            assert got[ch].view(np.uint32) == want.view(np.uint32), (ch, got[ch], want)
+            unfused_differs += int(max(u, F32(0.0)) != want)
+    assert unfused_differs > 0
+
+
 # ------------------------------------------------------------------ tile coverage
 def _rect_min_q(cx, cy, a, b, c, x0, y0, x1, y1):
     """min over the pixel rectangle [x0,x1]x[y0,y1] of q(d) = a dx^2 + 2b dx dy + c dy^2,
