@@ -389,18 +389,17 @@ __device__ __forceinline__ Proj project_gaussian(const FrameDev& F, int g, float
 }
 
 // Render features of an emitting Gaussian: SH colour, scaled conic, and the
-// FP32 arrays and/or the 16-bit record, whichever the frame asks for.  16-bit
-// SH arrives already loaded (`raw`); FP32 SH is read here.
+// FP32 arrays and/or the 16-bit record, whichever the frame asks for.
 template <int SHP>
 __device__ __forceinline__ void write_features(const FrameDev& F, int g, const Proj& pj, float px,
-                                               float py, float pz, const ShRaw16& raw) {
+                                               float py, float pz) {
   const float4 con = make_float4((-0.5f * kLog2e) * pj.con_a, -kLog2e * pj.con_b,
                                  (-0.5f * kLog2e) * pj.con_c, pj.log2_opacity);
   float sh[48];
   if (SHP == kShF32)
     load_sh_f32(F.shs, g, sh);
   else
-    sh_from_raw16<SHP>(raw, sh);
+    sh_from_raw16<SHP>(load_sh_raw16(F.shs, g), sh);
   const F3 rgb = sh_color(px, py, pz, F.cp.campos, sh);
   if (F.xy) F.xy[g] = make_float2(pj.xy_x, pj.xy_y);
   if (F.conic) F.conic[g] = con;
@@ -446,12 +445,6 @@ __device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* _
   const CamParams& cp = F.cp;
   float px, py, pz;
   const Proj pj = project_gaussian(F, g, px, py, pz);
-  // 16-bit SH (96 bytes, 24 registers) of a Gaussian that survived the
-  // culls is loaded here, so that the read, the largest of the pass, overlaps
-  // the tile sweeps instead of following them; FP32 SH (192 bytes) is read
-  // when the features are written.
-  ShRaw16 shr{};
-  if (SHP != kShF32 && pj.valid) shr = load_sh_raw16(F.shs, g);
   const int area = rect_area(pj);
   const bool single_ok = area == 1 && tile_hits(pj.rmin_x, pj.rmin_y, pj, cp.block_x, cp.block_y);
   // Hit-pool words for rects of more than kMaskTiles tiles (see
@@ -499,7 +492,7 @@ __device__ __forceinline__ void pre_count_body(const FrameDev& F, int vb, int* _
                        : 0u;
     recs[(size_t)vb * kPreThreads + tid] = r;
   }
-  if (count > 0) write_features<SHP>(F, g, pj, px, py, pz, shr);
+  if (count > 0) write_features<SHP>(F, g, pj, px, py, pz);
   if (dkey) {  // uniform: the depth-first pipeline's depth words and plan
     if (g < F.P) dkey[g] = count > 0 ? f2u(pj.depth) : 0u;
     __shared__ uint32_t s_or[kPreThreads / kWave], s_and[kPreThreads / kWave];
@@ -775,18 +768,58 @@ __device__ __forceinline__ void emit_records_block(
   const int off = (int)start + excl;
   const CamParams& cp = F.cp;
   const int w = count > 0 ? (int)(r.x0w >> 24) : 0;
-  if (count > 0 && w > 0) {
-    const int x0 = (int)(r.x0w & 0xffffffu), y0 = (int)r.y0;
+  {
+    // Rects with a hit mask: the wave's pairs form one stream of tasks, 64
+    // per step, task t of lane o's run writing o's hit k = t - (o's start in
+    // the stream), so consecutive lanes write consecutive pairs whatever the
+    // spread of the counts.  Pair k of a Gaussian is the k-th set bit of its
+    // mask (row-major in its rect) at off + k, as a per-lane loop over the
+    // mask would write it.
+    const int c = (count > 0 && w > 0) ? imin(count, __builtin_popcount(r.mask)) : 0;
+    const int incl = wave_incl_scan(c, lane);
+    const int total_w = bcast(incl, 63);
     // t / w for t < 32 <= 2^10 and w <= 32 as (t * ceil(2^16 / w)) >> 16:
     // the rounding adds less than 2^-11 < 1/w to t / w, so the floor is exact.
-    const uint32_t inv_w = (65536u + (uint32_t)w - 1u) / (uint32_t)w;
-    uint32_t m = r.mask;
-    for (int k = 0; m != 0u && k < count; ++k) {
-      const int t = __builtin_ctz(m);
-      m &= m - 1u;
-      const int ty = (int)(((uint32_t)t * inv_w) >> 16), tx = t - ty * w;
-      out.put(off + k, (uint32_t)(F.tile0 + (y0 + ty) * cp.grid_x + (x0 + tx)), r.depth,
-              (uint32_t)g);
+    const uint32_t inv_w = w > 0 ? (65536u + (uint32_t)w - 1u) / (uint32_t)w : 0u;
+    // Owner data, packed: rect origin (16 bits each: the grid has fewer than
+    // 2^16 columns and rows), width and its reciprocal, and where the owner's
+    // task 0 lands (off - its start in the stream).
+    const int xy0 = (int)((r.x0w & 0xffffu) | (r.y0 << 16));
+    const int wv = (int)((uint32_t)w | (inv_w << 8));
+    const int excl_c = incl - c;
+    const int base = off - excl_c;
+    for (int t0 = 0; t0 < total_w; t0 += kWave) {  // uniform: every lane permutes
+      const int t = t0 + lane;
+      int lo = 0, hi = 63;
+#pragma unroll
+      for (int st = 0; st < 6; ++st) {
+        const int mid = (lo + hi) >> 1;
+        if (permute(incl, mid) > t) hi = mid; else lo = mid + 1;
+      }
+      const int o = lo;
+      const int k = t - permute(excl_c, o);
+      uint32_t m = (uint32_t)permute((int)r.mask, o);
+      const uint32_t o_wv = (uint32_t)permute(wv, o), o_xy = (uint32_t)permute(xy0, o);
+      const uint32_t o_depth = (uint32_t)permute((int)r.depth, o);
+      const int o_g = permute(g, o), o_base = permute(base, o);
+      if (t < total_w) {
+        // position of the k-th set bit of m (k < popcount(m))
+        int kk = k, pos = 0;
+#pragma unroll
+        for (int half = 16; half >= 1; half >>= 1) {
+          const int below_half = __builtin_popcount(m & ((1u << half) - 1u));
+          if (kk >= below_half) {
+            kk -= below_half;
+            m >>= half;
+            pos += half;
+          }
+        }
+        const int ow = (int)(o_wv & 0xffu);
+        const int ty = (int)(((uint32_t)pos * (o_wv >> 8)) >> 16), tx = pos - ty * ow;
+        const int ox0 = (int)(o_xy & 0xffffu), oy0 = (int)(o_xy >> 16);
+        out.put(o_base + t, (uint32_t)(F.tile0 + (oy0 + ty) * cp.grid_x + (ox0 + tx)), o_depth,
+                (uint32_t)o_g);
+      }
     }
   }
   const uint64_t below = lanes_below(lane);
