@@ -154,10 +154,12 @@ int render_persistent_launch(int n_host, const int* d_n, int W, int H, int feat_
                              const uint32_t* tile_order = nullptr,
                              const int* tile_desc = nullptr, const FrameConsts* fc = nullptr,
                              const TopKCache& tk = TopKCache());
+// `cleared`: the ranges (and the counter words) were already zeroed on the
+// stream since their last use, so no clearing launch is needed.
 int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles,
                             const uint64_t* keys, int* ranges, hipStream_t stream,
                             uint32_t mask = 0xffffffffu, int* zero_counter = nullptr,
-                            int zero_words = 1);
+                            int zero_words = 1, bool cleared = false);
 // Depth-first pipeline: the depth sort of the Gaussians and the tile sort of
 // their (tile id, Gaussian) pairs (sort.hip).
 int depth_sort_launch(const DepthSortArgs& A, int max_pairs, void* space, size_t space_bytes,

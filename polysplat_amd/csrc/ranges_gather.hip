@@ -82,6 +82,46 @@ __global__ __launch_bounds__(kThreads) void ps_ranges_scan(const uint64_t* __res
   }
 }
 
+// The same boundary scan two keys at a time from one 16-byte load (keys
+// 16-byte aligned): key i's tile id is the high word of its 64-bit key
+// (words 1 and 3 of the load), the key before the pair is read on its own
+// (mostly from cache: the neighbouring lane loaded it).
+__device__ __forceinline__ void mark_boundary(int i, int n, uint32_t prev, uint32_t cur, int tiles,
+                                              int2* __restrict__ ranges) {
+  const bool cur_ok = cur < (uint32_t)tiles;  // keys from a foreign caller may be garbage
+  if (i == 0) {
+    if (cur_ok) ranges[cur].x = 0;
+  } else if (cur != prev) {
+    if (prev < (uint32_t)tiles) ranges[prev].y = i;
+    if (cur_ok) ranges[cur].x = i;
+  }
+  if (i == n - 1 && cur_ok) ranges[cur].y = n;
+}
+
+__global__ __launch_bounds__(kThreads) void ps_ranges_scan2(const uint64_t* __restrict__ keys,
+                                                            const int* __restrict__ d_n,
+                                                            int n_host, int max_pairs, int tiles,
+                                                            uint32_t mask,
+                                                            int2* __restrict__ ranges) {
+  const int n = count_of(d_n, n_host, max_pairs);
+  const int pairs = (n + 1) >> 1;
+  const uint4* k4 = reinterpret_cast<const uint4*>(keys);
+  for (int j = blockIdx.x * kThreads + threadIdx.x; j < pairs; j += gridDim.x * kThreads) {
+    const int i = 2 * j;
+    uint32_t c0, c1 = 0u;
+    if (i + 1 < n) {
+      const uint4 v = k4[j];
+      c0 = v.y & mask;
+      c1 = v.w & mask;
+    } else {
+      c0 = tile_of(keys, i) & mask;
+    }
+    const uint32_t prev = i > 0 ? (tile_of(keys, i - 1) & mask) : 0u;
+    mark_boundary(i, n, prev, c0, tiles, ranges);
+    if (i + 1 < n) mark_boundary(i + 1, n, c0, c1, tiles, ranges);
+  }
+}
+
 // Sorted 32-bit tile ids to the 64-bit keys the renderers read (tile id in
 // the high word, low word zero, as the reference's depth-first pipeline
 // rebuilds them: sort.cu:205-215) and, when `ranges` is given, the tile
@@ -146,15 +186,22 @@ inline int grid_for(int n) {
 
 int tile_ranges_scan_launch(int n_host, const int* d_n, int max_pairs, int tiles,
                             const uint64_t* keys, int* ranges, hipStream_t stream, uint32_t mask,
-                            int* zero_counter, int zero_words) {
+                            int* zero_counter, int zero_words, bool cleared) {
   PS_REQUIRE(zero_words >= 0 && zero_words <= kThreads, "bad counter word count");
-  hipLaunchKernelGGL(ps_fill_int2, dim3(grid_for(tiles)), dim3(kThreads), 0, stream,
-                     reinterpret_cast<int2*>(ranges), tiles, zero_counter, zero_words);
-  PS_LAUNCH_CHECK();
+  if (!cleared) {
+    hipLaunchKernelGGL(ps_fill_int2, dim3(grid_for(tiles)), dim3(kThreads), 0, stream,
+                       reinterpret_cast<int2*>(ranges), tiles, zero_counter, zero_words);
+    PS_LAUNCH_CHECK();
+  }
   const int n = d_n ? max_pairs : n_host;
   if (n > 0) {
-    hipLaunchKernelGGL(ps_ranges_scan, dim3(grid_for(n)), dim3(kThreads), 0, stream, keys, d_n,
-                       n_host, max_pairs, tiles, mask, reinterpret_cast<int2*>(ranges));
+    if ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0u)
+      hipLaunchKernelGGL(ps_ranges_scan2, dim3(grid_for((n + 1) / 2)), dim3(kThreads), 0, stream,
+                         keys, d_n, n_host, max_pairs, tiles, mask,
+                         reinterpret_cast<int2*>(ranges));
+    else
+      hipLaunchKernelGGL(ps_ranges_scan, dim3(grid_for(n)), dim3(kThreads), 0, stream, keys,
+                         d_n, n_host, max_pairs, tiles, mask, reinterpret_cast<int2*>(ranges));
     PS_LAUNCH_CHECK();
   }
   return PS_OK;

@@ -162,12 +162,28 @@ __global__ __launch_bounds__(kSortThreads) void ps_sort_upsweep(const KeyT* __re
                    hist);
 }
 
+// Buffers a sort clears in passing (see ZeroReq).
+struct ZeroReq {
+  int2* ranges = nullptr;  // `n` (0, 0) ranges
+  int n = 0;
+  int* words = nullptr;  // `nwords` (<= 256) counter words
+  int nwords = 0;
+};
+
 // One workgroup per digit: exclusive scan of counts[d][0..kSortBlocks) into
-// offsets[d][...], digit total into totals[d].
+// offsets[d][...], digit total into totals[d].  The last digit pass's scan
+// also clears the buffers of `z` (the tile ranges the boundary scan after
+// the sort fills, and the segment lists' counters), in place of a launch of
+// its own.
 __global__ __launch_bounds__(kSortThreads) void ps_sort_scan(const uint32_t* __restrict__ counts,
                                                              uint32_t* __restrict__ offsets,
-                                                             uint32_t* __restrict__ totals) {
+                                                             uint32_t* __restrict__ totals,
+                                                             ZeroReq z) {
   __shared__ int scratch[4];
+  if (z.ranges)
+    for (int i = blockIdx.x * kSortThreads + threadIdx.x; i < z.n; i += gridDim.x * kSortThreads)
+      z.ranges[i] = make_int2(0, 0);
+  if (z.words && blockIdx.x == 0 && (int)threadIdx.x < z.nwords) z.words[threadIdx.x] = 0;
   constexpr int per = kSortBlocks / kSortThreads;
   const int d = blockIdx.x, tid = threadIdx.x;
   const uint32_t* row = counts + (size_t)d * kSortBlocks;
@@ -1015,11 +1031,11 @@ inline bool use_segmented() {
 template <int DB, typename KeyT>
 void launch_digit_pass(const KeyT* kin, const uint32_t* vin, KeyT* kout, uint32_t* vout,
                        int n_host, const int* d_n, int max_pairs, int spec, const SortWs& w,
-                       hipStream_t stream) {
+                       hipStream_t stream, const ZeroReq& z) {
   hipLaunchKernelGGL((ps_sort_upsweep<DB, KeyT>), dim3(kSortBlocks), dim3(kSortThreads), 0,
                      stream, kin, d_n, n_host, max_pairs, spec, w.counts);
   hipLaunchKernelGGL(ps_sort_scan, dim3(1 << DB), dim3(kSortThreads), 0, stream, w.counts,
-                     w.offsets, w.totals);
+                     w.offsets, w.totals, z);
   hipLaunchKernelGGL((ps_sort_downsweep<DB, KeyT>), dim3(kSortBlocks), dim3(kSortThreads), 0,
                      stream, kin, vin, kout, vout, d_n, n_host, max_pairs, spec, w.offsets,
                      w.totals);
@@ -1027,11 +1043,12 @@ void launch_digit_pass(const KeyT* kin, const uint32_t* vin, KeyT* kout, uint32_
 
 // Stable LSD passes over key bits [bit0, bit1): 8-bit digits, or 9-bit ones
 // when that needs fewer passes; the bits are spread evenly over the passes.
-// `alt_keys`/`alt_vals` hold the pairs between passes.
+// `alt_keys`/`alt_vals` hold the pairs between passes; the last pass's scan
+// clears what `z` names.
 template <typename KeyT>
 int digit_passes(int n_host, const int* d_n, int max_pairs, int bit0, int bit1, const SortWs& w,
                  const KeyT* keys_in, const uint32_t* vals_in, KeyT* keys_out,
-                 uint32_t* vals_out, hipStream_t stream) {
+                 uint32_t* vals_out, hipStream_t stream, const ZeroReq& z = ZeroReq()) {
   const int bits = bit1 - bit0;
   const int p8 = (bits + 7) / 8, p9 = (bits + kMaxDigitBits - 1) / kMaxDigitBits;
   const int passes = p9 < p8 ? p9 : p8;
@@ -1046,10 +1063,13 @@ int digit_passes(int n_host, const int* d_n, int max_pairs, int bit0, int bit1, 
     uint32_t* vout = to_out ? vals_out : w.alt_vals;
     const int width = (bit1 - lo + (passes - p) - 1) / (passes - p);  // <= 9
     const int spec = digit_spec(lo, width);
+    const ZeroReq zp = (p == passes - 1) ? z : ZeroReq();
     if (width > 8)
-      launch_digit_pass<9, KeyT>(kin, vin, kout, vout, n_host, d_n, max_pairs, spec, w, stream);
+      launch_digit_pass<9, KeyT>(kin, vin, kout, vout, n_host, d_n, max_pairs, spec, w, stream,
+                                 zp);
     else
-      launch_digit_pass<8, KeyT>(kin, vin, kout, vout, n_host, d_n, max_pairs, spec, w, stream);
+      launch_digit_pass<8, KeyT>(kin, vin, kout, vout, n_host, d_n, max_pairs, spec, w, stream,
+                                 zp);
     PS_LAUNCH_CHECK();
     lo += width;
     kin = kout;
@@ -1084,10 +1104,14 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
                                                       tile_ranges, stream);
       return PS_OK;
     }
+    ZeroReq z;
+    z.ranges = reinterpret_cast<int2*>(tile_ranges);
+    z.n = tile_ranges ? tiles : 0;
     int st = digit_passes(n_host, d_n, max_pairs, 0, end_bit, w, keys_in, vals_in, keys_out,
-                          vals_out, stream);
+                          vals_out, stream, z);
     if (st != PS_OK || tile_ranges == nullptr) return st;
-    return tile_ranges_scan_launch(n_host, d_n, max_pairs, tiles, keys_out, tile_ranges, stream);
+    return tile_ranges_scan_launch(n_host, d_n, max_pairs, tiles, keys_out, tile_ranges, stream,
+                                   0xffffffffu, nullptr, 0, /*cleared=*/true);
   }
   // Digit passes over the tile bits, then segment ranges, then the segment
   // pass in place in the output.
@@ -1100,12 +1124,22 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
     mask = (uint32_t)segs - 1u;
   }
   int st = PS_OK;
-  if (d_n != nullptr || n_host > 0)
+  // The last digit pass clears the segment ranges and the four list
+  // counters; without digit passes (nothing to sort) the boundary scan's own
+  // clearing launch does.
+  const bool passes = d_n != nullptr || n_host > 0;
+  if (passes) {
+    ZeroReq z;
+    z.ranges = reinterpret_cast<int2*>(ranges);
+    z.n = segs;
+    z.words = w.big_count;
+    z.nwords = 4;
     st = digit_passes(n_host, d_n, max_pairs, 32, end_bit, w, keys_in, vals_in, keys_out,
-                      vals_out, stream);
+                      vals_out, stream, z);
+  }
   if (st != PS_OK) return st;
   st = tile_ranges_scan_launch(n_host, d_n, max_pairs, segs, keys_out, ranges, stream, mask,
-                               w.big_count, 4);
+                               w.big_count, 4, /*cleared=*/passes);
   if (st != PS_OK) return st;
   if (d_n == nullptr && n_host == 0) return PS_OK;
   const int big_cap = (int)big_list_cap((size_t)(max_pairs > 0 ? max_pairs : 1));
@@ -1158,7 +1192,7 @@ int depth_sort_launch(const DepthSortArgs& A, int max_pairs, void* space, size_t
     hipLaunchKernelGGL(ps_dsort_upsweep, dim3(kSortBlocks), dim3(kSortThreads), 0, stream, A, q,
                        w.counts);
     hipLaunchKernelGGL(ps_sort_scan, dim3(256), dim3(kSortThreads), 0, stream, w.counts,
-                       w.offsets, w.totals);
+                       w.offsets, w.totals, ZeroReq());
     hipLaunchKernelGGL(ps_dsort_downsweep, dim3(kSortBlocks), dim3(kSortThreads), 0, stream, A,
                        q, (const uint32_t*)w.offsets, (const uint32_t*)w.totals);
     PS_LAUNCH_CHECK();
