@@ -69,6 +69,9 @@ struct alignas(16) EmitRec {
 constexpr int kStateBase = 0, kStateCount = 1, kStateFlags = 2, kStatePool = 4, kStateWords = 64;
 
 constexpr int kMaskTiles = 32;
+// Longest hit mask a lane of the emit pass walks by itself (see
+// emit_records_block).
+constexpr int kEmitLoopMax = 8;
 constexpr uint32_t kNoPool = 0xffffffffu;
 // Hit pool: each count-pass workgroup owns kPoolPerBlock words at
 // vb * kPoolPerBlock (its Gaussians take them by a block-wide prefix over
@@ -768,19 +771,35 @@ __device__ __forceinline__ void emit_records_block(
   const int off = (int)start + excl;
   const CamParams& cp = F.cp;
   const int w = count > 0 ? (int)(r.x0w >> 24) : 0;
-  {
-    // Rects with a hit mask: the wave's pairs form one stream of tasks, 64
-    // per step, task t of lane o's run writing o's hit k = t - (o's start in
-    // the stream), so consecutive lanes write consecutive pairs whatever the
-    // spread of the counts.  Pair k of a Gaussian is the k-th set bit of its
-    // mask (row-major in its rect) at off + k, as a per-lane loop over the
-    // mask would write it.
-    const int c = (count > 0 && w > 0) ? imin(count, __builtin_popcount(r.mask)) : 0;
+  // Rects with a hit mask: pair k of a Gaussian is the k-th set bit of its
+  // mask (row-major in its rect), written at off + k.  When every such rect
+  // of the wave has at most kEmitLoopMax hits, each lane walks its own mask;
+  // otherwise a lane with a long mask would keep the wave stepping with the
+  // others idle, and the wave's pairs become one stream of tasks, 64 per
+  // step, task t of lane o's run writing o's hit k = t - (o's start in the
+  // stream), so consecutive lanes write consecutive pairs.  Both forms write
+  // the same pairs to the same slots.
+  const int c = (count > 0 && w > 0) ? imin(count, __builtin_popcount(r.mask)) : 0;
+  if (ballot(c > kEmitLoopMax) == 0) {
+    if (c > 0) {
+      const int x0 = (int)(r.x0w & 0xffffffu), y0 = (int)r.y0;
+      // t / w for t < 32 <= 2^10 and w <= 32 as (t * ceil(2^16 / w)) >> 16:
+      // the rounding adds less than 2^-11 < 1/w to t / w, so the floor is
+      // exact.
+      const uint32_t inv_w = (65536u + (uint32_t)w - 1u) / (uint32_t)w;
+      uint32_t m = r.mask;
+      for (int k = 0; k < c; ++k) {
+        const int t = __builtin_ctz(m);
+        m &= m - 1u;
+        const int ty = (int)(((uint32_t)t * inv_w) >> 16), tx = t - ty * w;
+        out.put(off + k, (uint32_t)(F.tile0 + (y0 + ty) * cp.grid_x + (x0 + tx)), r.depth,
+                (uint32_t)g);
+      }
+    }
+  } else {
     const int incl = wave_incl_scan(c, lane);
     const int total_w = bcast(incl, 63);
-    // t / w for t < 32 <= 2^10 and w <= 32 as (t * ceil(2^16 / w)) >> 16:
-    // the rounding adds less than 2^-11 < 1/w to t / w, so the floor is exact.
-    const uint32_t inv_w = w > 0 ? (65536u + (uint32_t)w - 1u) / (uint32_t)w : 0u;
+    const uint32_t inv_w = w > 0 ? (65536u + (uint32_t)w - 1u) / (uint32_t)w : 0u;  // as above
     // Owner data, packed: rect origin (16 bits each: the grid has fewer than
     // 2^16 columns and rows), width and its reciprocal, and where the owner's
     // task 0 lands (off - its start in the stream).
