@@ -425,13 +425,13 @@ def main():
                   "mean_pairs": round(float(lens.mean()), 1), "p99_pairs": int(np.percentile(lens, 99)),
                   "max_pairs": int(lens.max()),
                   # tiles and pairs per depth-sort route: one wave (<= 896 pairs),
-                  # a 256-thread workgroup (<= 4096), a 512-thread one (<= 8192),
+                  # a 256-thread workgroup (<= 4096), a 512-thread one (<= 12288),
                   # a 1024-thread one (<= 32768), global-memory passes (longer)
                   "by_sort_route": {
                       name: {"tiles": int(((lens > lo_) & (lens <= hi_)).sum()),
                              "pairs": int(lens[(lens > lo_) & (lens <= hi_)].sum())}
                       for name, lo_, hi_ in (("wave", 1, 896), ("wg256", 896, 4096),
-                                             ("wg512", 4096, 8192), ("wg1024", 8192, 32768),
+                                             ("wg512", 4096, 12288), ("wg1024", 12288, 32768),
                                              ("global", 32768, 1 << 62))}}
     stage_ms = np.zeros(4)
     reps = max(3, args.steps // 2)

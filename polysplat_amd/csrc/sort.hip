@@ -655,14 +655,19 @@ __global__ __launch_bounds__(kSegWaves * kWave) void ps_seg_small(
 // new order, then the indices the same way.  At the end the values are
 // gathered by index through LDS (every read of the segment's old values
 // completes before the first write) and written with the sorted keys.
-// Longer (or mixed) segments go on to the next list: 4 and 8 waves x 16
-// rounds (4096, 8192 pairs), then 16 waves x 32 rounds (32768, most of a
-// CU's LDS), then ps_seg_big.  Fewer waves per segment mean fewer waves
+// Longer (or mixed) segments go on to the next list: 4 waves x 16 rounds
+// (4096 pairs), 8 waves x 24 rounds (12288), then 16 waves x 32 rounds
+// (32768, most of a CU's LDS), then ps_seg_big.  The 8-wave route's 24
+// rounds (two workgroups per CU) keep the runs just past 8192 pairs out of
+// the one-per-CU 16-wave route (30M Gaussians at 1920x1080, 2828 runs of
+// 8193-9131 pairs: whole sort 1959 -> 1357 us; 32 rounds spill and were
+// slower).  Fewer waves per segment mean fewer waves
 // waiting at each barrier and more segments in flight per CU (4 waves
 // instead of 8 for segments of 897..4096 pairs: 893 -> 563 us for a
 // 30M-Gaussian frame at 3840x2160); a 2-wave route for up to 2048 pairs
 // was measured slower (one more launch, no faster per segment).
-constexpr int kGroupRounds = 16;  // 4 and 8 waves: 4096, 8192 pairs
+constexpr int kGroupRounds = 16;  // 4 waves: 4096 pairs
+constexpr int kMidRounds = 24;    // 8 waves: 12288 pairs
 constexpr int kLargeRounds = 32;  // 16 waves: 32768 pairs
 
 // Appends a segment to the next kernel's list (dropped past its capacity,
@@ -1160,7 +1165,7 @@ int sort_launch(int n_host, const int* d_n, int max_pairs, int end_bit, void* sp
                      w.big_count + 1, big_cap);
   PS_LAUNCH_CHECK();
   // Two 512-thread workgroups per CU.
-  hipLaunchKernelGGL((ps_seg_group<8, kGroupRounds, 4>), dim3(1024), dim3(8 * kWave), 0, stream,
+  hipLaunchKernelGGL((ps_seg_group<8, kMidRounds, 4>), dim3(1024), dim3(8 * kWave), 0, stream,
                      keys_out, vals_out, rg, w.big_list2, w.big_count + 1, big_cap, w.big_list3,
                      w.big_count + 2, big_cap);
   PS_LAUNCH_CHECK();

@@ -69,13 +69,13 @@ def test_sort_segments_around_the_lds_limits(emu, mode, monkeypatch):
 
 @pytest.mark.parametrize("threads", [1, 8])
 def test_sort_long_segments_with_full_depth_words(emu, threads, monkeypatch):
-    # Segments for each workgroup route (up to 4096, 8192 and 32768 pairs)
+    # Segments for each workgroup route (up to 4096, 12288 and 32768 pairs)
     # and past them, with depths over the full 32 bits (every digit
     # pass runs) and many ties at the segment's largest depth (the padding
     # value).
     monkeypatch.setenv("EMU_THREADS", str(threads))
-    counts = [1000, 4096, 4097, 5000, 8192, 8193, 16383, 16384, 16385, 20000, 32767, 32768,
-              32769, 40000]
+    counts = [1000, 4096, 4097, 5000, 8192, 8193, 12288, 12289, 16384, 16385, 20000, 32767,
+              32768, 32769, 40000]
     rng = np.random.default_rng(123)
     t = np.repeat(np.arange(len(counts), dtype=np.uint64), counts)
     rng.shuffle(t)

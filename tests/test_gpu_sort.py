@@ -138,13 +138,13 @@ def test_sort_segment_sizes_around_the_lds_limit(gpu, mode, monkeypatch):
 
 @pytest.mark.parametrize("mode", ["segmented", "lsd"])
 def test_sort_long_segments_with_full_depth_words(gpu, mode, monkeypatch):
-    # Segments for each workgroup LDS route (up to 4096, 8192 and 32768
+    # Segments for each workgroup LDS route (up to 4096, 12288 and 32768
     # pairs) and past them, depths over all 32 bits (every digit pass
     # runs), with ties at the segment's largest depth (the padding value).
     if mode == "lsd":
         monkeypatch.setenv("POLYSPLAT_SORT", "lsd")
-    counts = [1000, 4096, 4097, 5000, 8192, 8193, 16383, 16384, 16385, 20000, 32767, 32768,
-              32769, 40000, 100000]
+    counts = [1000, 4096, 4097, 5000, 8192, 8193, 12288, 12289, 16384, 16385, 20000, 32767,
+              32768, 32769, 40000, 100000]
     rng = np.random.default_rng(123)
     t = np.repeat(np.arange(len(counts), dtype=np.uint64), counts)
     rng.shuffle(t)
