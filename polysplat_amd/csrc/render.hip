@@ -138,10 +138,16 @@ __device__ __forceinline__ uint8_t encode(float x) {
 // a negative semidefinite quadratic form, and q(d) <= lmax |d|^2 for lmax the
 // eigenvalue of q nearest zero (taken with a relative rounding allowance), so
 // w + lmax * dist^2(centre, quadrant) bounds the power over the quadrant.
-// A Gaussian left out of a quadrant would have added less than 2^-32 of
-// colour and left T unchanged in FP32.  The neutral padding Gaussian (power
+// A Gaussian left out of a quadrant has alpha below 2^-24 at each of its
+// pixels: it would have added less than 2^-24 T of colour (T <= 1), under a
+// ten-thousandth of the 1/255 output step, and changed T by at most one unit
+// in its last place.  The output may differ from blending every Gaussian by
+// one level where its scaled value lies that close to an integer (measured:
+// 11 of 64M pixels of the 56M-Gaussian city, none at 6M Gaussians
+// 3840x2160); against 2^-32 the tighter masks make the sparse path 3-5%
+// faster on frames of small splats.  The neutral padding Gaussian (power
 // -inf) has no bits; a NaN keeps every bit.
-constexpr float kQuadSkipPower = -32.0f;
+constexpr float kQuadSkipPower = -24.0f;
 
 __device__ __forceinline__ float dist_to_span(float v, float lo, float hi) {
   return v < lo ? lo - v : (v > hi ? v - hi : 0.0f);
@@ -387,7 +393,7 @@ __device__ __forceinline__ void render_tile(
 // frames with fewer tiles than waves in flight, so that each tile's work is
 // split four ways (a frame's longest tile otherwise sets the render time).
 // Per pixel the same operations in the same order as render_tile; the
-// Gaussians left out are those whose alpha stays below 2^-32 in this
+// Gaussians left out are those whose alpha stays below 2^-24 in this
 // quadrant, and the quadrant stops on its own pixels' transmittance, where a
 // tile waits for all four (pixels past saturation add < 1e-4 of colour).
 template <int PREC, bool GATHERED, int CHECK, bool DT>
