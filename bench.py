@@ -573,9 +573,15 @@ def main():
         if "valu_insts_per_launch" in t:
             insts = float(t["valu_insts_per_launch"])
             rate = insts / (rend_ms * 1e-3) / 1e9
+            # The instruction count comes from a separate profiled run, and a
+            # launch timed here can run at a clock above the nominal one, so
+            # the ratio can come out a little above 1: it is reported capped
+            # at 1 (issue-bound), with the uncapped ratio beside it.
             roof["valu"] = {"insts_per_launch": int(insts), "achieved_ginst_s": round(rate, 1),
                             "peak_ginst_s": VALU_PEAK_GINST,
-                            "frac": round(rate / VALU_PEAK_GINST, 4)}
+                            "frac": round(min(rate / VALU_PEAK_GINST, 1.0), 4)}
+            if rate > VALU_PEAK_GINST:
+                roof["valu"]["ratio_to_nominal_uncapped"] = round(rate / VALU_PEAK_GINST, 4)
             if shaded:
                 # wave-level instructions per (tile, Gaussian) pair the tiles
                 # actually blend (the oracle's stop points at cadence 8)

@@ -94,6 +94,35 @@ def test_sort_and_ranges_bit_exact(case):
     np.testing.assert_array_equal(g.ranges_scan(), O.tile_ranges(ref["keys_sorted"], T, 1))
 
 
+def test_ranges_scan_any_alignment_and_count(case):
+    # The boundary scan reads two keys per 16-byte load when the keys are
+    # 16-byte aligned and one key at a time otherwise: both forms, with the
+    # pair count from the host and from device memory, even and odd.
+    name, scene, spec, ref = case
+    g = Gpu(scene, spec, capacity=_cap(ref))
+    g.preprocess()
+    g.sort()
+    M, T = g.M, g.tiles
+    say("ranges alignment", name, M)
+    buf = torch.zeros(M + 2, dtype=torch.int64, device=g.ks.device)
+    buf[1:M + 1] = g.ks[:M]
+    views = {"aligned": g.ks, "unaligned": buf[1:]}
+    assert views["aligned"].data_ptr() % 16 == 0 and views["unaligned"].data_ptr() % 16 == 8
+    for n in sorted({M, max(M - 1, 0)}):
+        want = O.tile_ranges(ref["keys_sorted"][:n], T, 1)
+        d_n = torch.tensor([n], dtype=torch.int32, device=g.ks.device)
+        for kind, keys in views.items():
+            for dev_count in (False, True):
+                out = torch.full((T, 2), -5, dtype=torch.int32, device=g.ks.device)
+                capi.check(capi.lib().ps_precompute_tile_ranges_scan(
+                    0 if dev_count else n, capi.ptr(d_n) if dev_count else None, M,
+                    spec.width, spec.height, 16, 16, capi.ptr(keys), capi.ptr(out),
+                    capi.stream()))
+                torch.cuda.synchronize()
+                np.testing.assert_array_equal(out.cpu().numpy(), want,
+                                              err_msg=f"{kind} n={n} device count={dev_count}")
+
+
 @pytest.mark.parametrize("mode", ["plain", "persistent", "persistent_tiles", "fused", "dt"])
 def test_render_fp32_parity(case, mode, monkeypatch):
     # These frames have fewer tiles than the persistent renderer's waves, so
