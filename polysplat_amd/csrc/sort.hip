@@ -1066,7 +1066,9 @@ int digit_passes(int n_host, const int* d_n, int max_pairs, int bit0, int bit1, 
     const bool to_out = ((passes - 1 - p) % 2) == 0;
     KeyT* kout = to_out ? keys_out : alt_keys;
     uint32_t* vout = to_out ? vals_out : w.alt_vals;
-    const int width = (bit1 - lo + (passes - p) - 1) / (passes - p);  // <= 9
+    // The narrower digit first (C3's 15 tile bits as 7 + 8): measured 1-2%
+    // faster whole sorts than the wider one first on every odd split.
+    const int width = (bit1 - lo) / (passes - p);  // <= 9
     const int spec = digit_spec(lo, width);
     const ZeroReq zp = (p == passes - 1) ? z : ZeroReq();
     if (width > 8)
