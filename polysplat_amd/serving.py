@@ -20,9 +20,11 @@ the loop keeps several groups of frames in flight on separate HIP streams:
   count (read back asynchronously) and re-queued at the front with the
   measured count.
 
-Latency is arrival -> image ready: the completion of a batch is the GPU
-timestamp of an event recorded after its render, mapped onto the host
-clock through a start event recorded on an idle device.
+Latency is arrival -> image ready: a frame's completion is the GPU
+timestamp of an event recorded after its own render (the pipelined
+executor records one per frame, so a frame is not charged for the frames
+queued behind it in its group), mapped onto the host clock through a start
+event recorded on an idle device.
 
 The scheduling policy (``Scheduler``) is independent of the GPU, so it is
 tested on the CPU with a simulated executor.
@@ -213,6 +215,8 @@ class InFlight:
     slot: int
     done: object = None            # torch.cuda.Event after the group's last frame
     host_counts: object = None     # pinned int32 [frames]: pairs each frame needed
+    frame_events: list = field(default_factory=list)  # one event after each frame
+    frame_ms: list = field(default_factory=list)      # each frame's completion, ms
     buffers: list = field(default_factory=list)
     outputs: list = field(default_factory=list)
 
@@ -566,9 +570,15 @@ class PipelinedExecutor(SequentialExecutor):
             tickets.append(t)
             fl.outputs.append(out)
             fl.capacities[i] = self.cap  # the room the frame had
+            # A request's image is ready when its own frame's kernels are
+            # done, not when the rest of its group is.
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record(self.stream)
+            fl.frame_events.append(ev)
         fl.host_counts = tickets
-        fl.done = torch.cuda.Event(enable_timing=True)
-        fl.done.record(self.stream)
+        fl.done = fl.frame_events[-1] if fl.frame_events else torch.cuda.Event(enable_timing=True)
+        if not fl.frame_events:
+            fl.done.record(self.stream)
         self.inflight.append(fl)
         return fl
 
@@ -585,6 +595,7 @@ class PipelinedExecutor(SequentialExecutor):
                 if overflowed:
                     self.grows += 1
                     self._ensure_keys(int(n * self.margin) + 4096)
+            fl.frame_ms = [self.start.elapsed_time(ev) for ev in fl.frame_events]
             out.append((fl, self.start.elapsed_time(fl.done), counts))
         return out
 
@@ -603,7 +614,7 @@ class ServingLoop:
         self.batches: list = []  # (bid, [rid...], launch_s, done_ms)
         self.overflows = 0
 
-    def run(self, requests, spin_s: float = 2e-4):
+    def run(self, requests, spin_s: float = 2e-5):
         reqs = sorted(requests, key=lambda r: r.arrival_s)
         pending = collections.deque(reqs)
         done: list = []
@@ -618,13 +629,15 @@ class ServingLoop:
                 progressed = True
                 self.sched.release(fl.nbytes)
                 retry = []
-                for r, c, cap, img in zip(fl.reqs, counts, fl.capacities, fl.outputs):
+                frame_ms = getattr(fl, "frame_ms", None) or [done_ms] * len(fl.reqs)
+                for r, c, cap, img, f_ms in zip(fl.reqs, counts, fl.capacities, fl.outputs,
+                                                frame_ms):
                     self.sched.est.observe(r.key, pairs=c)
                     if c > cap:
                         retry.append(r)  # not rendered: its pairs did not fit
                         continue
                     r.pairs = c
-                    r.done_ms = done_ms
+                    r.done_ms = f_ms
                     r.batch = fl.bid
                     if self.keep_outputs:
                         self.outputs[r.rid] = img
