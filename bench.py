@@ -66,6 +66,11 @@ def parse():
     ap.add_argument("--serve-tight-budget-gib", type=float, default=1.5)
     ap.add_argument("--serve-rate", type=float, default=240.0)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving stream")
+    ap.add_argument("--serve-order", choices=("fifo", "slack"), default="slack",
+                    help="queue order of the serving scheduler (slack: estimated cost minus "
+                         "time waited, smallest first)")
+    ap.add_argument("--serve-slots", type=int, default=3,
+                    help="groups of frames the serving executor keeps enqueued on the GPU")
     ap.add_argument("--serve-graphs", choices=("on", "off"), default="on",
                     help="replay each (scene, resolution) pipeline from a captured HIP graph")
     ap.add_argument("--tile-orders", action="store_true",
@@ -105,7 +110,7 @@ def serve(args):
     sc = SV.make_resident_scenes(SV.SCENE_SIZES, device="cuda:0")
     torch.cuda.synchronize()
     log(f"serving: {len(sc)} resident scenes generated in {time.time() - t:.1f}s")
-    ex = SV.make_executor(sc, graphs=args.serve_graphs == "on")
+    ex = SV.make_executor(sc, graphs=args.serve_graphs == "on", slots=args.serve_slots)
     est = SV.Estimates()
     t = time.time()
     SV.calibrate(ex, est, len(sc))
@@ -113,7 +118,8 @@ def serve(args):
     for k in sorted(est.pairs):
         log(f"  scene {SV.SCENE_SIZES[k[0]]:>9d} {k[1]}x{k[2]}: pairs {est.pairs[k]:>10d}, "
             f"{est.ms[k]:.3f} ms alone")
-    sched = SV.Scheduler(SV.SCENE_SIZES, 48 << 30, est, max_frames=8, max_batch_ms=6.0)
+    sched = SV.Scheduler(SV.SCENE_SIZES, 48 << 30, est, max_frames=8, max_batch_ms=6.0,
+                         order=args.serve_order)
     reqs = SV.poisson_requests(args.serve_requests, args.serve_rate, len(sc))
     loop = SV.ServingLoop(sched, ex)
     t = time.perf_counter()
@@ -123,7 +129,8 @@ def serve(args):
     res = SV.summarize(done, sched.rejected, wall, batches=loop.batches)
     res["overflow_retries"] = loop.overflows + ex.grows
     res["memory_waits"] = sched.memory_waits
-    res["executor"] = (f"{type(ex).__name__}: one stream, frames in arrival order, enqueued "
+    res["queue_order"] = args.serve_order
+    res["executor"] = (f"{type(ex).__name__}: one stream, frames in scheduler order, enqueued "
                        "back to back with up to "
                        f"{ex.slots_n} groups in flight; pair counts read back from pinned "
                        "slots after each frame (no per-frame host wait)" +
@@ -143,7 +150,8 @@ def serve(args):
     res["by_resolution"] = by_res
     if args.serve_tight_requests > 0:
         budget = int(args.serve_tight_budget_gib * (1 << 30))
-        sched2 = SV.Scheduler(SV.SCENE_SIZES, budget, est, max_frames=8, max_batch_ms=6.0)
+        sched2 = SV.Scheduler(SV.SCENE_SIZES, budget, est, max_frames=8, max_batch_ms=6.0,
+                              order=args.serve_order)
         reqs2 = SV.poisson_requests(args.serve_tight_requests, args.serve_rate, len(sc), seed=8)
         loop2 = SV.ServingLoop(sched2, ex)
         t = time.perf_counter()

@@ -145,15 +145,26 @@ class Scheduler:
     """Admission and batch formation (no GPU).
 
     ``budget`` bounds the bytes of all batches in flight.  ``form_batch``
-    takes waiting requests in arrival order while the batch stays within
+    takes waiting requests in queue order while the batch stays within
     ``max_frames`` and ``max_batch_ms`` of estimated GPU time (a single
     request is always allowed) and the budget has room; the first request
-    that does not fit ends the batch, so arrival order is kept.  A request
-    that could never fit the whole budget is rejected.
+    that does not fit ends the batch.  A request that could never fit the
+    whole budget is rejected.
+
+    Queue order: ``"fifo"`` is arrival order; ``"slack"`` takes the waiting
+    requests by increasing estimated cost minus ``slack_k`` times the time
+    they have waited (ms), so short frames go ahead of long ones that have
+    not waited long.  Requests put back after an overflow stay first either
+    way.
     """
 
     def __init__(self, scene_sizes, budget: int, est: Estimates, max_frames: int = 8,
-                 max_batch_ms: float = 6.0, feat_bytes: int = 24):
+                 max_batch_ms: float = 6.0, feat_bytes: int = 24, order: str = "fifo",
+                 slack_k: float = 1.0):
+        if order not in ("fifo", "slack"):
+            raise ValueError(f"unknown queue order {order!r}")
+        self.order = order
+        self.slack_k = float(slack_k)
         self.P = list(scene_sizes)
         self.budget = int(budget)
         self.est = est
@@ -180,8 +191,19 @@ class Scheduler:
             r.attempts += 1
             self.waiting.appendleft(r)
 
-    def form_batch(self):
-        """Returns (requests, bytes) or ([], 0) when nothing can start now."""
+    def _reorder(self, now_s: float):
+        retried = [r for r in self.waiting if r.attempts > 0]
+        fresh = [r for r in self.waiting if r.attempts == 0]
+        fresh.sort(key=lambda r: (self.est.cost_ms(r.key)
+                                  - self.slack_k * 1e3 * (now_s - r.arrival_s), r.arrival_s))
+        self.waiting = collections.deque(retried + fresh)
+
+    def form_batch(self, now_s: float | None = None):
+        """Returns (requests, bytes) or ([], 0) when nothing can start now.
+        ``now_s`` (the host clock of the arrivals) is needed by the slack
+        order."""
+        if self.order == "slack" and now_s is not None and len(self.waiting) > 1:
+            self._reorder(now_s)
         batch, nbytes, ms = [], SORT_FIXED_BYTES, 0.0
         while self.waiting and len(batch) < self.max_frames:
             r = self.waiting[0]
@@ -651,7 +673,7 @@ class ServingLoop:
                 slot = self.ex.free_slot()
                 if slot is None:
                     break
-                batch, nbytes = self.sched.form_batch()
+                batch, nbytes = self.sched.form_batch(self.ex.now_s())
                 if not batch:
                     break
                 caps = [self.sched.est.capacity(r.key) for r in batch]

@@ -135,6 +135,37 @@ def test_batches_are_capped_by_frames_and_estimated_time():
     assert [len(sched.form_batch()[0]) for _ in range(4)] == [1, 1, 1, 1]
 
 
+def test_slack_order_puts_short_frames_first_until_long_ones_have_waited():
+    # costs: scene 1 at 128x96 is 3.0 ms, scene 0 at 64x64 is 0.5 ms
+    sizes, res, cost, pairs, est, sched = _setup(max_frames=1)
+    big = SV.Request(0, 1, *res[1], 0.0, arrival_s=0.0)
+    small = SV.Request(1, 0, *res[0], 0.0, arrival_s=0.001)
+    for order, now, first in (("fifo", 0.002, 0),
+                              ("slack", 0.002, 1),    # 3.0 - 2 > 0.5 - 1
+                              ("slack", 0.010, 1)):   # 3.0 - 10 > 0.5 - 9
+        sizes, res, cost, pairs, est, sched = _setup(max_frames=1)
+        sched.order = order
+        sched.arrive(big)
+        sched.arrive(small)
+        assert sched.form_batch(now)[0][0].rid == first, (order, now)
+    # the long frame goes first once it has waited its extra cost longer
+    sizes, res, cost, pairs, est, sched = _setup(max_frames=1)
+    sched.order = "slack"
+    late_small = SV.Request(1, 0, *res[0], 0.0, arrival_s=0.005)
+    sched.arrive(big)
+    sched.arrive(late_small)
+    assert sched.form_batch(0.006)[0][0].rid == 0  # 3.0 - 6 < 0.5 - 1
+    # a request put back after an overflow stays first
+    sizes, res, cost, pairs, est, sched = _setup(max_frames=1)
+    sched.order = "slack"
+    sched.arrive(small)
+    retried = SV.Request(2, 1, *res[1], 0.0, arrival_s=0.0015)
+    sched.requeue_front([retried])
+    assert sched.form_batch(0.002)[0][0].rid == 2
+    with pytest.raises(ValueError):
+        SV.Scheduler(sizes, 1 << 30, est, order="lifo")
+
+
 def test_loop_completes_everything_in_order_with_latencies():
     sizes, res, cost, pairs, est, sched = _setup()
     reqs = SV.poisson_requests(300, 400.0, 2, resolutions=res, seed=5)
