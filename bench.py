@@ -98,7 +98,8 @@ def serve(args):
     """Serving stream: 4 resident scenes (1M, 3M, 10M, 30M Gaussians, FP16 SH
     and features), Poisson requests at --serve-rate with orbit poses and
     resolutions drawn from 1280x720 .. 7680x4320, admitted by buffer memory,
-    run in arrival order on one stream with the next frames enqueued while
+    run in --serve-order (default: slack, the estimated frame cost minus the
+    time waited) on one stream with the next frames enqueued while
     the host reads back the last ones' pair counts.  A second, shorter
     stream runs under a memory budget too small for every frame, so that
     admission rejects and defers."""
